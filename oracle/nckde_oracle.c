@@ -1,0 +1,2043 @@
+/*
+ * nckde_oracle.c -- FP64 CPU restatement of NumCosmo's kernel-density engine.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP engine.  Loaded
+ * only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg;
+ * never by the product library.
+ *
+ * Built with -ffp-contract=off so that every sum and product rounds exactly
+ * as written (the reference is built without FMA contraction on x86-64).
+ * Reference third-party arithmetic (GSL 2.8, OpenBLAS 0.3.34, LAPACK 3.11)
+ * is restated from the published algorithms: dpotrf as the textbook
+ * dot-product Cholesky, dsyevr as cyclic Jacobi, dsysv as Gaussian
+ * elimination with partial pivoting, dgels as Householder QR.
+ *
+ * Parity pinning (what fixes each function, see tests/test_oracle.py):
+ *   kernels, lnnorm, sum0/sum1 ... closed forms (reference KATs
+ *       tests/c/ncm/stats/test_ncm_stats_dist_kernel.c:183-420)
+ *   Cholesky/lndet ............... numpy.linalg.cholesky / slogdet
+ *   nearPD ....................... Higham invariants (SPD, diagonal kept)
+ *   Welford covariance ........... numpy.cov (test_ncm_stats_vec.c:320-353)
+ *   kNN .......................... brute-force sort on (dist2, index)
+ *   NNLS ......................... KKT conditions + scipy.optimize.nnls
+ *   eval_m2lnp / eval ............ brute-force logsumexp over the factors;
+ *                                  Gaussian KDE with a fixed covariance
+ *                                  reduces to a Gaussian mixture
+ *   kernel_choose ................ serial prefix sum + bisection (exact)
+ *   nearPD iterate order, NNLS active-set trajectory and BLAS summation
+ *   order are tolerance-only ("parity unpinned" beyond tolerance).
+ */
+#include "nckde_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OK_LN2PI 1.8378770664093454835606594728112353 /* ncm_c.h:434-437 */
+#define OK_LNPI  1.1447298858494001741434273513530587 /* ncm_c.h:440-443 */
+#define OK_LOG_DBL_EPSILON (-3.6043653389117154e+01)  /* GSL_LOG_DBL_EPSILON */
+
+static __thread char ok_errbuf[512];
+
+const char *
+ok_last_error (void)
+{
+  return ok_errbuf;
+}
+
+static void
+ok_set_error (const char *msg)
+{
+  snprintf (ok_errbuf, sizeof (ok_errbuf), "%s", msg);
+}
+
+/* ======================================================================= */
+/* Kernel family                                                           */
+/* ======================================================================= */
+
+/* ncm_stats_dist_kernel_gauss.c:193-199 and ncm_stats_dist_kernel_st.c:225-239 */
+double
+ok_kernel_rot_bandwidth (int kernel, double nu, int d_, double n)
+{
+  const unsigned d = (unsigned) d_;
+
+  if (kernel == OK_GAUSS)
+    return pow (4.0 / (n * (d + 2.0)), 1.0 / (d + 4.0));
+
+  {
+    const double nu3 = (nu >= 3.0) ? nu : 3.0;
+
+    return pow (
+      16.0 * ((nu3 - 2) * (nu3 - 2)) * (1.0 + d + nu3) * (3.0 + d + nu3) /
+      ((2.0 + d) * (d + nu3) * (2.0 + d + nu3) * (d + 2.0 * nu3) * (2.0 + d + 2.0 * nu3) * n),
+      1.0 / (d + 4.0));
+  }
+}
+
+/* ncm_matrix.c:1157-1186 (frexp-guarded log-determinant of U^T U) */
+double
+ok_cholesky_lndet (int n, const double *U, int lda)
+{
+  const double lb = 1.0e-200;
+  const double ub = 1.0e+200;
+  double detL     = 1.0;
+  long exponent   = 0;
+  int i;
+
+  for (i = 0; i < n; i++)
+  {
+    const double Lii   = fabs (U[(size_t) i * lda + i]);
+    const double ndetL = detL * Lii;
+
+    if ((ndetL < lb) || (ndetL > ub))
+    {
+      int exponent_i = 0;
+
+      detL      = frexp (ndetL, &exponent_i);
+      exponent += exponent_i;
+    }
+    else
+    {
+      detL = ndetL;
+    }
+  }
+
+  return 2.0 * (log (detL) + exponent * M_LN2);
+}
+
+/* ncm_stats_dist_kernel_gauss.c:201-207, ncm_stats_dist_kernel_st.c:241-253 */
+double
+ok_kernel_lnnorm (int kernel, double nu, int d_, const double *U)
+{
+  const unsigned d = (unsigned) d_;
+
+  if (kernel == OK_GAUSS)
+    return 0.5 * (d * OK_LN2PI + ok_cholesky_lndet (d_, U, d_));
+
+  {
+    const double lg_lnnorm   = lgamma (nu / 2.0) - lgamma ((nu + d) / 2.0);
+    const double chol_lnnorm = 0.5 * ok_cholesky_lndet (d_, U, d_);
+    const double nc_lnnorm   = (d / 2.0) * (OK_LNPI + log (nu));
+
+    return lg_lnnorm + nc_lnnorm + chol_lnnorm;
+  }
+}
+
+/* ncm_stats_dist_kernel_gauss.c:209-213, ncm_stats_dist_kernel_st.c:255-263 */
+double
+ok_kernel_eval_unnorm (int kernel, double nu, int d_, double chi2)
+{
+  const unsigned d = (unsigned) d_;
+
+  if (kernel == OK_GAUSS)
+    return exp (-0.5 * chi2);
+
+  return pow (1.0 + chi2 / nu, -0.5 * (nu + d));
+}
+
+static inline double
+ok_lnt0 (int kernel, double kappa, double nu, double chi2_i, double lnu_i, double w_i)
+{
+  if (kernel == OK_GAUSS)
+    return -0.5 * chi2_i - lnu_i + log (w_i);     /* kernel_gauss.c:269 */
+
+  return kappa * log1p (chi2_i / nu) - lnu_i + log (w_i); /* kernel_st.c:320 */
+}
+
+/* ncm_stats_dist_kernel_gauss.c:246-290, ncm_stats_dist_kernel_st.c:295-341:
+ * gamma = first maximum of lnt_i, lambda = sum over the other terms of
+ * exp(lnt_i - gamma), in index order. */
+void
+ok_kernel_sum0 (int kernel, double nu, int d_, int n, const double *chi2,
+                const double *w, const double *lnnorms, double *gamma, double *lambda)
+{
+  const unsigned d    = (unsigned) d_;
+  const double kappa  = -0.5 * (nu + d);
+  double *lnK         = malloc (sizeof (double) * (size_t) (n > 0 ? n : 1));
+  double lnt_max      = -INFINITY;
+  int i, i_max = 0;
+
+  for (i = 0; i < n; i++)
+  {
+    const double lnt_i = ok_lnt0 (kernel, kappa, nu, chi2[i], lnnorms[i], w[i]);
+
+    if (lnt_i > lnt_max)
+    {
+      i_max   = i;
+      lnt_max = lnt_i;
+    }
+
+    lnK[i] = lnt_i;
+  }
+
+  lambda[0] = 0.0;
+
+  for (i = 0; i < i_max; i++)
+    lambda[0] += exp (lnK[i] - lnt_max);
+
+  for (i = i_max + 1; i < n; i++)
+    lambda[0] += exp (lnK[i] - lnt_max);
+
+  gamma[0] = lnt_max;
+  free (lnK);
+}
+
+/* ncm_stats_dist_kernel_gauss.c:292-333, ncm_stats_dist_kernel_st.c:343-386 */
+void
+ok_kernel_sum1 (int kernel, double nu, int d_, int n, const double *chi2,
+                const double *w, double lnnorm, double *gamma, double *lambda)
+{
+  const unsigned d   = (unsigned) d_;
+  const double kappa = -0.5 * (nu + d);
+  double *lnK        = malloc (sizeof (double) * (size_t) (n > 0 ? n : 1));
+  double lnt_max     = -INFINITY;
+  int i, i_max = 0;
+
+  for (i = 0; i < n; i++)
+  {
+    double lnt_i;
+
+    if (kernel == OK_GAUSS)
+      lnt_i = -0.5 * chi2[i] + log (w[i]);
+    else
+      lnt_i = kappa * log1p (chi2[i] / nu) + log (w[i]);
+
+    if (lnt_i > lnt_max)
+    {
+      i_max   = i;
+      lnt_max = lnt_i;
+    }
+
+    lnK[i] = lnt_i;
+  }
+
+  lambda[0] = 0.0;
+
+  for (i = 0; i < i_max; i++)
+    lambda[0] += exp (lnK[i] - lnt_max);
+
+  for (i = i_max + 1; i < n; i++)
+    lambda[0] += exp (lnK[i] - lnt_max);
+
+  gamma[0] = lnt_max - lnnorm;
+  free (lnK);
+}
+
+/* ======================================================================= */
+/* Dense algebra                                                           */
+/* ======================================================================= */
+
+/*
+ * ncm_matrix.c:1123-1129 -> ncm_lapack.c:215-237: dpotrf on the row-major
+ * upper triangle (column-major 'L'), so A = U^T U.  Only the upper triangle
+ * is read and written; the strict lower triangle is left as it was, as
+ * LAPACK does.  Returns 0, or j+1 when the leading minor of order j+1 is not
+ * positive definite (the LAPACK info convention).
+ */
+int
+ok_cholesky_upper (int d, double *A, int lda)
+{
+  int j, i, k;
+
+  for (j = 0; j < d; j++)
+  {
+    double s = A[(size_t) j * lda + j];
+
+    for (k = 0; k < j; k++)
+    {
+      const double ukj = A[(size_t) k * lda + j];
+
+      s = s - ukj * ukj;
+    }
+
+    if (!(s > 0.0))
+    {
+      A[(size_t) j * lda + j] = s;
+      return j + 1;
+    }
+
+    s                       = sqrt (s);
+    A[(size_t) j * lda + j] = s;
+
+    for (i = j + 1; i < d; i++)
+    {
+      double t = A[(size_t) j * lda + i];
+
+      for (k = 0; k < j; k++)
+        t = t - A[(size_t) k * lda + j] * A[(size_t) k * lda + i];
+
+      A[(size_t) j * lda + i] = t / s;
+    }
+  }
+
+  return 0;
+}
+
+/*
+ * Symmetric eigen-decomposition (stand-in for LAPACK dsyevr, range 'A').
+ * Cyclic Jacobi on the symmetric matrix whose upper triangle is A.
+ * Output: eigenvalues ascending in eva[d]; eigenvector i in row i of eve.
+ */
+int
+ok_jacobi_eigen (int d, const double *A, double *eva, double *eve)
+{
+  double *a = malloc (sizeof (double) * d * d);
+  double *v = malloc (sizeof (double) * d * d);
+  int *ord  = malloc (sizeof (int) * d);
+  int i, j, p, q, sweep;
+
+  for (i = 0; i < d; i++)
+    for (j = 0; j < d; j++)
+    {
+      a[i * d + j] = (j >= i) ? A[i * d + j] : A[j * d + i];
+      v[i * d + j] = (i == j) ? 1.0 : 0.0;
+    }
+
+  for (sweep = 0; sweep < 100; sweep++)
+  {
+    double off = 0.0;
+
+    for (p = 0; p < d; p++)
+      for (q = p + 1; q < d; q++)
+        off += a[p * d + q] * a[p * d + q];
+
+    if (off == 0.0)
+      break;
+
+    for (p = 0; p < d; p++)
+    {
+      for (q = p + 1; q < d; q++)
+      {
+        const double apq = a[p * d + q];
+        const double app = a[p * d + p];
+        const double aqq = a[q * d + q];
+
+        if (fabs (apq) <= DBL_EPSILON * 1.0e-3 * sqrt (fabs (app * aqq)) || apq == 0.0)
+        {
+          a[p * d + q] = 0.0;
+          a[q * d + p] = 0.0;
+          continue;
+        }
+
+        {
+          const double theta = (aqq - app) / (2.0 * apq);
+          const double t     = (theta >= 0.0 ? 1.0 : -1.0) / (fabs (theta) + sqrt (theta * theta + 1.0));
+          const double c     = 1.0 / sqrt (t * t + 1.0);
+          const double s     = t * c;
+          int k;
+
+          for (k = 0; k < d; k++)
+          {
+            const double akp = a[k * d + p];
+            const double akq = a[k * d + q];
+
+            a[k * d + p] = c * akp - s * akq;
+            a[k * d + q] = s * akp + c * akq;
+          }
+
+          for (k = 0; k < d; k++)
+          {
+            const double apk = a[p * d + k];
+            const double aqk = a[q * d + k];
+
+            a[p * d + k] = c * apk - s * aqk;
+            a[q * d + k] = s * apk + c * aqk;
+          }
+
+          for (k = 0; k < d; k++)
+          {
+            const double vkp = v[k * d + p];
+            const double vkq = v[k * d + q];
+
+            v[k * d + p] = c * vkp - s * vkq;
+            v[k * d + q] = s * vkp + c * vkq;
+          }
+        }
+      }
+    }
+  }
+
+  for (i = 0; i < d; i++)
+    ord[i] = i;
+
+  /* insertion sort, ascending eigenvalue */
+  for (i = 1; i < d; i++)
+  {
+    const int oi = ord[i];
+
+    j = i - 1;
+
+    while (j >= 0 && a[ord[j] * d + ord[j]] > a[oi * d + oi])
+    {
+      ord[j + 1] = ord[j];
+      j--;
+    }
+
+    ord[j + 1] = oi;
+  }
+
+  for (i = 0; i < d; i++)
+  {
+    eva[i] = a[ord[i] * d + ord[i]];
+
+    for (j = 0; j < d; j++)
+      eve[i * d + j] = v[j * d + ord[i]];
+  }
+
+  free (a);
+  free (v);
+  free (ord);
+
+  return 0;
+}
+
+/*
+ * ncm_matrix.c:1247-1344: nearest positive-definite matrix (Higham 2002,
+ * alternating projections with Dykstra's correction) on the upper triangle;
+ * negative eigenvalues are raised to min_pos_ev * DBL_EPSILON; the diagonal
+ * is restored after each projection; stops once the Cholesky succeeds or
+ * after maxiter + 2 projections.  On exit A holds the Cholesky factor (upper
+ * triangle) and the return value is the last Cholesky info.
+ */
+int
+ok_nearPD (int n, double *cm, int maxiter)
+{
+  double *eva  = malloc (sizeof (double) * n);
+  double *diag = malloc (sizeof (double) * n);
+  double *eve  = malloc (sizeof (double) * n * n);
+  double *D_S  = calloc ((size_t) n * n, sizeof (double));
+  double *R    = malloc (sizeof (double) * n * n);
+  int ret      = 0;
+  int iter     = 0;
+  int i, j, k;
+
+  for (i = 0; i < n; i++)
+    diag[i] = cm[i * n + i];
+
+  while (1)
+  {
+    double min_pos_ev = INFINITY;
+
+    for (i = 0; i < n * n; i++)
+      cm[i] = cm[i] - D_S[i];
+
+    memcpy (R, cm, sizeof (double) * n * n);
+
+    ok_jacobi_eigen (n, cm, eva, eve);
+
+    for (i = 0; i < n; i++)
+      if (eva[i] > 0.0)
+        min_pos_ev = fmin (eva[i], min_pos_ev);
+
+    for (i = 0; i < n; i++)
+    {
+      const double s = (eva[i] < 0.0) ? sqrt (min_pos_ev * DBL_EPSILON) : sqrt (eva[i]);
+
+      if (eva[i] < 0.0)
+        eva[i] = min_pos_ev * DBL_EPSILON;
+
+      for (j = 0; j < n; j++)
+        eve[i * n + j] = s * eve[i * n + j];
+    }
+
+    /* dsyrk, row-major, upper, trans: C = eve^T eve on the upper triangle */
+    for (i = 0; i < n; i++)
+      for (j = i; j < n; j++)
+      {
+        double s = 0.0;
+
+        for (k = 0; k < n; k++)
+          s = s + eve[k * n + i] * eve[k * n + j];
+
+        cm[i * n + j] = s;
+      }
+
+    for (i = 0; i < n * n; i++)
+      D_S[i] = cm[i] - R[i];
+
+    for (i = 0; i < n; i++)
+      cm[i * n + i] = diag[i];
+
+    memcpy (R, cm, sizeof (double) * n * n);
+
+    if ((ret = ok_cholesky_upper (n, R, n)) == 0)
+      break;
+
+    if (iter > maxiter)
+      break;
+
+    iter++;
+  }
+
+  memcpy (cm, R, sizeof (double) * n * n);
+
+  free (eva);
+  free (diag);
+  free (eve);
+  free (D_S);
+  free (R);
+
+  return ret;
+}
+
+/* ncm_stats_dist_vkde.c:337-360 (identical to ncm_stats_dist_kde.c:344-367):
+ * Cholesky, else nearPD, else the Cholesky of the diagonal. */
+void
+ok_cholesky_decomp_fallback (int d, const double *cov, double *U, int maxiter)
+{
+  memcpy (U, cov, sizeof (double) * d * d);
+
+  if (ok_cholesky_upper (d, U, d) != 0)
+  {
+    memcpy (U, cov, sizeof (double) * d * d);
+
+    if (ok_nearPD (d, U, maxiter) != 0)
+    {
+      int i;
+
+      memset (U, 0, sizeof (double) * d * d);
+
+      for (i = 0; i < d; i++)
+        U[i * d + i] = cov[i * d + i];
+
+      if (ok_cholesky_upper (d, U, d) != 0)
+        ok_set_error ("_cholesky_decomp: diagonal covariance is not positive.");
+    }
+  }
+}
+
+/* ======================================================================= */
+/* NcmStatsVec: weighted one-pass covariance                               */
+/* ======================================================================= */
+
+/*
+ * ncm_stats_vec.c:510-565 (update, weight 1) and :2376-2399 (get_cov_matrix):
+ * West/Welford update in insertion order; off-diagonal from the co-moment,
+ * diagonal from the variance accumulator, scaled by
+ * bias_wt = 1 / (W - W2 / W).
+ */
+void
+ok_cov_welford (int d, int nrows, const double *X, const int *rows, double *cov)
+{
+  double *mean  = calloc (d, sizeof (double));
+  double *var   = calloc (d, sizeof (double));
+  double *C     = calloc ((size_t) d * d, sizeof (double));
+  double weight = 0.0, weight2 = 0.0, bias_wt = 0.0;
+  const double w = 1.0;
+  int t, i, j;
+
+  for (t = 0; t < nrows; t++)
+  {
+    const double *x        = X + (size_t) (rows ? rows[t] : t) * d;
+    const double curweight = weight + w;
+
+    for (i = 0; i < d; i++)
+    {
+      double mean_i        = mean[i];
+      const double x_i     = x[i];
+      const double delta_i = x_i - mean_i;
+      const double R_i     = delta_i * w / curweight;
+      const double vv      = var[i];
+      const double dvar    = weight * delta_i * R_i;
+
+      mean_i += R_i;
+      mean[i] = mean_i;
+      var[i]  = vv + dvar;
+
+      for (j = i + 1; j < d; j++)
+      {
+        const double x_j    = x[j];
+        const double mean_j = mean[j];
+        const double dC_ij  = w * (x_i - mean_i) * (x_j - mean_j);
+        const double C_ij   = C[i * d + j] + dC_ij;
+
+        C[i * d + j] = C_ij;
+        C[j * d + i] = C_ij;
+      }
+    }
+
+    weight   = curweight;
+    weight2 += w * w;
+    bias_wt  = 1.0 / (weight - weight2 / weight);
+  }
+
+  for (i = 0; i < d; i++)
+    for (j = 0; j < d; j++)
+      cov[i * d + j] = ((i == j) ? var[i] : C[i * d + j]) * bias_wt;
+
+  free (mean);
+  free (var);
+  free (C);
+}
+
+/* ======================================================================= */
+/* Exact kNN                                                               */
+/* ======================================================================= */
+
+typedef struct
+{
+  double d2;
+  int j;
+} ok_pair;
+
+static inline int
+ok_pair_lt (const ok_pair *a, const ok_pair *b)
+{
+  /* rb_knn_list.c:31-40: order by distance, then by index */
+  if (a->d2 < b->d2)
+    return 1;
+
+  if (a->d2 > b->d2)
+    return 0;
+
+  return a->j < b->j;
+}
+
+static int
+ok_pair_cmp (const void *pa, const void *pb)
+{
+  const ok_pair *a = pa, *b = pb;
+
+  if (ok_pair_lt (a, b))
+    return -1;
+
+  if (ok_pair_lt (b, a))
+    return 1;
+
+  return 0;
+}
+
+static void
+ok_select (ok_pair *p, int n, int k)
+{
+  /* quickselect: afterwards p[0..k-1] hold the k smallest keys (any order);
+   * keys are distinct, so the selected set does not depend on pivots */
+  int lo = 0, hi = n - 1;
+
+  while (hi > lo)
+  {
+    const ok_pair piv = p[lo + (hi - lo) / 2];
+    int i = lo, j = hi;
+
+    while (i <= j)
+    {
+      while (ok_pair_lt (&p[i], &piv))
+        i++;
+
+      while (ok_pair_lt (&piv, &p[j]))
+        j--;
+
+      if (i <= j)
+      {
+        const ok_pair t = p[i];
+
+        p[i] = p[j];
+        p[j] = t;
+        i++;
+        j--;
+      }
+    }
+
+    if (k - 1 <= j)
+      hi = j;
+    else if (k - 1 >= i)
+      lo = i;
+    else
+      break;
+  }
+}
+
+/*
+ * kdtree.c:27-43 (squared distance summed over coordinates in order),
+ * kdtree.c:192-320 (exact k-NN search) and rb_knn_list.c:31-40 ((dist2,
+ * index) ordering).  The tree search returns exactly the k smallest
+ * (dist2, index) keys, in ascending order; this is a brute-force restatement.
+ */
+void
+ok_knn (int n, int d, const double *Y, int q, int k, int *out_idx, double *out_d2)
+{
+  ok_pair *p       = malloc (sizeof (ok_pair) * n);
+  const double *yq = Y + (size_t) q * d;
+  int j, l;
+
+  for (j = 0; j < n; j++)
+  {
+    const double *yj = Y + (size_t) j * d;
+    double dist      = 0.0;
+
+    for (l = 0; l < d; l++)
+    {
+      const double t = yj[l] - yq[l];
+
+      dist = dist + t * t;
+    }
+
+    p[j].d2 = dist;
+    p[j].j  = j;
+  }
+
+  if (k < n)
+    ok_select (p, n, k);
+
+  qsort (p, (k < n) ? k : n, sizeof (ok_pair), ok_pair_cmp);
+
+  for (l = 0; l < k && l < n; l++)
+  {
+    out_idx[l] = p[l].j;
+
+    if (out_d2)
+      out_d2[l] = p[l].d2;
+  }
+
+  free (p);
+}
+
+/* ======================================================================= */
+/* Active-set NNLS                                                         */
+/* ======================================================================= */
+
+/* Cholesky solve of an SPD system (dposv, ncm_matrix.c:1198-1208).
+ * S is |P|x|P| row-major (upper triangle used), b overwritten by x. */
+static int
+ok_chol_solve (int n, double *S, double *b)
+{
+  int i, k;
+  int ret = ok_cholesky_upper (n, S, n);
+
+  if (ret != 0)
+    return ret;
+
+  /* U^T y = b */
+  for (i = 0; i < n; i++)
+  {
+    double t = b[i];
+
+    for (k = 0; k < i; k++)
+      t = t - S[(size_t) k * n + i] * b[k];
+
+    b[i] = t / S[(size_t) i * n + i];
+  }
+
+  /* U x = y */
+  for (i = n - 1; i >= 0; i--)
+  {
+    double t = b[i];
+
+    for (k = i + 1; k < n; k++)
+      t = t - S[(size_t) i * n + k] * b[k];
+
+    b[i] = t / S[(size_t) i * n + i];
+  }
+
+  return 0;
+}
+
+/* Symmetric solve fallback (stands in for dsysv, ncm_nnls.c:574-605):
+ * Gaussian elimination with partial pivoting on the full symmetric matrix.
+ * Returns >0 when a pivot is exactly zero. */
+static int
+ok_lu_solve (int n, double *S, double *b)
+{
+  int i, j, k;
+
+  for (i = 0; i < n; i++)
+    for (j = 0; j < i; j++)
+      S[(size_t) i * n + j] = S[(size_t) j * n + i];
+
+  for (k = 0; k < n; k++)
+  {
+    int piv      = k;
+    double amax  = fabs (S[(size_t) k * n + k]);
+
+    for (i = k + 1; i < n; i++)
+      if (fabs (S[(size_t) i * n + k]) > amax)
+      {
+        amax = fabs (S[(size_t) i * n + k]);
+        piv  = i;
+      }
+
+    if (amax == 0.0)
+      return k + 1;
+
+    if (piv != k)
+    {
+      for (j = 0; j < n; j++)
+      {
+        const double t = S[(size_t) k * n + j];
+
+        S[(size_t) k * n + j]   = S[(size_t) piv * n + j];
+        S[(size_t) piv * n + j] = t;
+      }
+
+      {
+        const double t = b[k];
+
+        b[k]   = b[piv];
+        b[piv] = t;
+      }
+    }
+
+    for (i = k + 1; i < n; i++)
+    {
+      const double l = S[(size_t) i * n + k] / S[(size_t) k * n + k];
+
+      for (j = k + 1; j < n; j++)
+        S[(size_t) i * n + j] = S[(size_t) i * n + j] - l * S[(size_t) k * n + j];
+
+      b[i] = b[i] - l * b[k];
+    }
+  }
+
+  for (i = n - 1; i >= 0; i--)
+  {
+    double t = b[i];
+
+    for (j = i + 1; j < n; j++)
+      t = t - S[(size_t) i * n + j] * b[j];
+
+    b[i] = t / S[(size_t) i * n + i];
+  }
+
+  return 0;
+}
+
+/* Least squares min |A_P x - f| by Householder QR (stands in for dgels,
+ * ncm_nnls.c:608-640).  A is nrows x ncols row-major, P the column list. */
+static void
+ok_qr_lstsq (int nrows, int ncols, const double *A, const int *P, int np, const double *f, double *x)
+{
+  double *Q  = malloc (sizeof (double) * (size_t) nrows * np);
+  double *rh = malloc (sizeof (double) * nrows);
+  int i, j, k;
+
+  for (i = 0; i < nrows; i++)
+    for (j = 0; j < np; j++)
+      Q[(size_t) i * np + j] = A[(size_t) i * ncols + P[j]];
+
+  memcpy (rh, f, sizeof (double) * nrows);
+
+  for (k = 0; k < np && k < nrows; k++)
+  {
+    double nrm = 0.0, alpha, vtv = 0.0;
+
+    for (i = k; i < nrows; i++)
+      nrm = nrm + Q[(size_t) i * np + k] * Q[(size_t) i * np + k];
+
+    nrm   = sqrt (nrm);
+    alpha = (Q[(size_t) k * np + k] > 0.0) ? -nrm : nrm;
+
+    if (nrm == 0.0)
+      continue;
+
+    Q[(size_t) k * np + k] -= alpha;
+
+    for (i = k; i < nrows; i++)
+      vtv = vtv + Q[(size_t) i * np + k] * Q[(size_t) i * np + k];
+
+    for (j = k + 1; j < np; j++)
+    {
+      double s = 0.0;
+
+      for (i = k; i < nrows; i++)
+        s = s + Q[(size_t) i * np + k] * Q[(size_t) i * np + j];
+
+      s = 2.0 * s / vtv;
+
+      for (i = k; i < nrows; i++)
+        Q[(size_t) i * np + j] -= s * Q[(size_t) i * np + k];
+    }
+
+    {
+      double s = 0.0;
+
+      for (i = k; i < nrows; i++)
+        s = s + Q[(size_t) i * np + k] * rh[i];
+
+      s = 2.0 * s / vtv;
+
+      for (i = k; i < nrows; i++)
+        rh[i] -= s * Q[(size_t) i * np + k];
+    }
+
+    Q[(size_t) k * np + k] = alpha;
+  }
+
+  for (k = ((np < nrows) ? np : nrows) - 1; k >= 0; k--)
+  {
+    double t = rh[k];
+
+    for (j = k + 1; j < np && j < nrows; j++)
+      t = t - Q[(size_t) k * np + j] * x[j];
+
+    x[k] = (Q[(size_t) k * np + k] != 0.0) ? t / Q[(size_t) k * np + k] : 0.0;
+  }
+
+  for (k = nrows; k < np; k++)
+    x[k] = 0.0;
+
+  free (Q);
+  free (rh);
+}
+
+typedef struct
+{
+  int nrows, ncols;
+  const double *A;
+  const double *f;
+  double *M;   /* A^T A, full */
+  double *b;   /* A^T f */
+  double *S;   /* scratch |P|^2 */
+  double *bs;  /* scratch |P| */
+  int *P;      /* sorted index list */
+} ok_nnls_ws;
+
+/* ncm_nnls.c:655-666 -> :574-605 -> :608-640: normal equations by Cholesky,
+ * else the symmetric-indefinite solve, else least squares by QR.
+ * Writes the solution on the sorted index list into sol[0..np-1]. */
+static void
+ok_nnls_unconstrained (ok_nnls_ws *ws, const char *inP, double *sol)
+{
+  int np = 0, a, c;
+
+  for (a = 0; a < ws->ncols; a++)
+    if (inP[a])
+      ws->P[np++] = a;
+
+  for (a = 0; a < np; a++)
+  {
+    for (c = 0; c < np; c++)
+      ws->S[(size_t) a * np + c] = ws->M[(size_t) ws->P[a] * ws->ncols + ws->P[c]];
+
+    ws->bs[a] = ws->b[ws->P[a]];
+  }
+
+  if (ok_chol_solve (np, ws->S, ws->bs) > 0)
+  {
+    for (a = 0; a < np; a++)
+    {
+      for (c = 0; c < np; c++)
+        ws->S[(size_t) a * np + c] = ws->M[(size_t) ws->P[a] * ws->ncols + ws->P[c]];
+
+      ws->bs[a] = ws->b[ws->P[a]];
+    }
+
+    if (ok_lu_solve (np, ws->S, ws->bs) > 0)
+      ok_qr_lstsq (ws->nrows, ws->ncols, ws->A, ws->P, np, ws->f, ws->bs);
+  }
+
+  memcpy (sol, ws->bs, sizeof (double) * np);
+}
+
+/* ncm_nnls.c:728-751: solve on P, drop indices whose solution is < 1e-300
+ * (at most max_remove per pass, smallest first, ncm_iset.c:919-979) and
+ * repeat until the solution is feasible. */
+static void
+ok_nnls_feasible (ok_nnls_ws *ws, char *inP, double *x, int max_remove)
+{
+  const int n  = ws->ncols;
+  double *sol  = malloc (sizeof (double) * n);
+  int *inv     = malloc (sizeof (int) * n);
+  int ninv, a, np;
+
+  while (1)
+  {
+    ok_nnls_unconstrained (ws, inP, sol);
+
+    memset (x, 0, sizeof (double) * n);
+    np = 0;
+
+    for (a = 0; a < n; a++)
+      if (inP[a])
+        x[a] = sol[np++];
+
+    ninv = 0;
+
+    for (a = 0; a < n; a++)
+      if (inP[a] && (x[a] < 1.0e-300))
+        inv[ninv++] = a;
+
+    if (ninv == 0)
+      break;
+
+    if (max_remove >= ninv)
+    {
+      for (a = 0; a < ninv; a++)
+        inP[inv[a]] = 0;
+    }
+    else
+    {
+      /* remove the max_remove smallest x (ties by index) */
+      ok_pair *pp = malloc (sizeof (ok_pair) * ninv);
+
+      for (a = 0; a < ninv; a++)
+      {
+        pp[a].d2 = x[inv[a]];
+        pp[a].j  = inv[a];
+      }
+
+      qsort (pp, ninv, sizeof (ok_pair), ok_pair_cmp);
+
+      for (a = 0; a < max_remove; a++)
+        inP[pp[a].j] = 0;
+
+      free (pp);
+    }
+  }
+
+  free (sol);
+  free (inv);
+}
+
+static double
+ok_nnls_residuals (ok_nnls_ws *ws, const double *x, double *r)
+{
+  double s = 0.0;
+  int i, j;
+
+  for (i = 0; i < ws->nrows; i++)
+  {
+    double t = 0.0;
+
+    for (j = 0; j < ws->ncols; j++)
+      t = t + ws->A[(size_t) i * ws->ncols + j] * x[j];
+
+    r[i] = ws->f[i] - t;
+    s    = s + r[i] * r[i];
+  }
+
+  return sqrt (s);
+}
+
+static void
+ok_nnls_mgrad (ok_nnls_ws *ws, const double *r, double *g)
+{
+  int i, j;
+
+  for (j = 0; j < ws->ncols; j++)
+    g[j] = 0.0;
+
+  for (i = 0; i < ws->nrows; i++)
+    for (j = 0; j < ws->ncols; j++)
+      g[j] = g[j] + ws->A[(size_t) i * ws->ncols + j] * r[i];
+}
+
+/*
+ * ncm_nnls.c:767-871 (umethod NORMAL, reltol DBL_EPSILON): active-set NNLS
+ * on the normal equations.  Returns the residual norm |f - A x|.
+ */
+double
+ok_nnls (int nrows, int ncols, const double *A, const double *f, double *x)
+{
+  const double reltol = DBL_EPSILON;
+  ok_nnls_ws ws;
+  char *P     = malloc (ncols);
+  char *Ptry  = malloc (ncols);
+  double *xt  = malloc (sizeof (double) * ncols);
+  double *r   = malloc (sizeof (double) * nrows);
+  double *rt  = malloc (sizeof (double) * nrows);
+  double *g   = malloc (sizeof (double) * ncols);
+  ok_pair *cand = malloc (sizeof (ok_pair) * ncols);
+  double rnorm;
+  int i, j, k;
+
+  ws.nrows = nrows;
+  ws.ncols = ncols;
+  ws.A     = A;
+  ws.f     = f;
+  ws.M     = malloc (sizeof (double) * (size_t) ncols * ncols);
+  ws.b     = malloc (sizeof (double) * ncols);
+  ws.S     = malloc (sizeof (double) * (size_t) ncols * ncols);
+  ws.bs    = malloc (sizeof (double) * ncols);
+  ws.P     = malloc (sizeof (int) * ncols);
+
+  /* M = A^T A (dsyrk, ncm_matrix.c:1547-1575), b = A^T f (dgemv) */
+  #pragma omp parallel for schedule(dynamic, 8) private(j, k)
+  for (i = 0; i < ncols; i++)
+  {
+    for (j = i; j < ncols; j++)
+    {
+      double s = 0.0;
+
+      for (k = 0; k < nrows; k++)
+        s = s + A[(size_t) k * ncols + i] * A[(size_t) k * ncols + j];
+
+      ws.M[(size_t) i * ncols + j] = s;
+      ws.M[(size_t) j * ncols + i] = s;
+    }
+  }
+
+  for (j = 0; j < ncols; j++)
+  {
+    double s = 0.0;
+
+    for (k = 0; k < nrows; k++)
+      s = s + A[(size_t) k * ncols + j] * f[k];
+
+    ws.b[j] = s;
+  }
+
+  memset (P, 1, ncols);
+
+  ok_nnls_feasible (&ws, P, x, ncols);
+  rnorm = ok_nnls_residuals (&ws, x, r);
+  ok_nnls_mgrad (&ws, r, g);
+
+  while (1)
+  {
+    double add_frac = 1.0;
+    int finish      = 0;
+    double lrnorm;
+    unsigned added;
+
+    while (1)
+    {
+      /* ncm_iset.c:993-1078: candidates j not in P with g_j > rnorm*reltol,
+       * add the largest k*add_frac of them (at least one) */
+      const double gmin = rnorm * reltol;
+      unsigned kc       = 0;
+
+      memcpy (Ptry, P, ncols);
+
+      for (j = 0; j < ncols; j++)
+        if (!Ptry[j] && (g[j] > gmin))
+        {
+          cand[kc].d2 = -g[j]; /* ascending on -g == descending on g */
+          cand[kc].j  = j;
+          kc++;
+        }
+
+      {
+        const double kf  = kc * add_frac;
+        const double mx  = (kf > 1) ? kf : 1;
+        const double adf = (kc < mx) ? kc : mx;
+
+        added = (unsigned) adf;
+      }
+
+      if (kc == 0)
+        added = 0;
+
+      add_frac *= 0.5;
+
+      if (added == 0)
+      {
+        finish = 1;
+        break;
+      }
+
+      qsort (cand, kc, sizeof (ok_pair), ok_pair_cmp);
+
+      for (j = 0; j < (int) added; j++)
+        Ptry[cand[j].j] = 1;
+
+      ok_nnls_feasible (&ws, Ptry, xt, (int) added);
+      lrnorm = ok_nnls_residuals (&ws, xt, rt);
+
+      if (rnorm - lrnorm > rnorm * reltol)
+      {
+        memcpy (P, Ptry, ncols);
+        memcpy (x, xt, sizeof (double) * ncols);
+        memcpy (r, rt, sizeof (double) * nrows);
+        rnorm = lrnorm;
+        break;
+      }
+
+      if (added == 1)
+      {
+        finish = 1;
+        break;
+      }
+    }
+
+    if (finish)
+      break;
+
+    ok_nnls_mgrad (&ws, r, g);
+  }
+
+  free (ws.M);
+  free (ws.b);
+  free (ws.S);
+  free (ws.bs);
+  free (ws.P);
+  free (P);
+  free (Ptry);
+  free (xt);
+  free (r);
+  free (rt);
+  free (g);
+  free (cand);
+
+  return rnorm;
+}
+
+/* ======================================================================= */
+/* NcmStatsDist / KDE / VKDE                                               */
+/* ======================================================================= */
+
+struct ok_sd
+{
+  int kind, kernel, d;
+  double nu;
+  double over_smooth, shrink, local_frac;
+  int use_rot_href, cov_type, nearPD_maxiter, use_threads;
+  double *cov_fixed, *cov_fixed_decomp;
+
+  double *X;            /* sample array (add_obs copies), row-major */
+  int n_samples, n_alloc;
+
+  int prepared, n_obs, n_kernels, k;
+  double href, kernel_lnnorm, rnorm, min_m2lnp, max_m2lnp;
+  double *weights, *wcum;
+  int wcum_ready;
+  double *cov, *cov_decomp;  /* global d x d */
+  double *invU;              /* n_obs x d whitened sample */
+  double *U_arr;             /* VKDE: n_kernels x d x d */
+  double *lnnorms;           /* VKDE: n_kernels */
+  int *knn;                  /* VKDE: n_kernels x k */
+};
+
+ok_sd *
+ok_new (int kind, int kernel, double nu, int d)
+{
+  ok_sd *sd = calloc (1, sizeof (ok_sd));
+
+  sd->kind           = kind;
+  sd->kernel         = kernel;
+  sd->nu             = nu;
+  sd->d              = d;
+  sd->over_smooth    = 1.0;   /* ncm_stats_dist.c:389-396 */
+  sd->shrink         = 0.01;  /* ncm_stats_dist.c:421-427 */
+  sd->local_frac     = 0.05;  /* ncm_stats_dist_vkde.c:276-282 */
+  sd->use_rot_href   = 0;
+  sd->cov_type       = OK_COV_SAMPLE;
+  sd->nearPD_maxiter = 200;   /* ncm_stats_dist_kde.c:290-296 */
+  sd->use_threads    = 0;
+
+  return sd;
+}
+
+static void
+ok_clear_prepared (ok_sd *sd)
+{
+  free (sd->weights);
+  free (sd->wcum);
+  free (sd->cov);
+  free (sd->cov_decomp);
+  free (sd->invU);
+  free (sd->U_arr);
+  free (sd->lnnorms);
+  free (sd->knn);
+  sd->weights = sd->wcum = sd->cov = sd->cov_decomp = sd->invU = sd->U_arr = sd->lnnorms = NULL;
+  sd->knn      = NULL;
+  sd->prepared = 0;
+}
+
+void
+ok_free (ok_sd *sd)
+{
+  if (sd == NULL)
+    return;
+
+  ok_clear_prepared (sd);
+  free (sd->X);
+  free (sd->cov_fixed);
+  free (sd->cov_fixed_decomp);
+  free (sd);
+}
+
+void ok_set_over_smooth (ok_sd *sd, double v) { sd->over_smooth = v; }
+void ok_set_shrink (ok_sd *sd, double v) { sd->shrink = v; }
+void ok_set_local_frac (ok_sd *sd, double v) { sd->local_frac = v; }
+void ok_set_use_rot_href (ok_sd *sd, int v) { sd->use_rot_href = v; }
+void ok_set_nearPD_maxiter (ok_sd *sd, int v) { sd->nearPD_maxiter = v; }
+void ok_set_use_threads (ok_sd *sd, int v) { sd->use_threads = v; }
+
+/* ncm_stats_dist_kde.c:731-745 and :770-791 */
+static int
+ok_decompose_fixed (ok_sd *sd)
+{
+  const int d = sd->d;
+
+  if (sd->cov_fixed == NULL)
+    return 0;
+
+  if (sd->cov_fixed_decomp == NULL)
+    sd->cov_fixed_decomp = malloc (sizeof (double) * d * d);
+
+  memcpy (sd->cov_fixed_decomp, sd->cov_fixed, sizeof (double) * d * d);
+
+  if (ok_cholesky_upper (d, sd->cov_fixed_decomp, d) != 0)
+  {
+    ok_set_error ("ncm_stats_dist_kde_set_cov_fixed: matrix cov_fixed is not positive definite.");
+    return -1;
+  }
+
+  return 0;
+}
+
+int
+ok_set_cov_type (ok_sd *sd, int cov_type)
+{
+  sd->cov_type = cov_type;
+
+  if (cov_type == OK_COV_FIXED)
+    return ok_decompose_fixed (sd);
+
+  return 0;
+}
+
+int
+ok_set_cov_fixed (ok_sd *sd, const double *cov)
+{
+  const int d = sd->d;
+
+  if (sd->cov_fixed == NULL)
+    sd->cov_fixed = malloc (sizeof (double) * d * d);
+
+  memcpy (sd->cov_fixed, cov, sizeof (double) * d * d);
+
+  if (sd->cov_type == OK_COV_FIXED)
+    return ok_decompose_fixed (sd);
+
+  return 0;
+}
+
+/* ncm_stats_dist.c:1096-1101 */
+void
+ok_reset (ok_sd *sd)
+{
+  sd->n_samples = 0;
+}
+
+/* ncm_stats_dist.c:1680-1686: the vector is duplicated */
+void
+ok_add_obs (ok_sd *sd, const double *x)
+{
+  if (sd->n_samples == sd->n_alloc)
+  {
+    sd->n_alloc = (sd->n_alloc == 0) ? 64 : 2 * sd->n_alloc;
+    sd->X       = realloc (sd->X, sizeof (double) * (size_t) sd->n_alloc * sd->d);
+  }
+
+  memcpy (sd->X + (size_t) sd->n_samples * sd->d, x, sizeof (double) * sd->d);
+  sd->n_samples++;
+}
+
+/* ncm_stats_dist.c:476-482, ncm_stats_dist_vkde.c:316-335 */
+static double
+ok_get_href_internal (ok_sd *sd)
+{
+  if (sd->kind == OK_KDE)
+    return sd->over_smooth * ok_kernel_rot_bandwidth (sd->kernel, sd->nu, sd->d, sd->n_kernels);
+
+  if (sd->use_rot_href)
+    return sd->over_smooth * ok_kernel_rot_bandwidth (sd->kernel, sd->nu, sd->d, sd->n_kernels) / sd->local_frac;
+
+  return sd->over_smooth;
+}
+
+/* ncm_stats_dist_kde.c:378-490 */
+static int
+ok_kde_prepare_kernel (ok_sd *sd)
+{
+  const int d = sd->d;
+  int i, l;
+
+  sd->cov        = malloc (sizeof (double) * d * d);
+  sd->cov_decomp = malloc (sizeof (double) * d * d);
+
+  ok_cov_welford (d, sd->n_kernels, sd->X, NULL, sd->cov);
+
+  if (sd->cov_type == OK_COV_SAMPLE)
+  {
+    ok_cholesky_decomp_fallback (d, sd->cov, sd->cov_decomp, sd->nearPD_maxiter);
+  }
+  else
+  {
+    if (sd->cov_fixed == NULL)
+    {
+      ok_set_error ("_ncm_stats_dist_kde_prepare_kernel: cov_type is FIXED but a fixed "
+                    "covariance matrix was not provided, use ncm_stats_dist_kde_set_cov_fixed to set one.");
+      return -1;
+    }
+
+    memcpy (sd->cov, sd->cov_fixed, sizeof (double) * d * d);
+    memcpy (sd->cov_decomp, sd->cov_fixed_decomp, sizeof (double) * d * d);
+  }
+
+  sd->kernel_lnnorm = ok_kernel_lnnorm (sd->kernel, sd->nu, d, sd->cov_decomp);
+
+  /* invUsample = X U^{-1} (gsl_blas_dtrsm Right/Upper/NoTrans): each row
+   * solves y U = x by forward substitution */
+  sd->invU = malloc (sizeof (double) * (size_t) sd->n_obs * d);
+
+  for (i = 0; i < sd->n_obs; i++)
+  {
+    const double *x = sd->X + (size_t) i * d;
+    double *y       = sd->invU + (size_t) i * d;
+
+    for (l = 0; l < d; l++)
+    {
+      double t = x[l];
+      int m;
+
+      for (m = 0; m < l; m++)
+        t = t - y[m] * sd->cov_decomp[m * d + l];
+
+      y[l] = t / sd->cov_decomp[l * d + l];
+    }
+  }
+
+  return 0;
+}
+
+/* ncm_stats_dist_vkde.c:362-496 */
+static int
+ok_vkde_build_cov_array (ok_sd *sd)
+{
+  const int d = sd->d;
+  int i;
+
+  sd->k       = (int) (size_t) ((sd->local_frac * sd->n_obs > 2) ? sd->local_frac * sd->n_obs : 2);
+  sd->U_arr   = malloc (sizeof (double) * (size_t) sd->n_kernels * d * d);
+  sd->lnnorms = malloc (sizeof (double) * sd->n_kernels);
+  sd->knn     = malloc (sizeof (int) * (size_t) sd->n_kernels * sd->k);
+
+  #pragma omp parallel for schedule(dynamic, 1) if (sd->use_threads)
+  for (i = 0; i < sd->n_kernels; i++)
+  {
+    double *cov = malloc (sizeof (double) * d * d);
+    int *nb     = sd->knn + (size_t) i * sd->k;
+    double *U_i = sd->U_arr + (size_t) i * d * d;
+
+    ok_knn (sd->n_obs, d, sd->invU, i, sd->k, nb, NULL);
+    ok_cov_welford (d, sd->k, sd->X, nb, cov);
+    ok_cholesky_decomp_fallback (d, cov, U_i, sd->nearPD_maxiter);
+    sd->lnnorms[i] = ok_kernel_lnnorm (sd->kernel, sd->nu, d, U_i);
+    free (cov);
+  }
+
+  return 0;
+}
+
+/* ncm_stats_dist.c:703-789 (CV_NONE) */
+int
+ok_prepare (ok_sd *sd)
+{
+  int i;
+
+  ok_clear_prepared (sd);
+
+  sd->n_obs     = sd->n_samples;
+  sd->n_kernels = sd->n_samples;
+
+  if (sd->n_obs <= sd->d)
+  {
+    ok_set_error ("_ncm_stats_dist_prepare: the sample is too small.");
+    return -1;
+  }
+
+  if (sd->kind == OK_VKDE)
+  {
+    /* ncm_stats_dist_vkde.c:498-515 */
+    if (sd->local_frac * sd->n_obs < 2)
+    {
+      ok_set_error ("Too few observations.");
+      return -1;
+    }
+  }
+
+  if (ok_kde_prepare_kernel (sd) != 0)
+    return -1;
+
+  if (sd->kind == OK_VKDE)
+    ok_vkde_build_cov_array (sd);
+
+  sd->weights = malloc (sizeof (double) * sd->n_kernels);
+  sd->wcum    = malloc (sizeof (double) * (sd->n_kernels + 1));
+  sd->href    = ok_get_href_internal (sd);
+
+  for (i = 0; i < sd->n_kernels; i++)
+    sd->weights[i] = 1.0 / (1.0 * sd->n_kernels);
+
+  sd->wcum_ready = 0;
+  sd->prepared   = 1;
+  sd->rnorm      = 0.0;
+
+  return 0;
+}
+
+/* ncm_stats_dist_kde.c:492-557 and ncm_stats_dist_vkde.c:517-606 */
+int
+ok_compute_IM (ok_sd *sd, double *IM)
+{
+  const int d      = sd->d;
+  const int n_obs  = sd->n_obs;
+  const int n_k    = sd->n_kernels;
+  const double h   = sd->href;
+  const double h2  = h * h;
+  int i;
+
+  if (sd->kind == OK_KDE)
+  {
+    for (i = 0; i < n_k; i++)
+    {
+      const double *yi = sd->invU + (size_t) i * d;
+      int j;
+
+      IM[(size_t) i * n_k + i] = 0.0;
+
+      for (j = i + 1; j < n_k; j++)
+      {
+        const double *yj = sd->invU + (size_t) j * d;
+        double chi2      = 0.0;
+        int l;
+
+        for (l = 0; l < d; l++)
+        {
+          const double t = yi[l] - yj[l];
+
+          chi2 = chi2 + t * t;
+        }
+
+        chi2 = chi2 / h2;
+
+        IM[(size_t) i * n_k + j] = chi2;
+        IM[(size_t) j * n_k + i] = chi2;
+      }
+    }
+
+    for (i = 0; i < n_obs * n_k; i++)
+      IM[i] = ok_kernel_eval_unnorm (sd->kernel, sd->nu, d, IM[i]);
+
+    {
+      const double s = exp (-(sd->kernel_lnnorm + d * log (h)));
+
+      for (i = 0; i < n_obs * n_k; i++)
+        IM[i] = IM[i] * s;
+    }
+
+    return 0;
+  }
+
+  {
+    const double one_href2 = 1.0 / h2;
+
+    #pragma omp parallel for schedule(dynamic, 1) if (sd->use_threads)
+    for (i = 0; i < n_k; i++)
+    {
+      const double *U  = sd->U_arr + (size_t) i * d * d;
+      const double *ti = sd->X + (size_t) i * d;
+      double *z        = malloc (sizeof (double) * d);
+      int j, l, m;
+
+      for (j = 0; j < n_obs; j++)
+      {
+        const double *tj = sd->X + (size_t) j * d;
+        double chi2      = 0.0;
+
+        for (l = 0; l < d; l++)
+          z[l] = tj[l] + (-1.0) * ti[l];
+
+        /* dtrsm Right/Upper/NoTrans: z U = delta */
+        for (l = 0; l < d; l++)
+        {
+          double t = z[l];
+
+          for (m = 0; m < l; m++)
+            t = t - z[m] * U[m * d + l];
+
+          z[l] = t / U[l * d + l];
+        }
+
+        for (l = 0; l < d; l++)
+          chi2 = chi2 + z[l] * z[l];
+
+        IM[(size_t) j * n_k + i] = chi2 * one_href2;
+      }
+
+      free (z);
+    }
+
+    for (i = 0; i < n_obs * n_k; i++)
+      IM[i] = ok_kernel_eval_unnorm (sd->kernel, sd->nu, d, IM[i]);
+
+    {
+      const double lnnorm_href = d * log (h);
+
+      for (i = 0; i < n_k; i++)
+      {
+        const double norm_i = exp (sd->lnnorms[i] + lnnorm_href);
+        const double s      = 1.0 / norm_i;
+        int j;
+
+        for (j = 0; j < n_obs; j++)
+          IM[(size_t) j * n_k + i] = IM[(size_t) j * n_k + i] * s;
+      }
+    }
+  }
+
+  return 0;
+}
+
+static int
+ok_sort_cmp_val (const void *a, const void *b)
+{
+  const ok_pair *pa = a, *pb = b;
+
+  return ok_pair_cmp (pa, pb);
+}
+
+/* ncm_stats_dist.c:878-1094 (CV_NONE) */
+int
+ok_prepare_interp (ok_sd *sd, const double *m2lnp, int len)
+{
+  const double limit = -2.0 * 2.0 * OK_LOG_DBL_EPSILON;
+  int i;
+
+  if (ok_prepare (sd) != 0)
+    return -1;
+
+  if (len != sd->n_obs)
+  {
+    ok_set_error ("ncm_stats_dist_prepare_interp: m2lnp length differs from the sample size.");
+    return -1;
+  }
+
+  sd->min_m2lnp = INFINITY;
+  sd->max_m2lnp = -INFINITY;
+
+  for (i = 0; i < sd->n_kernels; i++)
+  {
+    sd->min_m2lnp = fmin (sd->min_m2lnp, m2lnp[i]);
+    sd->max_m2lnp = fmax (sd->max_m2lnp, m2lnp[i]);
+  }
+
+  if (sd->max_m2lnp - sd->min_m2lnp > limit)
+  {
+    ok_pair *srt = malloc (sizeof (ok_pair) * sd->n_kernels);
+    int n_cut    = 0;
+
+    for (i = 0; i < sd->n_kernels; i++)
+    {
+      srt[i].d2 = m2lnp[i];
+      srt[i].j  = i;
+    }
+
+    qsort (srt, sd->n_kernels, sizeof (ok_pair), ok_sort_cmp_val);
+
+    for (i = 0; i < sd->n_kernels; i++)
+    {
+      if (srt[i].d2 - sd->min_m2lnp > limit)
+      {
+        n_cut = i;
+        break;
+      }
+    }
+
+    if (n_cut < (int) (unsigned) (0.5 * sd->n_obs))
+    {
+      for (i = 0; i < sd->n_kernels; i++)
+        sd->weights[i] = 0.1 / (sd->n_kernels - n_cut);
+
+      for (i = 0; i < n_cut; i++)
+        sd->weights[srt[i].j] = 0.9 / n_cut;
+
+      sd->wcum_ready = 0;
+      free (srt);
+      return 0;
+    }
+
+    {
+      double *m2lnp_cut = malloc (sizeof (double) * n_cut);
+      double *Xcut      = malloc (sizeof (double) * (size_t) n_cut * sd->d);
+      int j             = 0;
+      int ret;
+
+      for (i = 0; i < sd->n_obs; i++)
+      {
+        if (m2lnp[i] - sd->min_m2lnp <= limit)
+        {
+          m2lnp_cut[j] = m2lnp[i];
+          memcpy (Xcut + (size_t) j * sd->d, sd->X + (size_t) i * sd->d, sizeof (double) * sd->d);
+          j++;
+        }
+      }
+
+      memcpy (sd->X, Xcut, sizeof (double) * (size_t) n_cut * sd->d);
+      sd->n_samples = n_cut;
+
+      ret = ok_prepare_interp (sd, m2lnp_cut, n_cut);
+
+      free (m2lnp_cut);
+      free (Xcut);
+      free (srt);
+
+      return ret;
+    }
+  }
+
+  {
+    const int n_obs = sd->n_obs;
+    const int n_k   = sd->n_kernels;
+    double *IM      = malloc (sizeof (double) * (size_t) n_obs * n_k);
+    double *f       = malloc (sizeof (double) * n_obs);
+    double *f1      = malloc (sizeof (double) * n_obs);
+
+    for (i = 0; i < n_obs; i++)
+    {
+      f[i]  = exp (-0.5 * (m2lnp[i] - sd->min_m2lnp));
+      f1[i] = 1.0;
+    }
+
+    ok_compute_IM (sd, IM);
+
+    /* ncm_stats_dist.c:791-804: row i scaled by 1/f_i */
+    for (i = 0; i < n_obs; i++)
+    {
+      const double s = 1.0 / f[i];
+      int j;
+
+      for (j = 0; j < n_k; j++)
+        IM[(size_t) i * n_k + j] = IM[(size_t) i * n_k + j] * s;
+    }
+
+    sd->rnorm = ok_nnls (n_obs, n_k, IM, f1, sd->weights);
+
+    free (IM);
+    free (f);
+    free (f1);
+  }
+
+  {
+    double total = 0.0;
+
+    for (i = 0; i < sd->n_kernels; i++)
+      total += sd->weights[i];
+
+    if (!(total > 0.0))
+    {
+      ok_set_error ("_ncm_stats_dist_prepare_interp: total weight is not positive.");
+      return -1;
+    }
+
+    for (i = 0; i < sd->n_kernels; i++)
+      sd->weights[i] = sd->weights[i] * ((1.0 - sd->shrink) / total);
+
+    for (i = 0; i < sd->n_kernels; i++)
+      sd->weights[i] = sd->weights[i] + sd->shrink / sd->n_kernels;
+  }
+
+  sd->wcum_ready = 0;
+
+  return 0;
+}
+
+/* chi2 of every kernel at x (first loop of the eval_weights* functions) */
+static void
+ok_chi2_all (ok_sd *sd, const double *x, double *chi2)
+{
+  const int d     = sd->d;
+  const double h  = sd->href;
+  const double h2 = h * h;
+  double *v       = malloc (sizeof (double) * d);
+  int i, l, m;
+
+  if (sd->kind == OK_KDE)
+  {
+    /* ncm_stats_dist_kde.c:648-667: v = U^{-T} x (dtrsv Upper/Trans) */
+    for (l = 0; l < d; l++)
+    {
+      double t = x[l];
+
+      for (m = 0; m < l; m++)
+        t = t - sd->cov_decomp[m * d + l] * v[m];
+
+      v[l] = t / sd->cov_decomp[l * d + l];
+    }
+
+    for (i = 0; i < sd->n_kernels; i++)
+    {
+      const double *yi = sd->invU + (size_t) i * d;
+      double c         = 0.0;
+
+      for (l = 0; l < d; l++)
+      {
+        const double t = yi[l] - v[l];
+
+        c = c + t * t;
+      }
+
+      chi2[i] = c / h2;
+    }
+  }
+  else
+  {
+    /* ncm_stats_dist_vkde.c:695-712 */
+    const double one_href2 = 1.0 / h2;
+
+    for (i = 0; i < sd->n_kernels; i++)
+    {
+      const double *U  = sd->U_arr + (size_t) i * d * d;
+      const double *ti = sd->X + (size_t) i * d;
+      double c         = 0.0;
+
+      for (l = 0; l < d; l++)
+        v[l] = x[l] + (-1.0) * ti[l];
+
+      for (l = 0; l < d; l++)
+      {
+        double t = v[l];
+
+        for (m = 0; m < l; m++)
+          t = t - U[m * d + l] * v[m];
+
+        v[l] = t / U[l * d + l];
+      }
+
+      for (l = 0; l < d; l++)
+        c = c + v[l] * v[l];
+
+      chi2[i] = c * one_href2;
+    }
+  }
+
+  free (v);
+}
+
+/* ncm_stats_dist_kde.c:639-681, ncm_stats_dist_vkde.c:681-723 */
+double
+ok_eval_m2lnp (ok_sd *sd, const double *x)
+{
+  double *chi2 = malloc (sizeof (double) * sd->n_kernels);
+  double gamma, lambda;
+
+  ok_chi2_all (sd, x, chi2);
+
+  if (sd->kind == OK_KDE)
+    ok_kernel_sum1 (sd->kernel, sd->nu, sd->d, sd->n_kernels, chi2, sd->weights, sd->kernel_lnnorm, &gamma, &lambda);
+  else
+    ok_kernel_sum0 (sd->kernel, sd->nu, sd->d, sd->n_kernels, chi2, sd->weights, sd->lnnorms, &gamma, &lambda);
+
+  free (chi2);
+
+  return -2.0 * (gamma + log1p (lambda) - sd->d * log (sd->href));
+}
+
+/* ncm_stats_dist_kde.c:596-637, ncm_stats_dist_vkde.c:631-679 */
+double
+ok_eval (ok_sd *sd, const double *x)
+{
+  double *chi2 = malloc (sizeof (double) * sd->n_kernels);
+  double res   = 0.0;
+  int i;
+
+  ok_chi2_all (sd, x, chi2);
+
+  for (i = 0; i < sd->n_kernels; i++)
+    chi2[i] = ok_kernel_eval_unnorm (sd->kernel, sd->nu, sd->d, chi2[i]);
+
+  if (sd->kind == OK_KDE)
+  {
+    double s = 0.0;
+
+    for (i = 0; i < sd->n_kernels; i++)
+      s = s + chi2[i] * sd->weights[i];
+
+    res = s * exp (-(sd->kernel_lnnorm + sd->d * log (sd->href)));
+  }
+  else
+  {
+    double s = 0.0;
+
+    for (i = 0; i < sd->n_kernels; i++)
+    {
+      const double u_i = exp (sd->lnnorms[i]);
+
+      s += sd->weights[i] * (chi2[i] / u_i);
+    }
+
+    res = s / pow (sd->href, sd->d);
+  }
+
+  free (chi2);
+
+  return res;
+}
+
+void
+ok_eval_m2lnp_batch (ok_sd *sd, const double *X, int m, double *out)
+{
+  int p;
+
+  #pragma omp parallel for schedule(dynamic, 4)
+  for (p = 0; p < m; p++)
+    out[p] = ok_eval_m2lnp (sd, X + (size_t) p * sd->d);
+}
+
+void
+ok_eval_batch (ok_sd *sd, const double *X, int m, double *out)
+{
+  int p;
+
+  #pragma omp parallel for schedule(dynamic, 4)
+  for (p = 0; p < m; p++)
+    out[p] = ok_eval (sd, X + (size_t) p * sd->d);
+}
+
+/* ncm_stats_dist.c:1565-1606; p = u because gsl_ran_flat(r, 0, 1) = u */
+int
+ok_kernel_choose (ok_sd *sd, double p)
+{
+  int ilo = 0, ihi = sd->n_kernels;
+
+  if (!sd->wcum_ready)
+  {
+    double cum = 0.0;
+    int i;
+
+    sd->wcum[0] = cum;
+
+    for (i = 0; i < sd->n_kernels; i++)
+    {
+      cum           += sd->weights[i];
+      sd->wcum[i + 1] = cum;
+    }
+
+    {
+      const double s = 1.0 / cum;
+
+      for (i = 0; i <= sd->n_kernels; i++)
+        sd->wcum[i] = sd->wcum[i] * s;
+    }
+
+    sd->wcum_ready = 1;
+  }
+
+  while (ihi > ilo + 1)
+  {
+    const int mi = (ihi + ilo) / 2;
+
+    if (sd->wcum[mi] > p)
+      ihi = mi;
+    else
+      ilo = mi;
+  }
+
+  return ilo;
+}
+
+/* ncm_stats_dist.c:1618-1627 with ncm_stats_dist_kernel_gauss.c:335-355 and
+ * ncm_stats_dist_kernel_st.c:388-414; z holds the d standard normals and
+ * chi2 the chi-squared(nu) draw (ignored for the Gauss kernel). */
+int
+ok_sample (ok_sd *sd, double u, const double *z, double chi2, double *x)
+{
+  const int d = sd->d;
+  const int i = ok_kernel_choose (sd, u);
+  const double *U  = (sd->kind == OK_KDE) ? sd->cov_decomp : sd->U_arr + (size_t) i * d * d;
+  const double *mu = sd->X + (size_t) i * d;
+  int l, m;
+
+  for (l = 0; l < d; l++)
+    x[l] = z[l] * sd->href;
+
+  /* dtrmv Upper/Trans: x = U^T x */
+  for (l = d - 1; l >= 0; l--)
+  {
+    double t = U[l * d + l] * x[l];
+
+    for (m = 0; m < l; m++)
+      t = t + U[m * d + l] * x[m];
+
+    x[l] = t;
+  }
+
+  if (sd->kernel == OK_ST)
+  {
+    const double chi_scale = sqrt (sd->nu / chi2);
+
+    for (l = 0; l < d; l++)
+      x[l] = x[l] * chi_scale;
+  }
+
+  for (l = 0; l < d; l++)
+    x[l] = x[l] + mu[l];
+
+  return i;
+}
+
+int ok_get_dim (ok_sd *sd) { return sd->d; }
+int ok_get_sample_size (ok_sd *sd) { return sd->n_obs; }
+int ok_get_n_kernels (ok_sd *sd) { return sd->n_kernels; }
+int ok_get_n_samples (ok_sd *sd) { return sd->n_samples; }
+int ok_get_k (ok_sd *sd) { return sd->k; }
+double ok_get_href (ok_sd *sd) { return sd->href; }
+
+/* ncm_stats_dist.c:1663-1669 returns the squared residual norm */
+double ok_get_rnorm (ok_sd *sd) { return sd->rnorm * sd->rnorm; }
+
+/* ncm_stats_dist_kde.c:579-587, ncm_stats_dist_vkde.c:619-629 */
+double
+ok_get_lnnorm (ok_sd *sd, int i)
+{
+  if (sd->kind == OK_KDE)
+    return sd->kernel_lnnorm + sd->d * log (sd->href);
+
+  return sd->lnnorms[i] + sd->d * log (sd->href);
+}
+
+void
+ok_get_weights (ok_sd *sd, double *w)
+{
+  memcpy (w, sd->weights, sizeof (double) * sd->n_kernels);
+}
+
+void
+ok_get_cov_decomp (ok_sd *sd, int i, double *U)
+{
+  const int d = sd->d;
+
+  if (sd->kind == OK_KDE)
+    memcpy (U, sd->cov_decomp, sizeof (double) * d * d);
+  else
+    memcpy (U, sd->U_arr + (size_t) i * d * d, sizeof (double) * d * d);
+}
+
+void
+ok_get_full_cov (ok_sd *sd, double *cov)
+{
+  memcpy (cov, sd->cov, sizeof (double) * sd->d * sd->d);
+}
+
+void
+ok_get_full_cov_decomp (ok_sd *sd, double *U)
+{
+  memcpy (U, sd->cov_decomp, sizeof (double) * sd->d * sd->d);
+}
+
+void
+ok_get_sample (ok_sd *sd, double *X)
+{
+  memcpy (X, sd->X, sizeof (double) * (size_t) sd->n_samples * sd->d);
+}
+
+void
+ok_get_invUsample (ok_sd *sd, double *Y)
+{
+  memcpy (Y, sd->invU, sizeof (double) * (size_t) sd->n_obs * sd->d);
+}
+
+void
+ok_get_knn (ok_sd *sd, int *idx)
+{
+  if (sd->knn)
+    memcpy (idx, sd->knn, sizeof (int) * (size_t) sd->n_kernels * sd->k);
+}

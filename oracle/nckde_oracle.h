@@ -1,0 +1,99 @@
+/*
+ * nckde_oracle.h -- FP64 CPU restatement of NumCosmo's kernel-density engine
+ * (NcmStatsDistKDE / NcmStatsDistVKDE with the Gauss and Student-t kernels,
+ * the active-set NNLS weight fit and mixture sampling).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * engine in numcosmo_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  It is never linked into, or
+ * called by, the product library.
+ *
+ * Every function cites the reference file:line it restates.  Paths are
+ * relative to the NumCosmo source tree.
+ */
+#ifndef NCKDE_ORACLE_H
+#define NCKDE_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OK_KDE = 0, OK_VKDE = 1 };
+enum { OK_GAUSS = 0, OK_ST = 1 };
+enum { OK_COV_SAMPLE = 0, OK_COV_FIXED = 1 };
+
+typedef struct ok_sd ok_sd;
+
+/* ---- kernel family (ncm_stats_dist_kernel_{gauss,st}.c) ---------------- */
+double ok_kernel_rot_bandwidth (int kernel, double nu, int d, double n);
+double ok_kernel_lnnorm (int kernel, double nu, int d, const double *U);
+double ok_kernel_eval_unnorm (int kernel, double nu, int d, double chi2);
+void   ok_kernel_sum0 (int kernel, double nu, int d, int n, const double *chi2,
+                       const double *w, const double *lnnorms, double *gamma, double *lambda);
+void   ok_kernel_sum1 (int kernel, double nu, int d, int n, const double *chi2,
+                       const double *w, double lnnorm, double *gamma, double *lambda);
+
+/* ---- dense algebra (ncm_matrix.c, ncm_lapack.c) ----------------------- */
+int    ok_cholesky_upper (int d, double *A, int lda);
+double ok_cholesky_lndet (int d, const double *U, int lda);
+int    ok_nearPD (int d, double *A, int maxiter);
+void   ok_cholesky_decomp_fallback (int d, const double *cov, double *U, int maxiter);
+int    ok_jacobi_eigen (int d, const double *A, double *eva, double *eve);
+
+/* ---- NcmStatsVec weighted covariance (ncm_stats_vec.c) ---------------- */
+void   ok_cov_welford (int d, int nrows, const double *X, const int *rows, double *cov);
+
+/* ---- exact kNN with (dist2, index) order (kdtree.c, rb_knn_list.c) ---- */
+void   ok_knn (int n, int d, const double *Y, int q, int k, int *out_idx, double *out_d2);
+
+/* ---- active-set NNLS (ncm_nnls.c, ncm_iset.c) ------------------------- */
+double ok_nnls (int nrows, int ncols, const double *A, const double *f, double *x);
+
+/* ---- NcmStatsDist object ---------------------------------------------- */
+ok_sd *ok_new (int kind, int kernel, double nu, int d);
+void   ok_free (ok_sd *sd);
+void   ok_set_over_smooth (ok_sd *sd, double v);
+void   ok_set_shrink (ok_sd *sd, double v);
+void   ok_set_local_frac (ok_sd *sd, double v);
+void   ok_set_use_rot_href (ok_sd *sd, int v);
+void   ok_set_nearPD_maxiter (ok_sd *sd, int v);
+void   ok_set_use_threads (ok_sd *sd, int v);
+int    ok_set_cov_type (ok_sd *sd, int cov_type);
+int    ok_set_cov_fixed (ok_sd *sd, const double *cov);
+void   ok_reset (ok_sd *sd);
+void   ok_add_obs (ok_sd *sd, const double *x);
+int    ok_prepare (ok_sd *sd);
+int    ok_prepare_interp (ok_sd *sd, const double *m2lnp, int len);
+int    ok_compute_IM (ok_sd *sd, double *IM);
+double ok_eval_m2lnp (ok_sd *sd, const double *x);
+double ok_eval (ok_sd *sd, const double *x);
+void   ok_eval_m2lnp_batch (ok_sd *sd, const double *X, int m, double *out);
+void   ok_eval_batch (ok_sd *sd, const double *X, int m, double *out);
+int    ok_kernel_choose (ok_sd *sd, double u);
+int    ok_sample (ok_sd *sd, double u, const double *z, double chi2, double *x);
+
+/* peeks (valid after prepare) */
+int    ok_get_dim (ok_sd *sd);
+int    ok_get_sample_size (ok_sd *sd);
+int    ok_get_n_kernels (ok_sd *sd);
+int    ok_get_n_samples (ok_sd *sd);
+int    ok_get_k (ok_sd *sd);
+double ok_get_href (ok_sd *sd);
+double ok_get_rnorm (ok_sd *sd);
+double ok_get_lnnorm (ok_sd *sd, int i);
+void   ok_get_weights (ok_sd *sd, double *w);
+void   ok_get_cov_decomp (ok_sd *sd, int i, double *U);
+void   ok_get_full_cov (ok_sd *sd, double *cov);
+void   ok_get_full_cov_decomp (ok_sd *sd, double *U);
+void   ok_get_sample (ok_sd *sd, double *X);
+void   ok_get_invUsample (ok_sd *sd, double *Y);
+void   ok_get_knn (ok_sd *sd, int *idx);
+const char *ok_last_error (void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

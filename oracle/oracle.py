@@ -1,0 +1,324 @@
+"""ctypes wrapper of the FP64 CPU oracle (liboracle_nckde.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` as the checker.  The product package
+``numcosmo_amd`` never imports this module.
+
+The object API mirrors NumCosmo's ``NcmStatsDist`` (numcosmo/ncm/stats/
+ncm_stats_dist.h:168-222) so that tests read like the reference's own.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_nckde.so")
+_lib = None
+
+KDE, VKDE = 0, 1
+GAUSS, ST = 0, 1
+COV_SAMPLE, COV_FIXED = 0, 1
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        sig = {
+            "ok_kernel_rot_bandwidth": (C.c_double, [C.c_int, C.c_double, C.c_int, C.c_double]),
+            "ok_kernel_lnnorm": (C.c_double, [C.c_int, C.c_double, C.c_int, _dp]),
+            "ok_kernel_eval_unnorm": (C.c_double, [C.c_int, C.c_double, C.c_int, C.c_double]),
+            "ok_kernel_sum0": (None, [C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "ok_kernel_sum1": (None, [C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, C.c_double,
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "ok_cholesky_upper": (C.c_int, [C.c_int, _dp, C.c_int]),
+            "ok_cholesky_lndet": (C.c_double, [C.c_int, _dp, C.c_int]),
+            "ok_nearPD": (C.c_int, [C.c_int, _dp, C.c_int]),
+            "ok_cholesky_decomp_fallback": (None, [C.c_int, _dp, _dp, C.c_int]),
+            "ok_jacobi_eigen": (C.c_int, [C.c_int, _dp, _dp, _dp]),
+            "ok_cov_welford": (None, [C.c_int, C.c_int, _dp, C.c_void_p, _dp]),
+            "ok_knn": (None, [C.c_int, C.c_int, _dp, C.c_int, C.c_int, _ip, C.c_void_p]),
+            "ok_nnls": (C.c_double, [C.c_int, C.c_int, _dp, _dp, _dp]),
+            "ok_new": (C.c_void_p, [C.c_int, C.c_int, C.c_double, C.c_int]),
+            "ok_free": (None, [C.c_void_p]),
+            "ok_set_over_smooth": (None, [C.c_void_p, C.c_double]),
+            "ok_set_shrink": (None, [C.c_void_p, C.c_double]),
+            "ok_set_local_frac": (None, [C.c_void_p, C.c_double]),
+            "ok_set_use_rot_href": (None, [C.c_void_p, C.c_int]),
+            "ok_set_nearPD_maxiter": (None, [C.c_void_p, C.c_int]),
+            "ok_set_use_threads": (None, [C.c_void_p, C.c_int]),
+            "ok_set_cov_type": (C.c_int, [C.c_void_p, C.c_int]),
+            "ok_set_cov_fixed": (C.c_int, [C.c_void_p, _dp]),
+            "ok_reset": (None, [C.c_void_p]),
+            "ok_add_obs": (None, [C.c_void_p, _dp]),
+            "ok_prepare": (C.c_int, [C.c_void_p]),
+            "ok_prepare_interp": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+            "ok_compute_IM": (C.c_int, [C.c_void_p, _dp]),
+            "ok_eval_m2lnp": (C.c_double, [C.c_void_p, _dp]),
+            "ok_eval": (C.c_double, [C.c_void_p, _dp]),
+            "ok_eval_m2lnp_batch": (None, [C.c_void_p, _dp, C.c_int, _dp]),
+            "ok_eval_batch": (None, [C.c_void_p, _dp, C.c_int, _dp]),
+            "ok_kernel_choose": (C.c_int, [C.c_void_p, C.c_double]),
+            "ok_sample": (C.c_int, [C.c_void_p, C.c_double, _dp, C.c_double, _dp]),
+            "ok_get_dim": (C.c_int, [C.c_void_p]),
+            "ok_get_sample_size": (C.c_int, [C.c_void_p]),
+            "ok_get_n_kernels": (C.c_int, [C.c_void_p]),
+            "ok_get_n_samples": (C.c_int, [C.c_void_p]),
+            "ok_get_k": (C.c_int, [C.c_void_p]),
+            "ok_get_href": (C.c_double, [C.c_void_p]),
+            "ok_get_rnorm": (C.c_double, [C.c_void_p]),
+            "ok_get_lnnorm": (C.c_double, [C.c_void_p, C.c_int]),
+            "ok_get_weights": (None, [C.c_void_p, _dp]),
+            "ok_get_cov_decomp": (None, [C.c_void_p, C.c_int, _dp]),
+            "ok_get_full_cov": (None, [C.c_void_p, _dp]),
+            "ok_get_full_cov_decomp": (None, [C.c_void_p, _dp]),
+            "ok_get_sample": (None, [C.c_void_p, _dp]),
+            "ok_get_invUsample": (None, [C.c_void_p, _dp]),
+            "ok_get_knn": (None, [C.c_void_p, _ip]),
+            "ok_last_error": (C.c_char_p, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+# ---------------------------------------------------------------- kernels --
+def rot_bandwidth(kernel, nu, d, n):
+    return lib().ok_kernel_rot_bandwidth(kernel, nu, d, n)
+
+
+def kernel_lnnorm(kernel, nu, d, U):
+    return lib().ok_kernel_lnnorm(kernel, nu, d, _f64(U))
+
+
+def eval_unnorm(kernel, nu, d, chi2):
+    return lib().ok_kernel_eval_unnorm(kernel, nu, d, chi2)
+
+
+def sum0(kernel, nu, d, chi2, w, lnnorms):
+    g, lam = C.c_double(), C.c_double()
+    chi2 = _f64(chi2)
+    lib().ok_kernel_sum0(kernel, nu, d, len(chi2), chi2, _f64(w), _f64(lnnorms), C.byref(g), C.byref(lam))
+    return g.value, lam.value
+
+
+def sum1(kernel, nu, d, chi2, w, lnnorm):
+    g, lam = C.c_double(), C.c_double()
+    chi2 = _f64(chi2)
+    lib().ok_kernel_sum1(kernel, nu, d, len(chi2), chi2, _f64(w), lnnorm, C.byref(g), C.byref(lam))
+    return g.value, lam.value
+
+
+# ---------------------------------------------------------------- algebra --
+def cholesky_upper(A):
+    U = _f64(A).copy()
+    info = lib().ok_cholesky_upper(U.shape[0], U, U.shape[0])
+    return info, U
+
+
+def cholesky_lndet(U):
+    U = _f64(U)
+    return lib().ok_cholesky_lndet(U.shape[0], U, U.shape[0])
+
+
+def nearPD(A, maxiter=200):
+    U = _f64(A).copy()
+    info = lib().ok_nearPD(U.shape[0], U, maxiter)
+    return info, U
+
+
+def cholesky_decomp_fallback(cov, maxiter=200):
+    cov = _f64(cov)
+    U = np.empty_like(cov)
+    lib().ok_cholesky_decomp_fallback(cov.shape[0], cov, U, maxiter)
+    return U
+
+
+def jacobi_eigen(A):
+    A = _f64(A)
+    d = A.shape[0]
+    eva = np.empty(d)
+    eve = np.empty((d, d))
+    lib().ok_jacobi_eigen(d, A, eva, eve)
+    return eva, eve
+
+
+def cov_welford(X, rows=None):
+    X = _f64(X)
+    d = X.shape[1]
+    cov = np.empty((d, d))
+    if rows is None:
+        lib().ok_cov_welford(d, X.shape[0], X, None, cov)
+    else:
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        lib().ok_cov_welford(d, len(r), X, r.ctypes.data, cov)
+    return cov
+
+
+def knn(Y, q, k):
+    Y = _f64(Y)
+    idx = np.empty(k, dtype=np.int32)
+    d2 = np.empty(k)
+    lib().ok_knn(Y.shape[0], Y.shape[1], Y, q, k, idx, d2.ctypes.data)
+    return idx, d2
+
+
+def nnls(A, f):
+    A = _f64(A)
+    x = np.zeros(A.shape[1])
+    rnorm = lib().ok_nnls(A.shape[0], A.shape[1], A, _f64(f), x)
+    return x, rnorm
+
+
+# ------------------------------------------------------------ stats dist --
+class StatsDist:
+    """Oracle mirror of NcmStatsDistKDE / NcmStatsDistVKDE (CV_NONE)."""
+
+    def __init__(self, kind, kernel, d, nu=3.0):
+        self._h = lib().ok_new(kind, kernel, float(nu), int(d))
+        self.kind, self.kernel, self.d, self.nu = kind, kernel, int(d), float(nu)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().ok_free(self._h)
+            self._h = None
+
+    def _check(self, ret):
+        if ret != 0:
+            raise OracleError(lib().ok_last_error().decode())
+
+    # configuration
+    def set_over_smooth(self, v): lib().ok_set_over_smooth(self._h, v)
+    def set_shrink(self, v): lib().ok_set_shrink(self._h, v)
+    def set_local_frac(self, v): lib().ok_set_local_frac(self._h, v)
+    def set_use_rot_href(self, v): lib().ok_set_use_rot_href(self._h, int(v))
+    def set_nearPD_maxiter(self, v): lib().ok_set_nearPD_maxiter(self._h, int(v))
+    def set_use_threads(self, v): lib().ok_set_use_threads(self._h, int(v))
+    def set_cov_type(self, v): self._check(lib().ok_set_cov_type(self._h, int(v)))
+    def set_cov_fixed(self, cov): self._check(lib().ok_set_cov_fixed(self._h, _f64(cov)))
+
+    def reset(self): lib().ok_reset(self._h)
+
+    def add_obs(self, x):
+        lib().ok_add_obs(self._h, _f64(x))
+
+    def add_obs_array(self, X):
+        for row in _f64(X):
+            lib().ok_add_obs(self._h, row)
+
+    def prepare(self):
+        self._check(lib().ok_prepare(self._h))
+
+    def prepare_interp(self, m2lnp):
+        m2lnp = _f64(m2lnp)
+        self._check(lib().ok_prepare_interp(self._h, m2lnp, len(m2lnp)))
+
+    def compute_IM(self):
+        n = self.get_sample_size()
+        IM = np.empty((n, self.get_n_kernels()))
+        self._check(lib().ok_compute_IM(self._h, IM))
+        return IM
+
+    def eval_m2lnp(self, x):
+        return lib().ok_eval_m2lnp(self._h, _f64(x))
+
+    def eval(self, x):
+        return lib().ok_eval(self._h, _f64(x))
+
+    def eval_m2lnp_batch(self, X):
+        X = _f64(X)
+        out = np.empty(X.shape[0])
+        lib().ok_eval_m2lnp_batch(self._h, X, X.shape[0], out)
+        return out
+
+    def eval_batch(self, X):
+        X = _f64(X)
+        out = np.empty(X.shape[0])
+        lib().ok_eval_batch(self._h, X, X.shape[0], out)
+        return out
+
+    def kernel_choose(self, u):
+        return lib().ok_kernel_choose(self._h, float(u))
+
+    def sample(self, u, z, chi2=1.0):
+        x = np.empty(self.d)
+        i = lib().ok_sample(self._h, float(u), _f64(z), float(chi2), x)
+        return i, x
+
+    # peeks
+    def get_dim(self): return lib().ok_get_dim(self._h)
+    def get_sample_size(self): return lib().ok_get_sample_size(self._h)
+    def get_n_kernels(self): return lib().ok_get_n_kernels(self._h)
+    def get_n_samples(self): return lib().ok_get_n_samples(self._h)
+    def get_k(self): return lib().ok_get_k(self._h)
+    def get_href(self): return lib().ok_get_href(self._h)
+    def get_rnorm(self): return lib().ok_get_rnorm(self._h)
+    def get_lnnorm(self, i): return lib().ok_get_lnnorm(self._h, int(i))
+
+    def get_weights(self):
+        w = np.empty(self.get_n_kernels())
+        lib().ok_get_weights(self._h, w)
+        return w
+
+    def get_cov_decomp(self, i):
+        U = np.empty((self.d, self.d))
+        lib().ok_get_cov_decomp(self._h, int(i), U)
+        return U
+
+    def get_lnnorms_raw(self):
+        """lnnorm_i without the d*ln(href) term (the kernel's own)."""
+        lnh = self.d * np.log(self.get_href())
+        return np.array([self.get_lnnorm(i) for i in range(self.get_n_kernels())]) - lnh
+
+    def get_full_cov(self):
+        c = np.empty((self.d, self.d))
+        lib().ok_get_full_cov(self._h, c)
+        return c
+
+    def get_full_cov_decomp(self):
+        c = np.empty((self.d, self.d))
+        lib().ok_get_full_cov_decomp(self._h, c)
+        return c
+
+    def get_sample(self):
+        X = np.empty((self.get_n_samples(), self.d))
+        lib().ok_get_sample(self._h, X)
+        return X
+
+    def get_invUsample(self):
+        Y = np.empty((self.get_sample_size(), self.d))
+        lib().ok_get_invUsample(self._h, Y)
+        return Y
+
+    def get_knn(self):
+        k = self.get_k()
+        idx = np.empty((self.get_n_kernels(), k), dtype=np.int32)
+        lib().ok_get_knn(self._h, idx)
+        return idx
