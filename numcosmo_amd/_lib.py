@@ -1,0 +1,90 @@
+"""ctypes binding of libnckde.so (the C-ABI declared in include/nckde.h).
+
+There is no fallback: if the HIP library is missing or cannot be loaded,
+``lib()`` raises, and every object of the package fails with it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnckde.so")
+
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_h = C.c_void_p
+_sz = C.c_size_t
+
+# name -> (restype, argtypes); the complete exported surface of nckde.h
+SIGNATURES = {
+    "nckde_last_error": (C.c_char_p, []),
+    "nckde_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_double, C.c_uint, C.c_int]),
+    "nckde_destroy": (C.c_int, [_h]),
+    "nckde_set_over_smooth": (C.c_int, [_h, C.c_double]),
+    "nckde_set_shrink": (C.c_int, [_h, C.c_double]),
+    "nckde_set_local_frac": (C.c_int, [_h, C.c_double]),
+    "nckde_set_use_rot_href": (C.c_int, [_h, C.c_int]),
+    "nckde_set_cov_type": (C.c_int, [_h, C.c_int]),
+    "nckde_set_cov_fixed": (C.c_int, [_h, _dp]),
+    "nckde_set_nearPD_maxiter": (C.c_int, [_h, C.c_uint]),
+    "nckde_set_use_threads": (C.c_int, [_h, C.c_int]),
+    "nckde_add_obs": (C.c_int, [_h, _dp]),
+    "nckde_add_obs_batch": (C.c_int, [_h, _dp, _sz, _sz]),
+    "nckde_reset": (C.c_int, [_h]),
+    "nckde_prepare": (C.c_int, [_h]),
+    "nckde_prepare_interp": (C.c_int, [_h, _dp, _sz]),
+    "nckde_compute_IM": (C.c_int, [_h, _dp]),
+    "nckde_eval_m2lnp": (C.c_int, [_h, _dp, _sz, _sz, _dp, C.c_int]),
+    "nckde_eval": (C.c_int, [_h, _dp, _sz, _sz, _dp, C.c_int]),
+    "nckde_eval_m2lnp_device": (C.c_int, [_h, C.c_void_p, _sz, C.c_void_p, C.c_int, C.c_void_p]),
+    "nckde_kernel_choose": (C.c_int, [_h, _dp, _sz, _up]),
+    "nckde_sample": (C.c_int, [_h, _dp, _dp, C.c_void_p, _sz, _dp, C.c_void_p]),
+    "nckde_get_dim": (C.c_int, [_h, C.POINTER(C.c_uint)]),
+    "nckde_get_sample_size": (C.c_int, [_h, C.POINTER(_sz)]),
+    "nckde_get_n_kernels": (C.c_int, [_h, C.POINTER(_sz)]),
+    "nckde_get_n_samples": (C.c_int, [_h, C.POINTER(_sz)]),
+    "nckde_get_href": (C.c_int, [_h, C.POINTER(C.c_double)]),
+    "nckde_get_rnorm": (C.c_int, [_h, C.POINTER(C.c_double)]),
+    "nckde_get_lnnorm": (C.c_int, [_h, _sz, C.POINTER(C.c_double)]),
+    "nckde_get_weights": (C.c_int, [_h, _dp]),
+    "nckde_get_cov_decomp": (C.c_int, [_h, _sz, _dp]),
+    "nckde_get_full_cov": (C.c_int, [_h, _dp]),
+    "nckde_get_full_cov_decomp": (C.c_int, [_h, _dp]),
+    "nckde_get_sample": (C.c_int, [_h, _dp]),
+    "nckde_get_invUsample": (C.c_int, [_h, _dp]),
+    "nckde_get_knn": (C.c_int, [_h, C.POINTER(_sz), C.c_void_p]),
+}
+
+
+class NckdeError(RuntimeError):
+    """A failing nckde call; the message is the library's (the reference's
+    g_error text where the reference aborts)."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m numcosmo_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int):
+    if status != 0:
+        raise NckdeError(status, lib().nckde_last_error().decode(errors="replace"))

@@ -1,0 +1,1483 @@
+// api.cpp -- host side of the nckde C-ABI.
+//
+// Control flow follows NcmStatsDist (ncm_stats_dist.c:703-1094), the KDE /
+// VKDE prepare_kernel methods (ncm_stats_dist_kde.c:378-490,
+// ncm_stats_dist_vkde.c:362-515) and ncm_nnls_solve (ncm_nnls.c:767-871).
+// Every arithmetic step on sample-sized data runs in a HIP kernel; the host
+// keeps the index-set bookkeeping of the active-set loop and the O(n)
+// decisions of prepare_interp (range check, cut, fallback weights).
+#include "nckde_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace nck
+{
+
+static thread_local char g_err[1024];
+
+void
+set_error (const char *fmt, ...)
+{
+  va_list ap;
+
+  va_start (ap, fmt);
+  vsnprintf (g_err, sizeof (g_err), fmt, ap);
+  va_end (ap);
+}
+
+hipError_t
+DBuf::ensure (size_t nbytes)
+{
+  if (nbytes <= bytes && p != nullptr)
+    return hipSuccess;
+
+  release ();
+
+  if (nbytes == 0)
+    nbytes = 8;
+
+  hipError_t e = hipMalloc (&p, nbytes);
+
+  if (e != hipSuccess)
+  {
+    p     = nullptr;
+    bytes = 0;
+    return e;
+  }
+
+  bytes = nbytes;
+
+  return hipSuccess;
+}
+
+void
+DBuf::release ()
+{
+  if (p)
+    (void) hipFree (p);
+
+  p     = nullptr;
+  bytes = 0;
+}
+
+} // namespace nck
+
+using nck::DBuf;
+using nck::set_error;
+
+struct nckde
+{
+  int kind, kernel, cv;
+  double nu;
+  int d;
+
+  double over_smooth      = 1.0;
+  double shrink           = 0.01;
+  double local_frac       = 0.05;
+  int use_rot_href        = 0;
+  int cov_type            = NCKDE_COV_SAMPLE;
+  unsigned nearPD_maxiter = 200;
+
+  bool has_cov_fixed = false;
+  std::vector<double> cov_fixed, cov_fixed_decomp;
+  double cov_fixed_lnnorm = 0.0;
+
+  std::vector<double> X;  // sample array (copies), row-major
+  size_t n_samples = 0;
+
+  bool prepared = false;
+  size_t n_obs = 0, n_k = 0, k = 0;
+  int D = 0;
+  double href = 0.0, kernel_lnnorm = 0.0, rnorm = 0.0;
+  bool wcum_ready = false;
+
+  hipStream_t stream      = nullptr;
+  hipStream_t last_stream = nullptr;  // stream of the last device-form eval
+  std::mutex mtx;
+
+  // prepared device state
+  DBuf dX, dY, dcovg, dUg, dlng, dcov, dU, dinfo, dlnnorm, dw, db, dWp, dTh, dknn, dkeys, dscratch, dstatus,
+       dwcum, dwhich;
+  // work buffers
+  DBuf dXe, dout, dpartM, dpartS, du, dz, dchi2, didx;
+  DBuf dAT, dM, dS, dbv, dbs, dx, dxt, dr, drt, dg, dP, dpart, dscal, dm2lnp, dlu;
+};
+
+#define HIPCHK(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+    {                                                                                \
+      set_error ("%s failed: %s", #call, hipGetErrorString (e_));                    \
+      return NCKDE_EHIP;                                                             \
+    }                                                                                \
+  } while (0)
+
+#define TRY(call)                                                                    \
+  do {                                                                               \
+    int r_ = (call);                                                                 \
+    if (r_ != NCKDE_OK)                                                              \
+      return r_;                                                                     \
+  } while (0)
+
+static double
+rot_bandwidth (const nckde *h, double n)
+{
+  // ncm_stats_dist_kernel_gauss.c:193-199, ncm_stats_dist_kernel_st.c:225-239
+  const unsigned d = (unsigned) h->d;
+
+  if (h->kernel == NCKDE_KERNEL_GAUSS)
+    return std::pow (4.0 / (n * (d + 2.0)), 1.0 / (d + 4.0));
+
+  const double nu = (h->nu >= 3.0) ? h->nu : 3.0;
+
+  return std::pow (16.0 * ((nu - 2) * (nu - 2)) * (1.0 + d + nu) * (3.0 + d + nu) /
+                   ((2.0 + d) * (d + nu) * (2.0 + d + nu) * (d + 2.0 * nu) * (2.0 + d + 2.0 * nu) * n),
+                   1.0 / (d + 4.0));
+}
+
+static double
+st_lnnorm_const (const nckde *h)
+{
+  // lgamma(nu/2) - lgamma((nu+d)/2) + (d/2)(ln pi + ln nu)
+  // (ncm_stats_dist_kernel_st.c:241-253)
+  const unsigned d = (unsigned) h->d;
+  const double lg  = std::lgamma (h->nu / 2.0) - std::lgamma ((h->nu + d) / 2.0);
+  const double nc  = (d / 2.0) * (nck::LNPI + std::log (h->nu));
+
+  return lg + nc;
+}
+
+static double
+get_href (const nckde *h)
+{
+  // ncm_stats_dist.c:476-482, ncm_stats_dist_vkde.c:316-335
+  if (h->kind == NCKDE_KDE)
+    return h->over_smooth * rot_bandwidth (h, (double) h->n_k);
+
+  if (h->use_rot_href)
+    return h->over_smooth * rot_bandwidth (h, (double) h->n_k) / h->local_frac;
+
+  return h->over_smooth;
+}
+
+static int
+check_handle (nckde *h)
+{
+  if (h == nullptr)
+  {
+    set_error ("nckde: NULL handle");
+    return NCKDE_EINVAL;
+  }
+
+  return NCKDE_OK;
+}
+
+static int
+check_prepared (nckde *h)
+{
+  TRY (check_handle (h));
+
+  if (!h->prepared)
+  {
+    set_error ("nckde: object not prepared (call prepare or prepare_interp first)");
+    return NCKDE_ESTATE;
+  }
+
+  return NCKDE_OK;
+}
+
+// Batched Cholesky with the reference's nearPD / diagonal fallback
+// (ncm_stats_dist_kde.c:344-367): factor nmats matrices of dcov into dU,
+// then repair the failures on the device.
+static int
+factor_batch (nckde *h, const double *dcov, double *dU, double *dlnnorm, size_t nmats)
+{
+  const int d          = h->d;
+  const double lconst  = (h->kernel == NCKDE_KERNEL_ST) ? st_lnnorm_const (h) : 0.0;
+  hipStream_t s        = h->stream;
+
+  HIPCHK (h->dinfo.ensure (sizeof (int) * nmats));
+  HIPCHK (nck::launch_chol_batched (dcov, dU, h->dinfo.as<int> (), dlnnorm, d, (int) nmats, h->kernel, lconst, s));
+
+  std::vector<int> info (nmats);
+
+  HIPCHK (hipMemcpyAsync (info.data (), h->dinfo.p, sizeof (int) * nmats, hipMemcpyDeviceToHost, s));
+  HIPCHK (hipStreamSynchronize (s));
+
+  std::vector<int> which;
+
+  for (size_t i = 0; i < nmats; ++i)
+    if (info[i] != 0)
+      which.push_back ((int) i);
+
+  if (which.empty ())
+    return NCKDE_OK;
+
+  const size_t per = (6 * (size_t) d * d + 3 * (size_t) d) * sizeof (double) + 64;
+  int zero         = 0;
+
+  HIPCHK (h->dwhich.ensure (sizeof (int) * which.size ()));
+  HIPCHK (h->dscratch.ensure (per * which.size ()));
+  HIPCHK (h->dstatus.ensure (sizeof (int) * 4));
+  HIPCHK (hipMemcpyAsync (h->dwhich.p, which.data (), sizeof (int) * which.size (), hipMemcpyHostToDevice, s));
+  HIPCHK (hipMemcpyAsync (h->dstatus.p, &zero, sizeof (int), hipMemcpyHostToDevice, s));
+  HIPCHK (nck::launch_nearpd_fix (h->dwhich.as<int> (), (int) which.size (), dcov, dU, dlnnorm,
+                                  h->dscratch.as<double> (), d, (int) h->nearPD_maxiter, h->kernel, lconst,
+                                  h->dstatus.as<int> (), s));
+
+  int status = 0;
+
+  HIPCHK (hipMemcpyAsync (&status, h->dstatus.p, sizeof (int), hipMemcpyDeviceToHost, s));
+  HIPCHK (hipStreamSynchronize (s));
+
+  if (status != 0)
+  {
+    set_error ("_cholesky_decomp: the diagonal of the covariance is not positive.");
+    return NCKDE_ENUMERIC;
+  }
+
+  return NCKDE_OK;
+}
+
+// KDE prepare_kernel (ncm_stats_dist_kde.c:378-490)
+static int
+kde_prepare_kernel (nckde *h)
+{
+  const int d   = h->d;
+  hipStream_t s = h->stream;
+
+  HIPCHK (h->dX.ensure (sizeof (double) * h->n_obs * d));
+  HIPCHK (h->dY.ensure (sizeof (double) * h->n_obs * d));
+  HIPCHK (h->dcovg.ensure (sizeof (double) * d * d));
+  HIPCHK (h->dUg.ensure (sizeof (double) * d * d));
+  HIPCHK (h->dlng.ensure (sizeof (double)));
+  HIPCHK (hipMemcpyAsync (h->dX.p, h->X.data (), sizeof (double) * h->n_obs * d, hipMemcpyHostToDevice, s));
+
+  HIPCHK (nck::launch_welford_cov (h->dX.as<double> (), d, (int) h->n_k, nullptr, 0, 1, h->dcovg.as<double> (), s));
+
+  if (h->cov_type == NCKDE_COV_SAMPLE)
+  {
+    TRY (factor_batch (h, h->dcovg.as<double> (), h->dUg.as<double> (), h->dlng.as<double> (), 1));
+    HIPCHK (hipMemcpyAsync (&h->kernel_lnnorm, h->dlng.p, sizeof (double), hipMemcpyDeviceToHost, s));
+  }
+  else if (h->cov_type == NCKDE_COV_FIXED)
+  {
+    if (!h->has_cov_fixed)
+    {
+      set_error ("_ncm_stats_dist_kde_prepare_kernel: cov_type is FIXED but a fixed covariance matrix was not "
+                 "provided, use ncm_stats_dist_kde_set_cov_fixed to set one.");
+      return NCKDE_ESTATE;
+    }
+
+    HIPCHK (hipMemcpyAsync (h->dcovg.p, h->cov_fixed.data (), sizeof (double) * d * d, hipMemcpyHostToDevice, s));
+    HIPCHK (hipMemcpyAsync (h->dUg.p, h->cov_fixed_decomp.data (), sizeof (double) * d * d, hipMemcpyHostToDevice, s));
+    h->kernel_lnnorm = h->cov_fixed_lnnorm;
+  }
+  else
+  {
+    set_error ("nckde: robust covariance types are not implemented");
+    return NCKDE_EUNSUPPORTED;
+  }
+
+  HIPCHK (nck::launch_whiten (h->dX.as<double> (), h->dUg.as<double> (), h->dY.as<double> (), (int) h->n_obs, d, s));
+  HIPCHK (hipStreamSynchronize (s));
+
+  return NCKDE_OK;
+}
+
+// VKDE _build_cov_array_kdtree (ncm_stats_dist_vkde.c:362-496)
+static int
+vkde_build_cov_array (nckde *h)
+{
+  const int d   = h->d;
+  hipStream_t s = h->stream;
+  const double kk = (h->local_frac * h->n_obs > 2) ? h->local_frac * h->n_obs : 2;
+
+  h->k = (size_t) kk;
+
+  if (h->k > nck::knn_max_k ())
+  {
+    set_error ("nckde: k = %zu nearest neighbours exceeds the supported maximum %zu", h->k, nck::knn_max_k ());
+    return NCKDE_EUNSUPPORTED;
+  }
+
+  HIPCHK (h->dknn.ensure (sizeof (int) * h->n_k * h->k));
+  HIPCHK (h->dstatus.ensure (sizeof (int) * 4));
+  HIPCHK (hipMemsetAsync (h->dstatus.p, 0, sizeof (int) * 4, s));
+
+  // queries in batches so that the distance keys stay under ~512 MiB
+  const size_t budget = (size_t) 512 << 20;
+  size_t nq           = budget / (sizeof (uint64_t) * h->n_obs);
+
+  nq = std::max<size_t> (1, std::min (nq, h->n_k));
+  HIPCHK (h->dkeys.ensure (sizeof (uint64_t) * h->n_obs * nq));
+
+  for (size_t q0 = 0; q0 < h->n_k; q0 += nq)
+  {
+    const size_t cnt = std::min (nq, h->n_k - q0);
+
+    HIPCHK (nck::launch_knn (h->dY.as<double> (), (int) h->n_obs, d, (int) q0, (int) cnt, (int) h->k,
+                             h->dkeys.as<uint64_t> (), h->dknn.as<int> (), h->dstatus.as<int> (), s));
+  }
+
+  HIPCHK (h->dcov.ensure (sizeof (double) * h->n_k * d * d));
+  HIPCHK (h->dU.ensure (sizeof (double) * h->n_k * d * d));
+  HIPCHK (h->dlnnorm.ensure (sizeof (double) * h->n_k));
+  HIPCHK (nck::launch_welford_cov (h->dX.as<double> (), d, (int) h->k, h->dknn.as<int> (), (int) h->k, (int) h->n_k,
+                                   h->dcov.as<double> (), s));
+
+  int status = 0;
+
+  HIPCHK (hipMemcpyAsync (&status, h->dstatus.p, sizeof (int), hipMemcpyDeviceToHost, s));
+  HIPCHK (hipStreamSynchronize (s));
+
+  if (status != 0)
+  {
+    set_error ("nckde: kNN selection inconsistency (status %d)", status);
+    return NCKDE_ENUMERIC;
+  }
+
+  return factor_batch (h, h->dcov.as<double> (), h->dU.as<double> (), h->dlnnorm.as<double> (), h->n_k);
+}
+
+static int
+refresh_bvec (nckde *h)
+{
+  HIPCHK (h->db.ensure (sizeof (double) * h->n_k));
+  HIPCHK (nck::launch_bvec (h->dw.as<double> (), (h->kind == NCKDE_VKDE) ? h->dlnnorm.as<double> () : nullptr,
+                            (int) h->n_k, h->db.as<double> (), h->stream));
+  h->wcum_ready = false;
+
+  return NCKDE_OK;
+}
+
+// _ncm_stats_dist_prepare (ncm_stats_dist.c:703-789), CV_NONE
+static int
+do_prepare (nckde *h)
+{
+  const int d = h->d;
+
+  h->prepared  = false;
+  h->n_obs     = h->n_samples;
+  h->n_k       = h->n_samples;
+
+  if (h->n_obs <= (size_t) d)
+  {
+    set_error ("_ncm_stats_dist_prepare: the sample is too small.");
+    return NCKDE_ESAMPLE;
+  }
+
+  if (h->kind == NCKDE_VKDE && h->local_frac * h->n_obs < 2)
+  {
+    set_error ("Too few observations.\n\tThe number of observations is too small to use the local covariance "
+               "method.\n\tThe local fraction = %f times number of observations %d is less than 2.",
+               h->local_frac, (int) h->n_obs);
+    return NCKDE_ESAMPLE;
+  }
+
+  if (h->kind == NCKDE_VKDE && h->n_obs <= 2)
+  {
+    set_error ("_ncm_stats_dist_vkde_build_cov_array_kdtree: n_obs must exceed 2.");
+    return NCKDE_ESAMPLE;
+  }
+
+  h->D = nck::padded_dim (d);
+
+  if (h->D < 0)
+  {
+    set_error ("nckde: dimension %d above the supported maximum 64", d);
+    return NCKDE_EUNSUPPORTED;
+  }
+
+  TRY (kde_prepare_kernel (h));
+
+  if (h->kind == NCKDE_VKDE)
+    TRY (vkde_build_cov_array (h));
+
+  h->href = get_href (h);
+
+  {
+    std::vector<double> w (h->n_k, 1.0 / (1.0 * h->n_k));
+
+    HIPCHK (h->dw.ensure (sizeof (double) * h->n_k));
+    HIPCHK (hipMemcpyAsync (h->dw.p, w.data (), sizeof (double) * h->n_k, hipMemcpyHostToDevice, h->stream));
+  }
+
+  HIPCHK (h->dTh.ensure (sizeof (double) * h->n_k * h->D));
+
+  if (h->kind == NCKDE_VKDE)
+  {
+    HIPCHK (h->dWp.ensure (sizeof (double) * h->n_k * (h->D * (h->D + 1) / 2)));
+    HIPCHK (nck::launch_vkde_factors (h->dU.as<double> (), h->dX.as<double> (), (int) h->n_k, d, h->D,
+                                      h->dWp.as<double> (), h->dTh.as<double> (), h->stream));
+  }
+  else
+  {
+    HIPCHK (nck::launch_kde_factors (h->dY.as<double> (), (int) h->n_k, d, h->D, h->dTh.as<double> (), h->stream));
+  }
+
+  TRY (refresh_bvec (h));
+  HIPCHK (h->dwcum.ensure (sizeof (double) * (h->n_k + 1)));
+  HIPCHK (hipStreamSynchronize (h->stream));
+
+  h->rnorm    = 0.0;
+  h->prepared = true;
+
+  return NCKDE_OK;
+}
+
+// fills dAT (n_k x n_obs) with IM^T, optionally scaled by 1/f_j
+static int
+build_IM (nckde *h, const double *dm2lnp, double min_m2lnp)
+{
+  nck::IMArgs a;
+
+  HIPCHK (h->dAT.ensure (sizeof (double) * h->n_k * h->n_obs));
+
+  a.kind        = h->kind;
+  a.kernel      = h->kernel;
+  a.d           = h->d;
+  a.D           = h->D;
+  a.n_obs       = (int) h->n_obs;
+  a.n_k         = (int) h->n_k;
+  a.nu          = h->nu;
+  a.h2          = h->href * h->href;
+  a.inv_h2      = 1.0 / a.h2;
+  a.lnnorm_href = h->d * std::log (h->href);
+  a.kde_scale   = std::exp (-(h->kernel_lnnorm + h->d * std::log (h->href)));
+  a.X           = h->dX.as<double> ();
+  a.Y           = h->dY.as<double> ();
+  a.Th          = h->dTh.as<double> ();
+  a.Wp          = h->dWp.as<double> ();
+  a.lnnorm      = h->dlnnorm.as<double> ();
+  a.m2lnp       = dm2lnp;
+  a.min_m2lnp   = min_m2lnp;
+  a.AT          = h->dAT.as<double> ();
+
+  HIPCHK (nck::launch_build_IM (a, h->stream));
+
+  return NCKDE_OK;
+}
+
+// ---- active-set NNLS (ncm_nnls.c:464-871) --------------------------------
+
+struct NNLS
+{
+  nckde *h;
+  int n, nobs;
+  std::vector<int> Pidx;
+};
+
+// unconstrained solve on the sorted index list P (normal equations:
+// Cholesky, else Gaussian elimination); solution scattered into dxo
+static int
+nnls_unconstrained (nckde *h, int n, const std::vector<int> &P, double *dxo)
+{
+  hipStream_t s = h->stream;
+  const int np  = (int) P.size ();
+  int info      = 0;
+
+  if (np == 0)
+  {
+    HIPCHK (hipMemsetAsync (dxo, 0, sizeof (double) * n, s));
+    return NCKDE_OK;
+  }
+
+  HIPCHK (hipMemcpyAsync (h->dP.p, P.data (), sizeof (int) * np, hipMemcpyHostToDevice, s));
+  HIPCHK (nck::launch_gather_sym (h->dM.as<double> (), n, h->dP.as<int> (), np, h->dS.as<double> (),
+                                  h->dbv.as<double> (), h->dbs.as<double> (), s));
+  HIPCHK (hipMemsetAsync (h->dscal.p, 0, sizeof (int) * 2, s));
+  HIPCHK (nck::launch_potrf (h->dS.as<double> (), np, h->dscal.as<int> (), s));
+  HIPCHK (hipMemcpyAsync (&info, h->dscal.p, sizeof (int), hipMemcpyDeviceToHost, s));
+  HIPCHK (hipStreamSynchronize (s));
+
+  if (info == 0)
+  {
+    HIPCHK (nck::launch_trsv_pair (h->dS.as<double> (), np, h->dbs.as<double> (), s));
+  }
+  else
+  {
+    HIPCHK (nck::launch_gather_sym (h->dM.as<double> (), n, h->dP.as<int> (), np, h->dS.as<double> (),
+                                    h->dbv.as<double> (), h->dbs.as<double> (), s));
+    HIPCHK (nck::launch_lu_solve (h->dS.as<double> (), np, h->dbs.as<double> (), h->dscal.as<int> (), s));
+    HIPCHK (hipMemcpyAsync (&info, h->dscal.p, sizeof (int), hipMemcpyDeviceToHost, s));
+    HIPCHK (hipStreamSynchronize (s));
+
+    if (info != 0)
+    {
+      set_error ("ncm_nnls_solve: singular normal equations (column set of size %d)", np);
+      return NCKDE_ENUMERIC;
+    }
+  }
+
+  HIPCHK (nck::launch_scatter (h->dbs.as<double> (), h->dP.as<int> (), np, n, dxo, s));
+
+  return NCKDE_OK;
+}
+
+// _ncm_nnls_solve_feasible (ncm_nnls.c:728-751)
+static int
+nnls_feasible (nckde *h, int n, std::vector<char> &inP, double *dxo, std::vector<double> &x, int max_remove)
+{
+  std::vector<int> P;
+
+  while (true)
+  {
+    P.clear ();
+
+    for (int a = 0; a < n; ++a)
+      if (inP[a])
+        P.push_back (a);
+
+    TRY (nnls_unconstrained (h, n, P, dxo));
+    HIPCHK (hipMemcpyAsync (x.data (), dxo, sizeof (double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK (hipStreamSynchronize (h->stream));
+
+    std::vector<int> inv;
+
+    for (int a : P)
+      if (x[a] < 1.0e-300)
+        inv.push_back (a);
+
+    if (inv.empty ())
+      break;
+
+    if ((size_t) max_remove >= inv.size ())
+    {
+      for (int a : inv)
+        inP[a] = 0;
+    }
+    else
+    {
+      // ncm_iset_remove_smallest_subset (ncm_iset.c:919-979)
+      std::stable_sort (inv.begin (), inv.end (), [&] (int u, int v) { return x[u] < x[v]; });
+
+      for (int q = 0; q < max_remove; ++q)
+        inP[inv[q]] = 0;
+    }
+  }
+
+  return NCKDE_OK;
+}
+
+static int
+nnls_residual (nckde *h, const double *dxv, double *drv, double &rnorm)
+{
+  const int nb = ((int) h->n_obs + 255) / 256;
+  double r2    = 0.0;
+
+  HIPCHK (nck::launch_residual (h->dAT.as<double> (), (int) h->n_k, (int) h->n_obs, dxv, nullptr, drv,
+                                h->dpart.as<double> (), h->dpart.as<double> () + nb, h->stream));
+  HIPCHK (hipMemcpyAsync (&r2, h->dpart.as<double> () + nb, sizeof (double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK (hipStreamSynchronize (h->stream));
+  rnorm = std::sqrt (r2);
+
+  return NCKDE_OK;
+}
+
+static int
+nnls_mgrad (nckde *h, const double *drv, std::vector<double> &g)
+{
+  HIPCHK (nck::launch_row_dot (h->dAT.as<double> (), (int) h->n_k, (int) h->n_obs, drv, h->dg.as<double> (), h->stream));
+  HIPCHK (hipMemcpyAsync (g.data (), h->dg.p, sizeof (double) * h->n_k, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK (hipStreamSynchronize (h->stream));
+
+  return NCKDE_OK;
+}
+
+// ncm_nnls_solve (ncm_nnls.c:767-871) with A = dAT^T, f = 1; writes dw
+static int
+nnls_solve (nckde *h)
+{
+  const int n      = (int) h->n_k;
+  const int nobs   = (int) h->n_obs;
+  const double reltol = 2.220446049250313e-16;
+  hipStream_t s    = h->stream;
+
+  HIPCHK (h->dM.ensure (sizeof (double) * n * n));
+  HIPCHK (h->dS.ensure (sizeof (double) * n * n));
+  HIPCHK (h->dbv.ensure (sizeof (double) * n));
+  HIPCHK (h->dbs.ensure (sizeof (double) * n));
+  HIPCHK (h->dx.ensure (sizeof (double) * n));
+  HIPCHK (h->dxt.ensure (sizeof (double) * n));
+  HIPCHK (h->dr.ensure (sizeof (double) * nobs));
+  HIPCHK (h->drt.ensure (sizeof (double) * nobs));
+  HIPCHK (h->dg.ensure (sizeof (double) * n));
+  HIPCHK (h->dP.ensure (sizeof (int) * n));
+  HIPCHK (h->dpart.ensure (sizeof (double) * ((nobs + 255) / 256 + 4)));
+  HIPCHK (h->dscal.ensure (64));
+
+  // M = A^T A, b = A^T f
+  HIPCHK (nck::launch_gram_nt (h->dAT.as<double> (), n, nobs, h->dM.as<double> (), s));
+  HIPCHK (nck::launch_row_dot (h->dAT.as<double> (), n, nobs, nullptr, h->dbv.as<double> (), s));
+
+  std::vector<char> P (n, 1), Ptry (n);
+  std::vector<double> x (n), xt (n), g (n);
+  double rnorm, lrnorm;
+
+  TRY (nnls_feasible (h, n, P, h->dx.as<double> (), x, n));
+  TRY (nnls_residual (h, h->dx.as<double> (), h->dr.as<double> (), rnorm));
+  TRY (nnls_mgrad (h, h->dr.as<double> (), g));
+
+  while (true)
+  {
+    double add_frac = 1.0;
+    bool finish     = false;
+
+    while (true)
+    {
+      // ncm_iset_add_largest_subset (ncm_iset.c:993-1078)
+      const double gmin = rnorm * reltol;
+      std::vector<int> cand;
+
+      Ptry = P;
+
+      for (int j = 0; j < n; ++j)
+        if (!Ptry[j] && g[j] > gmin)
+          cand.push_back (j);
+
+      const unsigned kc = (unsigned) cand.size ();
+      unsigned added    = 0;
+
+      if (kc > 0)
+      {
+        const double kf = kc * add_frac;
+        const double mx = (kf > 1) ? kf : 1;
+
+        added = (unsigned) ((kc < mx) ? kc : mx);
+      }
+
+      add_frac *= 0.5;
+
+      if (added == 0)
+      {
+        finish = true;
+        break;
+      }
+
+      std::stable_sort (cand.begin (), cand.end (), [&] (int u, int v) { return g[u] > g[v]; });
+
+      for (unsigned q = 0; q < added; ++q)
+        Ptry[cand[q]] = 1;
+
+      TRY (nnls_feasible (h, n, Ptry, h->dxt.as<double> (), xt, (int) added));
+      TRY (nnls_residual (h, h->dxt.as<double> (), h->drt.as<double> (), lrnorm));
+
+      if (rnorm - lrnorm > rnorm * reltol)
+      {
+        P = Ptry;
+        HIPCHK (hipMemcpyAsync (h->dx.p, h->dxt.p, sizeof (double) * n, hipMemcpyDeviceToDevice, s));
+        HIPCHK (hipMemcpyAsync (h->dr.p, h->drt.p, sizeof (double) * nobs, hipMemcpyDeviceToDevice, s));
+        rnorm = lrnorm;
+        break;
+      }
+
+      if (added == 1)
+      {
+        finish = true;
+        break;
+      }
+    }
+
+    if (finish)
+      break;
+
+    TRY (nnls_mgrad (h, h->dr.as<double> (), g));
+  }
+
+  HIPCHK (hipMemcpyAsync (h->dw.p, h->dx.p, sizeof (double) * n, hipMemcpyDeviceToDevice, s));
+  h->rnorm = rnorm;
+
+  return NCKDE_OK;
+}
+
+// _ncm_stats_dist_prepare_interp (ncm_stats_dist.c:878-1094), CV_NONE
+static int
+do_prepare_interp (nckde *h, const double *m2lnp, size_t len)
+{
+  const double limit = -2.0 * 2.0 * nck::LOG_DBL_EPSILON;
+
+  TRY (do_prepare (h));
+
+  if (len != h->n_obs)
+  {
+    set_error ("ncm_stats_dist_prepare_interp: m2lnp has %zu entries, the sample has %zu.", len, h->n_obs);
+    return NCKDE_EINVAL;
+  }
+
+  double mn = INFINITY, mx = -INFINITY;
+
+  for (size_t i = 0; i < h->n_k; ++i)
+  {
+    mn = std::min (mn, m2lnp[i]);
+    mx = std::max (mx, m2lnp[i]);
+  }
+
+  if (mx - mn > limit)
+  {
+    std::vector<size_t> srt (h->n_k);
+
+    std::iota (srt.begin (), srt.end (), 0);
+    std::stable_sort (srt.begin (), srt.end (), [&] (size_t a, size_t b) { return m2lnp[a] < m2lnp[b]; });
+
+    size_t n_cut = 0;
+
+    for (size_t i = 0; i < h->n_k; ++i)
+      if (m2lnp[srt[i]] - mn > limit)
+      {
+        n_cut = i;
+        break;
+      }
+
+    if (n_cut < (size_t) (unsigned) (0.5 * h->n_obs))
+    {
+      // 90% of the weight inside the window, 10% outside; no shrink
+      std::vector<double> w (h->n_k, 0.1 / (h->n_k - n_cut));
+
+      for (size_t i = 0; i < n_cut; ++i)
+        w[srt[i]] = 0.9 / n_cut;
+
+      HIPCHK (hipMemcpyAsync (h->dw.p, w.data (), sizeof (double) * h->n_k, hipMemcpyHostToDevice, h->stream));
+      TRY (refresh_bvec (h));
+      HIPCHK (hipStreamSynchronize (h->stream));
+
+      return NCKDE_OK;
+    }
+
+    // drop the points outside the window and refit on the rest
+    std::vector<double> Xc, mc;
+
+    Xc.reserve (n_cut * h->d);
+    mc.reserve (n_cut);
+
+    for (size_t i = 0; i < h->n_obs; ++i)
+      if (m2lnp[i] - mn <= limit)
+      {
+        mc.push_back (m2lnp[i]);
+        Xc.insert (Xc.end (), h->X.begin () + i * h->d, h->X.begin () + (i + 1) * h->d);
+      }
+
+    h->X         = Xc;
+    h->n_samples = mc.size ();
+
+    return do_prepare_interp (h, mc.data (), mc.size ());
+  }
+
+  if (h->n_k > 20000)
+    fprintf (stderr, "** WARNING: _ncm_stats_dist_prepare_interp: very large system n = %u!\n", (unsigned) h->n_k);
+
+  HIPCHK (h->dm2lnp.ensure (sizeof (double) * h->n_obs));
+  HIPCHK (hipMemcpyAsync (h->dm2lnp.p, m2lnp, sizeof (double) * h->n_obs, hipMemcpyHostToDevice, h->stream));
+  TRY (build_IM (h, h->dm2lnp.as<double> (), mn));
+  TRY (nnls_solve (h));
+
+  HIPCHK (h->dstatus.ensure (sizeof (int) * 4));
+  HIPCHK (hipMemsetAsync (h->dstatus.p, 0, sizeof (int) * 4, h->stream));
+  HIPCHK (nck::launch_shrink (h->dw.as<double> (), (int) h->n_k, h->shrink, h->dstatus.as<int> (), h->stream));
+
+  int status = 0;
+
+  HIPCHK (hipMemcpyAsync (&status, h->dstatus.p, sizeof (int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK (hipStreamSynchronize (h->stream));
+
+  if (status != 0)
+  {
+    set_error ("_ncm_stats_dist_prepare_interp: total weight is not positive.");
+    return NCKDE_ENUMERIC;
+  }
+
+  TRY (refresh_bvec (h));
+  HIPCHK (hipStreamSynchronize (h->stream));
+
+  return NCKDE_OK;
+}
+
+static int
+eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, int linear, hipStream_t s)
+{
+  if (m == 0)
+    return NCKDE_OK;
+
+  if (precision == NCKDE_PREC_MIXED)
+  {
+    set_error ("nckde: the mixed-precision path is not built in this library version");
+    return NCKDE_EUNSUPPORTED;
+  }
+
+  if (precision != NCKDE_PREC_FP64)
+  {
+    set_error ("nckde: unknown precision %d", precision);
+    return NCKDE_EINVAL;
+  }
+
+  const int nblk = (int) ((m + 255) / 256);
+  int nsplit     = (2048 + nblk - 1) / nblk;
+  const int maxs = (int) ((h->n_k + 63) / 64);
+
+  nsplit = std::max (1, std::min (nsplit, maxs));
+
+  // the partial buffers belong to the handle: work queued on another stream
+  // must finish before this call reuses them
+  if (h->last_stream != nullptr && h->last_stream != s)
+    HIPCHK (hipStreamSynchronize (h->last_stream));
+
+  h->last_stream = s;
+
+  HIPCHK (h->dpartM.ensure (sizeof (double) * m * nsplit));
+  HIPCHK (h->dpartS.ensure (sizeof (double) * m * nsplit));
+
+  nck::EvalArgs a;
+
+  a.kind          = h->kind;
+  a.kernel        = h->kernel;
+  a.d             = h->d;
+  a.D             = h->D;
+  a.n             = (int) h->n_k;
+  a.m             = (int) m;
+  a.nu            = h->nu;
+  a.kappa         = -0.5 * (h->nu + (double) (unsigned) h->d);
+  a.inv_h2        = 1.0 / (h->href * h->href);
+  a.lnh_d         = h->d * std::log (h->href);
+  a.lnnorm_shared = (h->kind == NCKDE_KDE) ? h->kernel_lnnorm : 0.0;
+  a.Th            = h->dTh.as<double> ();
+  a.Wp            = h->dWp.as<double> ();
+  a.b             = h->db.as<double> ();
+  a.Ug            = h->dUg.as<double> ();
+  a.X             = dX;
+  a.partM         = h->dpartM.as<double> ();
+  a.partS         = h->dpartS.as<double> ();
+  a.nsplit        = nsplit;
+  a.out           = dout;
+  a.linear        = linear;
+
+  HIPCHK (nck::launch_eval_f64 (a, s));
+  HIPCHK (nck::launch_eval_merge (a, s));
+
+  return NCKDE_OK;
+}
+
+static int
+eval_host (nckde *h, const double *X, size_t m, size_t ld, double *out, int precision, int linear)
+{
+  const int d = h->d;
+
+  if (m == 0)
+    return NCKDE_OK;
+
+  if (X == nullptr || out == nullptr || ld < (size_t) d)
+  {
+    set_error ("nckde_eval: invalid arguments");
+    return NCKDE_EINVAL;
+  }
+
+  HIPCHK (h->dXe.ensure (sizeof (double) * m * d));
+  HIPCHK (h->dout.ensure (sizeof (double) * m));
+
+  if (ld == (size_t) d)
+    HIPCHK (hipMemcpyAsync (h->dXe.p, X, sizeof (double) * m * d, hipMemcpyHostToDevice, h->stream));
+  else
+    HIPCHK (hipMemcpy2DAsync (h->dXe.p, sizeof (double) * d, X, sizeof (double) * ld, sizeof (double) * d, m,
+                              hipMemcpyHostToDevice, h->stream));
+
+  TRY (eval_device (h, h->dXe.as<double> (), m, h->dout.as<double> (), precision, linear, h->stream));
+  HIPCHK (hipMemcpyAsync (out, h->dout.p, sizeof (double) * m, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK (hipStreamSynchronize (h->stream));
+
+  return NCKDE_OK;
+}
+
+static int
+ensure_wcum (nckde *h)
+{
+  if (!h->wcum_ready)
+  {
+    HIPCHK (nck::launch_wcum (h->dw.as<double> (), (int) h->n_k, h->dwcum.as<double> (), h->stream));
+    h->wcum_ready = true;
+  }
+
+  return NCKDE_OK;
+}
+
+// ========================================================================
+// C-ABI
+// ========================================================================
+extern "C" {
+
+const char *
+nckde_last_error (void)
+{
+  return nck::g_err;
+}
+
+int
+nckde_create (nckde **out, int kind, int kernel, double nu, unsigned dim, int cv_type)
+{
+  if (out == nullptr)
+  {
+    set_error ("nckde_create: NULL output pointer");
+    return NCKDE_EINVAL;
+  }
+
+  *out = nullptr;
+
+  if ((kind != NCKDE_KDE && kind != NCKDE_VKDE) || (kernel != NCKDE_KERNEL_GAUSS && kernel != NCKDE_KERNEL_ST) ||
+      dim == 0)
+  {
+    set_error ("nckde_create: invalid kind/kernel/dimension");
+    return NCKDE_EINVAL;
+  }
+
+  if (kernel == NCKDE_KERNEL_ST && !(nu >= 1.0))
+  {
+    set_error ("nckde_create: Student-t nu must be >= 1");
+    return NCKDE_EINVAL;
+  }
+
+  if (cv_type != NCKDE_CV_NONE)
+  {
+    set_error ("nckde_create: only NCM_STATS_DIST_CV_NONE is implemented");
+    return NCKDE_EUNSUPPORTED;
+  }
+
+  int ndev = 0;
+
+  if (hipGetDeviceCount (&ndev) != hipSuccess || ndev == 0)
+  {
+    set_error ("nckde_create: no HIP device available");
+    return NCKDE_EHIP;
+  }
+
+  nckde *h = new nckde ();
+
+  h->kind   = kind;
+  h->kernel = kernel;
+  h->cv     = cv_type;
+  h->nu     = (kernel == NCKDE_KERNEL_ST) ? nu : 0.0;
+  h->d      = (int) dim;
+
+  if (hipStreamCreateWithFlags (&h->stream, hipStreamNonBlocking) != hipSuccess)
+  {
+    set_error ("nckde_create: hipStreamCreate failed");
+    delete h;
+    return NCKDE_EHIP;
+  }
+
+  *out = h;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_destroy (nckde *h)
+{
+  if (h == nullptr)
+    return NCKDE_OK;
+
+  if (h->stream)
+  {
+    (void) hipStreamSynchronize (h->stream);
+    (void) hipStreamDestroy (h->stream);
+  }
+
+  delete h;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_over_smooth (nckde *h, double v)
+{
+  TRY (check_handle (h));
+
+  if (!(v >= 1.0e-5))
+  {
+    set_error ("nckde_set_over_smooth: value must be >= 1e-5");
+    return NCKDE_EINVAL;
+  }
+
+  h->over_smooth = v;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_shrink (nckde *h, double v)
+{
+  TRY (check_handle (h));
+
+  if (!(v >= 0.0 && v <= 1.0))
+  {
+    set_error ("nckde_set_shrink: shrink must be in [0, 1]");
+    return NCKDE_EINVAL;
+  }
+
+  h->shrink = v;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_local_frac (nckde *h, double v)
+{
+  TRY (check_handle (h));
+
+  if (!(v >= 0.001 && v <= 1.0))
+  {
+    set_error ("nckde_set_local_frac: local_frac must be in [0.001, 1]");
+    return NCKDE_EINVAL;
+  }
+
+  h->local_frac = v;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_use_rot_href (nckde *h, int v)
+{
+  TRY (check_handle (h));
+  h->use_rot_href = v ? 1 : 0;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_nearPD_maxiter (nckde *h, unsigned v)
+{
+  TRY (check_handle (h));
+  h->nearPD_maxiter = v;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_use_threads (nckde *h, int v)
+{
+  (void) v;
+
+  return check_handle (h);
+}
+
+// ncm_stats_dist_kde_set_cov_type / _set_cov_fixed (ncm_stats_dist_kde.c:
+// 731-745, 770-791): a fixed covariance is factorised when it is set and
+// must be positive definite.
+static int
+decompose_fixed (nckde *h)
+{
+  const int d = h->d;
+  DBuf dc, du, dl, di;
+  int info      = 0;
+  hipStream_t s = h->stream;
+
+  HIPCHK (dc.ensure (sizeof (double) * d * d));
+  HIPCHK (du.ensure (sizeof (double) * d * d));
+  HIPCHK (dl.ensure (sizeof (double)));
+  HIPCHK (di.ensure (sizeof (int)));
+  HIPCHK (hipMemcpyAsync (dc.p, h->cov_fixed.data (), sizeof (double) * d * d, hipMemcpyHostToDevice, s));
+  HIPCHK (nck::launch_chol_batched (dc.as<double> (), du.as<double> (), di.as<int> (), dl.as<double> (), d, 1,
+                                    h->kernel, (h->kernel == NCKDE_KERNEL_ST) ? st_lnnorm_const (h) : 0.0, s));
+  h->cov_fixed_decomp.resize (d * d);
+  HIPCHK (hipMemcpyAsync (&info, di.p, sizeof (int), hipMemcpyDeviceToHost, s));
+  HIPCHK (hipMemcpyAsync (h->cov_fixed_decomp.data (), du.p, sizeof (double) * d * d, hipMemcpyDeviceToHost, s));
+  HIPCHK (hipMemcpyAsync (&h->cov_fixed_lnnorm, dl.p, sizeof (double), hipMemcpyDeviceToHost, s));
+  HIPCHK (hipStreamSynchronize (s));
+
+  if (info != 0)
+  {
+    set_error ("ncm_stats_dist_kde_set_cov_fixed: matrix cov_fixed is not positive definite.");
+    return NCKDE_ENUMERIC;
+  }
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_cov_type (nckde *h, int cov_type)
+{
+  TRY (check_handle (h));
+
+  if (cov_type < NCKDE_COV_SAMPLE || cov_type > NCKDE_COV_ROBUST)
+  {
+    set_error ("nckde_set_cov_type: invalid covariance type");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  h->cov_type = cov_type;
+
+  if (cov_type == NCKDE_COV_FIXED && h->has_cov_fixed)
+    return decompose_fixed (h);
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_cov_fixed (nckde *h, const double *cov)
+{
+  TRY (check_handle (h));
+
+  if (cov == nullptr)
+  {
+    set_error ("nckde_set_cov_fixed: NULL matrix");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  h->cov_fixed.assign (cov, cov + h->d * h->d);
+  h->has_cov_fixed = true;
+
+  if (h->cov_type == NCKDE_COV_FIXED)
+    return decompose_fixed (h);
+
+  return NCKDE_OK;
+}
+
+int
+nckde_add_obs (nckde *h, const double *x)
+{
+  return nckde_add_obs_batch (h, x, 1, h ? (size_t) h->d : 0);
+}
+
+int
+nckde_add_obs_batch (nckde *h, const double *X, size_t n, size_t ld)
+{
+  TRY (check_handle (h));
+
+  if ((X == nullptr && n > 0) || ld < (size_t) h->d)
+  {
+    set_error ("nckde_add_obs: invalid arguments");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  for (size_t i = 0; i < n; ++i)
+    h->X.insert (h->X.end (), X + i * ld, X + i * ld + h->d);
+
+  h->n_samples += n;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_reset (nckde *h)
+{
+  TRY (check_handle (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  h->X.clear ();
+  h->n_samples = 0;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_prepare (nckde *h)
+{
+  TRY (check_handle (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  return do_prepare (h);
+}
+
+int
+nckde_prepare_interp (nckde *h, const double *m2lnp, size_t n)
+{
+  TRY (check_handle (h));
+
+  if (m2lnp == nullptr)
+  {
+    set_error ("nckde_prepare_interp: NULL m2lnp");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  return do_prepare_interp (h, m2lnp, n);
+}
+
+int
+nckde_compute_IM (nckde *h, double *IM)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+  DBuf tmp;
+
+  TRY (build_IM (h, nullptr, 0.0));
+  HIPCHK (tmp.ensure (sizeof (double) * h->n_obs * h->n_k));
+  HIPCHK (nck::launch_transpose (h->dAT.as<double> (), (int) h->n_k, (int) h->n_obs, tmp.as<double> (), h->stream));
+  HIPCHK (hipMemcpyAsync (IM, tmp.p, sizeof (double) * h->n_obs * h->n_k, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK (hipStreamSynchronize (h->stream));
+
+  return NCKDE_OK;
+}
+
+int
+nckde_eval_m2lnp (nckde *h, const double *X, size_t m, size_t ld, double *out, int precision)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  return eval_host (h, X, m, ld, out, precision, 0);
+}
+
+int
+nckde_eval (nckde *h, const double *X, size_t m, size_t ld, double *out, int precision)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  return eval_host (h, X, m, ld, out, precision, 1);
+}
+
+int
+nckde_eval_m2lnp_device (nckde *h, const double *dX, size_t m, double *dout, int precision, void *stream)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  return eval_device (h, dX, m, dout, precision, 0, stream ? (hipStream_t) stream : h->stream);
+}
+
+int
+nckde_kernel_choose (nckde *h, const double *u, size_t m, uint32_t *idx)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  if (m == 0)
+    return NCKDE_OK;
+
+  TRY (ensure_wcum (h));
+  HIPCHK (h->du.ensure (sizeof (double) * m));
+  HIPCHK (h->didx.ensure (sizeof (uint32_t) * m));
+  HIPCHK (hipMemcpyAsync (h->du.p, u, sizeof (double) * m, hipMemcpyHostToDevice, h->stream));
+  HIPCHK (nck::launch_choose (h->dwcum.as<double> (), (int) h->n_k, h->du.as<double> (), (int) m,
+                              h->didx.as<uint32_t> (), h->stream));
+  HIPCHK (hipMemcpyAsync (idx, h->didx.p, sizeof (uint32_t) * m, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK (hipStreamSynchronize (h->stream));
+
+  return NCKDE_OK;
+}
+
+int
+nckde_sample (nckde *h, const double *u, const double *z, const double *chi2, size_t m, double *out, uint32_t *idx)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+  const int d = h->d;
+
+  if (m == 0)
+    return NCKDE_OK;
+
+  if (h->kernel == NCKDE_KERNEL_ST && chi2 == nullptr)
+  {
+    set_error ("nckde_sample: the Student-t kernel needs chi2 draws");
+    return NCKDE_EINVAL;
+  }
+
+  TRY (ensure_wcum (h));
+  HIPCHK (h->du.ensure (sizeof (double) * m));
+  HIPCHK (h->dz.ensure (sizeof (double) * m * d));
+  HIPCHK (h->dchi2.ensure (sizeof (double) * m));
+  HIPCHK (h->dout.ensure (sizeof (double) * m * d));
+  HIPCHK (h->didx.ensure (sizeof (uint32_t) * m));
+  HIPCHK (hipMemcpyAsync (h->du.p, u, sizeof (double) * m, hipMemcpyHostToDevice, h->stream));
+  HIPCHK (hipMemcpyAsync (h->dz.p, z, sizeof (double) * m * d, hipMemcpyHostToDevice, h->stream));
+
+  if (chi2)
+    HIPCHK (hipMemcpyAsync (h->dchi2.p, chi2, sizeof (double) * m, hipMemcpyHostToDevice, h->stream));
+
+  HIPCHK (nck::launch_sample (h->dwcum.as<double> (), (int) h->n_k, h->du.as<double> (), h->dz.as<double> (),
+                              h->dchi2.as<double> (), (int) m, d, h->kind, h->kernel, h->nu, h->href,
+                              (h->kind == NCKDE_VKDE) ? h->dU.as<double> () : h->dUg.as<double> (),
+                              h->dX.as<double> (), h->dout.as<double> (), h->didx.as<uint32_t> (), h->stream));
+  HIPCHK (hipMemcpyAsync (out, h->dout.p, sizeof (double) * m * d, hipMemcpyDeviceToHost, h->stream));
+
+  if (idx)
+    HIPCHK (hipMemcpyAsync (idx, h->didx.p, sizeof (uint32_t) * m, hipMemcpyDeviceToHost, h->stream));
+
+  HIPCHK (hipStreamSynchronize (h->stream));
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_dim (nckde *h, unsigned *dim)
+{
+  TRY (check_handle (h));
+  *dim = (unsigned) h->d;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_sample_size (nckde *h, size_t *n)
+{
+  TRY (check_handle (h));
+  *n = h->n_obs;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_n_kernels (nckde *h, size_t *n)
+{
+  TRY (check_handle (h));
+  *n = h->n_k;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_n_samples (nckde *h, size_t *n)
+{
+  TRY (check_handle (h));
+  *n = h->n_samples;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_href (nckde *h, double *href)
+{
+  TRY (check_prepared (h));
+  *href = h->href;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_rnorm (nckde *h, double *rnorm)
+{
+  TRY (check_prepared (h));
+  *rnorm = h->rnorm * h->rnorm;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_lnnorm (nckde *h, size_t i, double *lnnorm)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  if (i >= h->n_k)
+  {
+    set_error ("nckde_get_lnnorm: index out of range");
+    return NCKDE_EINVAL;
+  }
+
+  double l = h->kernel_lnnorm;
+
+  if (h->kind == NCKDE_VKDE)
+    HIPCHK (hipMemcpy (&l, h->dlnnorm.as<double> () + i, sizeof (double), hipMemcpyDeviceToHost));
+
+  *lnnorm = l + h->d * std::log (h->href);
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_weights (nckde *h, double *w)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+  HIPCHK (hipMemcpy (w, h->dw.p, sizeof (double) * h->n_k, hipMemcpyDeviceToHost));
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_cov_decomp (nckde *h, size_t i, double *U)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+  const size_t dd = (size_t) h->d * h->d;
+
+  if (i >= h->n_k)
+  {
+    set_error ("nckde_get_cov_decomp: index out of range");
+    return NCKDE_EINVAL;
+  }
+
+  if (h->kind == NCKDE_VKDE)
+    HIPCHK (hipMemcpy (U, h->dU.as<double> () + i * dd, sizeof (double) * dd, hipMemcpyDeviceToHost));
+  else
+    HIPCHK (hipMemcpy (U, h->dUg.p, sizeof (double) * dd, hipMemcpyDeviceToHost));
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_full_cov (nckde *h, double *cov)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+  HIPCHK (hipMemcpy (cov, h->dcovg.p, sizeof (double) * h->d * h->d, hipMemcpyDeviceToHost));
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_full_cov_decomp (nckde *h, double *U)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+  HIPCHK (hipMemcpy (U, h->dUg.p, sizeof (double) * h->d * h->d, hipMemcpyDeviceToHost));
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_sample (nckde *h, double *X)
+{
+  TRY (check_handle (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+  std::memcpy (X, h->X.data (), sizeof (double) * h->n_samples * h->d);
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_invUsample (nckde *h, double *Y)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+  HIPCHK (hipMemcpy (Y, h->dY.p, sizeof (double) * h->n_obs * h->d, hipMemcpyDeviceToHost));
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_knn (nckde *h, size_t *k, int32_t *idx)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  if (h->kind != NCKDE_VKDE)
+  {
+    *k = 0;
+    return NCKDE_OK;
+  }
+
+  *k = h->k;
+
+  if (idx)
+    HIPCHK (hipMemcpy (idx, h->dknn.p, sizeof (int32_t) * h->n_k * h->k, hipMemcpyDeviceToHost));
+
+  return NCKDE_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,225 @@
+// k_eval.hip -- FP64 eval_m2lnp / eval for KDE and VKDE.
+//
+// One lane per evaluation point; every lane of a block walks the same kernel
+// range, so the kernel factors (packed W_i = U_i^{-T}, centre, b_i) are
+// wave-uniform and come in through scalar loads straight into SGPR operands
+// of the FP64 FMAs.  Per pair (VKDE, padded dim D): D subtractions,
+// D(D+1)/2 FMAs for z = W_i (x - theta_i), D FMAs for |z|^2, one exp (two
+// transcendentals for Student-t).  The log-sum-exp runs online in FP64 with
+// a running maximum.  The kernel range is split over blockIdx.y so that small
+// point counts still fill the 256 CUs; partial (max, sum) pairs are merged in
+// split order by k_eval_merge.
+//
+// Reference: ncm_stats_dist_vkde.c:681-723 and ncm_stats_dist_kde.c:639-681
+// (eval_weights_m2lnp), the sum0/sum1 reductions of
+// ncm_stats_dist_kernel_gauss.c:246-333 / ncm_stats_dist_kernel_st.c:295-386
+// and the linear eval_weights (vkde.c:631-679, kde.c:596-637).
+#include "nckde_internal.h"
+
+namespace nck
+{
+
+template <int D, int KERNEL, int KIND>
+__global__ void __launch_bounds__ (256)
+k_eval_f64 (EvalArgs a)
+{
+  const int p     = blockIdx.x * 256 + threadIdx.x;
+  const int chunk = (a.n + a.nsplit - 1) / a.nsplit;
+  const int i0    = blockIdx.y * chunk;
+  const int i1    = min (a.n, i0 + chunk);
+  const int pp    = (p < a.m) ? p : (a.m - 1);
+  const int d     = a.d;
+  double x[D];
+
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    x[k] = (k < d) ? a.X[(size_t) pp * d + k] : 0.0;
+
+  if (KIND == NCKDE_KDE)
+  {
+    // v = U_g^{-T} x (gsl_blas_dtrsv Upper/Trans, kde.c:648-652)
+    double v[D];
+
+#pragma unroll
+    for (int l = 0; l < D; ++l)
+    {
+      if (l < d)
+      {
+        double t = x[l];
+
+#pragma unroll
+        for (int m = 0; m < l; ++m)
+          t = fma (-v[m], a.Ug[m * d + l], t);
+
+        v[l] = t / a.Ug[l * d + l];
+      }
+      else
+      {
+        v[l] = 0.0;
+      }
+    }
+
+#pragma unroll
+    for (int l = 0; l < D; ++l)
+      x[l] = v[l];
+  }
+
+  double M = -INFINITY, S = 0.0;
+
+  for (int i = i0; i < i1; ++i)
+  {
+    const double *th = a.Th + (size_t) i * D;
+    double chi2      = 0.0;
+
+    if (KIND == NCKDE_VKDE)
+    {
+      const double *W = a.Wp + (size_t) i * (D * (D + 1) / 2);
+      double delta[D];
+
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+        delta[k] = x[k] - th[k];
+
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+      {
+        double z = 0.0;
+
+#pragma unroll
+        for (int c = 0; c <= r; ++c)
+          z = fma (W[r * (r + 1) / 2 + c], delta[c], z);
+
+        chi2 = fma (z, z, chi2);
+      }
+    }
+    else
+    {
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+      {
+        const double t = th[k] - x[k];
+
+        chi2 = fma (t, t, chi2);
+      }
+    }
+
+    chi2 *= a.inv_h2;
+
+    double lnt;
+
+    if (KERNEL == NCKDE_KERNEL_GAUSS)
+      lnt = fma (-0.5, chi2, a.b[i]);
+    else
+      lnt = fma (a.kappa, log1p (chi2 / a.nu), a.b[i]);
+
+    if (lnt > M)
+    {
+      S = S * exp (M - lnt);
+      M = lnt;
+    }
+
+    {
+      const double e = exp (lnt - M);
+
+      S += (lnt > -INFINITY) ? e : 0.0;
+    }
+  }
+
+  if (p < a.m)
+  {
+    a.partM[(size_t) blockIdx.y * a.m + p] = M;
+    a.partS[(size_t) blockIdx.y * a.m + p] = S;
+  }
+}
+
+// gamma = max over splits, 1 + lambda = sum_s S_s exp(M_s - gamma);
+// m2lnp = -2 (gamma - lnnorm_shared + log1p(lambda) - d ln h)
+__global__ void
+k_eval_merge (EvalArgs a)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+
+  if (p >= a.m)
+    return;
+
+  double M = -INFINITY;
+
+  for (int s = 0; s < a.nsplit; ++s)
+    M = fmax (M, a.partM[(size_t) s * a.m + p]);
+
+  double S = 0.0;
+
+  for (int s = 0; s < a.nsplit; ++s)
+  {
+    const double Ms = a.partM[(size_t) s * a.m + p];
+
+    if (Ms > -INFINITY)
+      S += a.partS[(size_t) s * a.m + p] * exp (Ms - M);
+  }
+
+  const double m2lnp = -2.0 * ((M - a.lnnorm_shared) + log1p (S - 1.0) - a.lnh_d);
+
+  a.out[p] = a.linear ? exp (-0.5 * m2lnp) : m2lnp;
+}
+
+template <int D, int KERNEL, int KIND>
+static hipError_t
+launch_t (const EvalArgs &a, hipStream_t s)
+{
+  dim3 grid ((a.m + 255) / 256, a.nsplit);
+
+  hipLaunchKernelGGL ((k_eval_f64<D, KERNEL, KIND>), grid, dim3 (256), 0, s, a);
+
+  return hipGetLastError ();
+}
+
+template <int D>
+static hipError_t
+launch_d (const EvalArgs &a, hipStream_t s)
+{
+  if (a.kind == NCKDE_VKDE)
+    return (a.kernel == NCKDE_KERNEL_GAUSS) ? launch_t<D, NCKDE_KERNEL_GAUSS, NCKDE_VKDE> (a, s)
+                                            : launch_t<D, NCKDE_KERNEL_ST, NCKDE_VKDE> (a, s);
+
+  return (a.kernel == NCKDE_KERNEL_GAUSS) ? launch_t<D, NCKDE_KERNEL_GAUSS, NCKDE_KDE> (a, s)
+                                          : launch_t<D, NCKDE_KERNEL_ST, NCKDE_KDE> (a, s);
+}
+
+int
+padded_dim (int d)
+{
+  static const int dims[] = {4, 8, 12, 16, 24, 32, 48, 64};
+
+  for (int D : dims)
+    if (d <= D)
+      return D;
+
+  return -1;
+}
+
+hipError_t
+launch_eval_f64 (const EvalArgs &a, hipStream_t s)
+{
+  switch (a.D)
+  {
+    case 4: return launch_d<4> (a, s);
+    case 8: return launch_d<8> (a, s);
+    case 12: return launch_d<12> (a, s);
+    case 16: return launch_d<16> (a, s);
+    case 24: return launch_d<24> (a, s);
+    case 32: return launch_d<32> (a, s);
+    case 48: return launch_d<48> (a, s);
+    case 64: return launch_d<64> (a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t
+launch_eval_merge (const EvalArgs &a, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_eval_merge, dim3 ((a.m + 255) / 256), dim3 (256), 0, s, a);
+
+  return hipGetLastError ();
+}
+
+} // namespace nck

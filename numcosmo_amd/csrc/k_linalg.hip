@@ -1,0 +1,846 @@
+// k_linalg.hip -- interpolation matrix and the FP64 linear algebra of the
+// active-set NNLS weight fit.
+//
+// The interpolation matrix is stored column-major, AT[i][j] = A[j][i]
+// (kernel i, observation j), so that building it, scaling its rows and both
+// matrix-vector products of the NNLS loop read and write unit-stride rows.
+//
+// Reference: _ncm_stats_dist_compute_IM_full (ncm_stats_dist.c:791-804),
+// compute_IM (ncm_stats_dist_kde.c:492-557, ncm_stats_dist_vkde.c:517-606),
+// ncm_nnls_solve and its normal-equation solvers (ncm_nnls.c:464-871).
+#include "nckde_internal.h"
+
+namespace nck
+{
+
+// --------------------------------------------------------------------------
+// A[j][i] = K(chi2_ij) / (u_i h^d) / f_j,  f_j = exp(-(m2lnp_j - min)/2)
+// --------------------------------------------------------------------------
+template <int D, int KERNEL, int KIND>
+__global__ void __launch_bounds__ (256)
+k_build_IM (IMArgs a)
+{
+  const int j     = blockIdx.x * 256 + threadIdx.x;
+  const int chunk = (a.n_k + gridDim.y - 1) / gridDim.y;
+  const int i0    = blockIdx.y * chunk;
+  const int i1    = min (a.n_k, i0 + chunk);
+  const int jj    = (j < a.n_obs) ? j : (a.n_obs - 1);
+  const int d     = a.d;
+  double xj[D];
+
+  if (KIND == NCKDE_VKDE)
+  {
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      xj[k] = (k < d) ? a.X[(size_t) jj * d + k] : 0.0;
+  }
+  else
+  {
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      xj[k] = (k < d) ? a.Y[(size_t) jj * d + k] : 0.0;
+  }
+
+  double rowscale = 1.0;
+
+  if (a.m2lnp)
+    rowscale = 1.0 / exp (-0.5 * (a.m2lnp[jj] - a.min_m2lnp));
+
+  const double kappa = -0.5 * (a.nu + (double) (unsigned) d);
+
+  for (int i = i0; i < i1; ++i)
+  {
+    const double *th = a.Th + (size_t) i * D;
+    double chi2      = 0.0;
+    double scale;
+
+    if (KIND == NCKDE_VKDE)
+    {
+      const double *W = a.Wp + (size_t) i * (D * (D + 1) / 2);
+      double delta[D];
+
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+        delta[k] = xj[k] - th[k];
+
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+      {
+        double z = 0.0;
+
+#pragma unroll
+        for (int c = 0; c <= r; ++c)
+          z = fma (W[r * (r + 1) / 2 + c], delta[c], z);
+
+        chi2 = fma (z, z, chi2);
+      }
+
+      chi2  = chi2 * a.inv_h2;
+      scale = 1.0 / exp (a.lnnorm[i] + a.lnnorm_href);
+    }
+    else
+    {
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+      {
+        const double t = th[k] - xj[k];
+
+        chi2 = fma (t, t, chi2);
+      }
+
+      chi2  = chi2 / a.h2;
+      scale = a.kde_scale;
+    }
+
+    double K;
+
+    if (KERNEL == NCKDE_KERNEL_GAUSS)
+      K = exp (-0.5 * chi2);
+    else
+      K = pow (1.0 + chi2 / a.nu, kappa);
+
+    if (j < a.n_obs)
+      a.AT[(size_t) i * a.n_obs + j] = (K * scale) * rowscale;
+  }
+}
+
+template <int D, int KERNEL, int KIND>
+static hipError_t
+im_t (const IMArgs &a, hipStream_t s)
+{
+  const int bx = (a.n_obs + 255) / 256;
+  int gy       = (2048 + bx - 1) / bx;
+
+  if (gy > a.n_k)
+    gy = a.n_k;
+
+  if (gy < 1)
+    gy = 1;
+
+  hipLaunchKernelGGL ((k_build_IM<D, KERNEL, KIND>), dim3 (bx, gy), dim3 (256), 0, s, a);
+
+  return hipGetLastError ();
+}
+
+template <int D>
+static hipError_t
+im_d (const IMArgs &a, hipStream_t s)
+{
+  if (a.kind == NCKDE_VKDE)
+    return (a.kernel == NCKDE_KERNEL_GAUSS) ? im_t<D, NCKDE_KERNEL_GAUSS, NCKDE_VKDE> (a, s)
+                                            : im_t<D, NCKDE_KERNEL_ST, NCKDE_VKDE> (a, s);
+
+  return (a.kernel == NCKDE_KERNEL_GAUSS) ? im_t<D, NCKDE_KERNEL_GAUSS, NCKDE_KDE> (a, s)
+                                          : im_t<D, NCKDE_KERNEL_ST, NCKDE_KDE> (a, s);
+}
+
+hipError_t
+launch_build_IM (const IMArgs &a, hipStream_t s)
+{
+  switch (a.D)
+  {
+    case 4: return im_d<4> (a, s);
+    case 8: return im_d<8> (a, s);
+    case 12: return im_d<12> (a, s);
+    case 16: return im_d<16> (a, s);
+    case 24: return im_d<24> (a, s);
+    case 32: return im_d<32> (a, s);
+    case 48: return im_d<48> (a, s);
+    case 64: return im_d<64> (a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// out[c][r] = in[r][c]
+__global__ void
+k_transpose (const double *__restrict__ in, int rows, int cols, double *__restrict__ out)
+{
+  __shared__ double tile[32][33];
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+
+  for (int k = ty; k < 32; k += 8)
+  {
+    const int r = r0 + k, c = c0 + tx;
+
+    tile[k][tx] = (r < rows && c < cols) ? in[(size_t) r * cols + c] : 0.0;
+  }
+
+  __syncthreads ();
+
+  for (int k = ty; k < 32; k += 8)
+  {
+    const int c = c0 + k, r = r0 + tx;
+
+    if (r < rows && c < cols)
+      out[(size_t) c * rows + r] = tile[tx][k];
+  }
+}
+
+hipError_t
+launch_transpose (const double *in, int rows, int cols, double *out, hipStream_t s)
+{
+  dim3 grid ((cols + 31) / 32, (rows + 31) / 32);
+
+  hipLaunchKernelGGL (k_transpose, grid, dim3 (32, 8), 0, s, in, rows, cols, out);
+
+  return hipGetLastError ();
+}
+
+// --------------------------------------------------------------------------
+// Symmetric rank-K products on 64 x 64 tiles of the upper triangle, 256
+// threads per tile, 4 x 4 outputs per thread, K staged through LDS in
+// slices of 16.  mode 0: M = AT AT^T (AT is n x K, row-major; the Gram
+// matrix A^T A of the NNLS normal equations, ncm_matrix.c:1547-1575).
+// mode 1: trailing update of the blocked Cholesky, S[a][b] -= sum_r
+// S[kb+r][a] S[kb+r][b] over the rows kb..kb+K-1, for off <= a <= b.
+// --------------------------------------------------------------------------
+template <int MODE>
+__global__ void __launch_bounds__ (256)
+k_syrk_tiles (const double *__restrict__ P, int n, int K, int ld, int off, int kb, double *__restrict__ M)
+{
+  __shared__ double As[16][65];
+  __shared__ double Bs[16][65];
+  const int nt = (n - off + 63) / 64;
+  int t        = blockIdx.x;
+  int ti       = 0;
+
+  while (t >= nt - ti)
+  {
+    t -= nt - ti;
+    ++ti;
+  }
+
+  const int tj  = ti + t;
+  const int r0  = off + ti * 64, c0 = off + tj * 64;
+  const int tx  = threadIdx.x % 16, ty = threadIdx.x / 16;
+  double acc[4][4];
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      acc[i][j] = 0.0;
+
+  for (int k0 = 0; k0 < K; k0 += 16)
+  {
+    for (int e = threadIdx.x; e < 1024; e += 256)
+    {
+      int rr, kk;
+
+      if (MODE == 0)
+      {
+        rr = e / 16;
+        kk = e % 16;
+        const int ra = r0 + rr, rb = c0 + rr, kc = k0 + kk;
+
+        As[kk][rr] = (ra < n && kc < K) ? P[(size_t) ra * ld + kc] : 0.0;
+        Bs[kk][rr] = (rb < n && kc < K) ? P[(size_t) rb * ld + kc] : 0.0;
+      }
+      else
+      {
+        kk = e / 64;
+        rr = e % 64;
+        const int ca = r0 + rr, cb = c0 + rr, kr = k0 + kk;
+
+        As[kk][rr] = (ca < n && kr < K) ? P[(size_t) (kb + kr) * ld + ca] : 0.0;
+        Bs[kk][rr] = (cb < n && kr < K) ? P[(size_t) (kb + kr) * ld + cb] : 0.0;
+      }
+    }
+
+    __syncthreads ();
+
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+    {
+      double av[4], bv[4];
+
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+      {
+        av[i] = As[kk][ty * 4 + i];
+        bv[i] = Bs[kk][tx * 4 + i];
+      }
+
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = fma (av[i], bv[j], acc[i][j]);
+    }
+
+    __syncthreads ();
+  }
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+  {
+    const int a = r0 + ty * 4 + i;
+
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+    {
+      const int b = c0 + tx * 4 + j;
+
+      if (a < n && b < n && a <= b)
+      {
+        if (MODE == 0)
+        {
+          M[(size_t) a * n + b] = acc[i][j];
+          M[(size_t) b * n + a] = acc[i][j];
+        }
+        else
+        {
+          M[(size_t) a * ld + b] -= acc[i][j];
+        }
+      }
+    }
+  }
+}
+
+hipError_t
+launch_gram_nt (const double *AT, int n, int K, double *M, hipStream_t s)
+{
+  const int nt = (n + 63) / 64;
+
+  hipLaunchKernelGGL (k_syrk_tiles<0>, dim3 (nt * (nt + 1) / 2), dim3 (256), 0, s, AT, n, K, K, 0, 0, M);
+
+  return hipGetLastError ();
+}
+
+// out[a] = sum_j AT[a][j] v[j] (v == NULL: ones).  One wave per row.
+__global__ void
+k_row_dot (const double *__restrict__ AT, int n, int K, const double *__restrict__ v, double *__restrict__ out)
+{
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x % 64;
+
+  if (wave >= n)
+    return;
+
+  const double *row = AT + (size_t) wave * K;
+  double s          = 0.0;
+
+  for (int j = lane; j < K; j += 64)
+    s = fma (row[j], v ? v[j] : 1.0, s);
+
+  for (int o = 32; o > 0; o >>= 1)
+    s += __shfl_xor (s, o, 64);
+
+  if (lane == 0)
+    out[wave] = s;
+}
+
+hipError_t
+launch_row_dot (const double *AT, int n, int K, const double *v, double *out, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_row_dot, dim3 ((n + 3) / 4), dim3 (256), 0, s, AT, n, K, v, out);
+
+  return hipGetLastError ();
+}
+
+// r = f - A x (f == NULL: ones), with per-block partial sums of r^2; x is
+// wave-uniform and zero entries (inactive columns) are skipped.
+__global__ void __launch_bounds__ (256)
+k_residual (const double *__restrict__ AT, int nk, int nobs, const double *__restrict__ x,
+            const double *__restrict__ f, double *__restrict__ r, double *__restrict__ partial)
+{
+  __shared__ double red[256];
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  double acc  = 0.0;
+
+  for (int a = 0; a < nk; ++a)
+  {
+    const double xa = x[a];
+
+    if (xa != 0.0 && j < nobs)
+      acc = fma (AT[(size_t) a * nobs + j], xa, acc);
+  }
+
+  double rj = 0.0;
+
+  if (j < nobs)
+  {
+    rj   = (f ? f[j] : 1.0) - acc;
+    r[j] = rj;
+  }
+
+  red[threadIdx.x] = rj * rj;
+  __syncthreads ();
+
+  for (int o = 128; o > 0; o >>= 1)
+  {
+    if (threadIdx.x < o)
+      red[threadIdx.x] += red[threadIdx.x + o];
+
+    __syncthreads ();
+  }
+
+  if (threadIdx.x == 0)
+    partial[blockIdx.x] = red[0];
+}
+
+__global__ void
+k_sum_partials (const double *__restrict__ partial, int np, double *__restrict__ out)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+  {
+    double s = 0.0;
+
+    for (int i = 0; i < np; ++i)
+      s += partial[i];
+
+    out[0] = s;
+  }
+}
+
+hipError_t
+launch_residual (const double *AT, int nk, int nobs, const double *x, const double *f, double *r, double *partial,
+                 double *rnorm2, hipStream_t s)
+{
+  const int nb = (nobs + 255) / 256;
+
+  hipLaunchKernelGGL (k_residual, dim3 (nb), dim3 (256), 0, s, AT, nk, nobs, x, f, r, partial);
+  hipLaunchKernelGGL (k_sum_partials, dim3 (1), dim3 (64), 0, s, partial, nb, rnorm2);
+
+  return hipGetLastError ();
+}
+
+// S = M[P][P], bs = b[P]
+__global__ void
+k_gather_sym (const double *__restrict__ M, int n, const int *__restrict__ P, int np, double *__restrict__ S,
+              const double *__restrict__ b, double *__restrict__ bs)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+
+  if (r >= np || c >= np)
+    return;
+
+  S[(size_t) r * np + c] = M[(size_t) P[r] * n + P[c]];
+
+  if (c == 0)
+    bs[r] = b[P[r]];
+}
+
+hipError_t
+launch_gather_sym (const double *M, int n, const int *P, int np, double *S, const double *b, double *bs,
+                   hipStream_t s)
+{
+  hipLaunchKernelGGL (k_gather_sym, dim3 ((np + 255) / 256, np), dim3 (256), 0, s, M, n, P, np, S, b, bs);
+
+  return hipGetLastError ();
+}
+
+// --------------------------------------------------------------------------
+// Blocked right-looking Cholesky of the SPD system S = U^T U (upper, as
+// dposv with the row-major 'U' of ncm_matrix.c:1198-1208), 64-wide panels:
+//   diagonal block in LDS (dot-product form), row-panel triangular solve,
+//   trailing rank-64 update of the upper triangle.
+// info = first failing global pivot + 1 (LAPACK convention), else 0.
+// --------------------------------------------------------------------------
+constexpr int CH_NB = 64;
+
+__global__ void __launch_bounds__ (256)
+k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ info)
+{
+  __shared__ double B[CH_NB][CH_NB + 1];
+  __shared__ int s_fail;
+  const int tid = threadIdx.x;
+
+  if (*info != 0)
+    return;
+
+  for (int e = tid; e < nb * nb; e += 256)
+  {
+    const int r = e / nb, c = e % nb;
+
+    B[r][c] = S[(size_t) (kb + r) * np + kb + c];
+  }
+
+  if (tid == 0)
+    s_fail = 0;
+
+  __syncthreads ();
+
+  for (int j = 0; j < nb; ++j)
+  {
+    if (tid == 0)
+    {
+      double s = B[j][j];
+
+      for (int k = 0; k < j; ++k)
+        s = fma (-B[k][j], B[k][j], s);
+
+      if (!(s > 0.0))
+        s_fail = kb + j + 1;
+      else
+        B[j][j] = sqrt (s);
+    }
+
+    __syncthreads ();
+
+    if (s_fail)
+      break;
+
+    const double ujj = B[j][j];
+
+    for (int i = j + 1 + tid; i < nb; i += 256)
+    {
+      double t = B[j][i];
+
+      for (int k = 0; k < j; ++k)
+        t = fma (-B[k][j], B[k][i], t);
+
+      B[j][i] = t / ujj;
+    }
+
+    __syncthreads ();
+  }
+
+  if (s_fail)
+  {
+    if (tid == 0)
+      *info = s_fail;
+
+    return;
+  }
+
+  for (int e = tid; e < nb * nb; e += 256)
+  {
+    const int r = e / nb, c = e % nb;
+
+    if (c >= r)
+      S[(size_t) (kb + r) * np + kb + c] = B[r][c];
+  }
+}
+
+// row panel: for each column c >= kb+nb solve U_kk^T x = S[kb:kb+nb][c]
+__global__ void __launch_bounds__ (256)
+k_trsm_row (double *__restrict__ S, int np, int kb, int nb, const int *__restrict__ info)
+{
+  __shared__ double U[CH_NB][CH_NB + 1];
+  const int c = kb + nb + blockIdx.x * 256 + threadIdx.x;
+
+  if (*info != 0)
+    return;
+
+  for (int e = threadIdx.x; e < nb * nb; e += 256)
+  {
+    const int r = e / nb, q = e % nb;
+
+    U[r][q] = S[(size_t) (kb + r) * np + kb + q];
+  }
+
+  __syncthreads ();
+
+  if (c >= np)
+    return;
+
+  for (int r = 0; r < nb; ++r)
+  {
+    double t = S[(size_t) (kb + r) * np + c];
+
+    for (int q = 0; q < r; ++q)
+      t = fma (-U[q][r], S[(size_t) (kb + q) * np + c], t);
+
+    S[(size_t) (kb + r) * np + c] = t / U[r][r];
+  }
+}
+
+// lower triangle <- transpose of the upper factor (for the backward solve)
+__global__ void
+k_mirror_lower (double *__restrict__ S, int np)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+
+  if (r < np && c < r)
+    S[(size_t) r * np + c] = S[(size_t) c * np + r];
+}
+
+hipError_t
+launch_potrf (double *S, int np, int *info, hipStream_t s)
+{
+  for (int kb = 0; kb < np; kb += CH_NB)
+  {
+    const int nb = (np - kb < CH_NB) ? np - kb : CH_NB;
+
+    hipLaunchKernelGGL (k_potrf_diag, dim3 (1), dim3 (256), 0, s, S, np, kb, nb, info);
+
+    if (kb + nb < np)
+    {
+      const int rest = np - kb - nb;
+      const int nt   = (rest + 63) / 64;
+
+      hipLaunchKernelGGL (k_trsm_row, dim3 ((rest + 255) / 256), dim3 (256), 0, s, S, np, kb, nb, info);
+      hipLaunchKernelGGL (k_syrk_tiles<1>, dim3 (nt * (nt + 1) / 2), dim3 (256), 0, s, S, np, nb, np,
+                          kb + nb, kb, S);
+    }
+  }
+
+  hipLaunchKernelGGL (k_mirror_lower, dim3 ((np + 255) / 256, np), dim3 (256), 0, s, S, np);
+
+  return hipGetLastError ();
+}
+
+// U^T y = b then U x = y, in place in b; one 1024-thread block.  Each step
+// divides by the pivot, then updates the remaining right-hand side with a
+// unit-stride row of U (forward) or of its mirrored transpose (backward).
+__global__ void __launch_bounds__ (1024)
+k_trsv_pair (const double *__restrict__ S, int np, double *__restrict__ b)
+{
+  __shared__ double yj;
+  const int tid = threadIdx.x;
+
+  for (int j = 0; j < np; ++j)
+  {
+    if (tid == 0)
+    {
+      yj   = b[j] / S[(size_t) j * np + j];
+      b[j] = yj;
+    }
+
+    __syncthreads ();
+
+    const double y = yj;
+
+    for (int i = j + 1 + tid; i < np; i += 1024)
+      b[i] = fma (-S[(size_t) j * np + i], y, b[i]);
+
+    __syncthreads ();
+  }
+
+  for (int j = np - 1; j >= 0; --j)
+  {
+    if (tid == 0)
+    {
+      yj   = b[j] / S[(size_t) j * np + j];
+      b[j] = yj;
+    }
+
+    __syncthreads ();
+
+    const double y = yj;
+
+    for (int i = tid; i < j; i += 1024)
+      b[i] = fma (-S[(size_t) j * np + i], y, b[i]);
+
+    __syncthreads ();
+  }
+}
+
+hipError_t
+launch_trsv_pair (const double *S, int np, double *b, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_trsv_pair, dim3 (1), dim3 (1024), 0, s, S, np, b);
+
+  return hipGetLastError ();
+}
+
+// --------------------------------------------------------------------------
+// Fallback when the normal matrix is not numerically SPD (the reference's
+// dsysv path, ncm_nnls.c:574-605): Gaussian elimination with partial
+// pivoting (first maximum) on the symmetrised matrix, one 1024-thread block.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__ (1024)
+k_lu_solve (double *__restrict__ S, int np, double *__restrict__ b, int *__restrict__ info)
+{
+  __shared__ double rv[1024];
+  __shared__ int ri[1024];
+  const int tid = threadIdx.x;
+
+  for (size_t e = tid; e < (size_t) np * np; e += 1024)
+  {
+    const int i = (int) (e / np), j = (int) (e % np);
+
+    if (j < i)
+      S[e] = S[(size_t) j * np + i];
+  }
+
+  __syncthreads ();
+
+  for (int k = 0; k < np; ++k)
+  {
+    double best = -1.0;
+    int bi      = np;
+
+    for (int i = k + tid; i < np; i += 1024)
+    {
+      const double v = fabs (S[(size_t) i * np + k]);
+
+      if (v > best)
+      {
+        best = v;
+        bi   = i;
+      }
+    }
+
+    rv[tid] = best;
+    ri[tid] = bi;
+    __syncthreads ();
+
+    for (int o = 512; o > 0; o >>= 1)
+    {
+      if (tid < o)
+      {
+        const double v2 = rv[tid + o];
+        const int i2    = ri[tid + o];
+
+        if (v2 > rv[tid] || (v2 == rv[tid] && i2 < ri[tid]))
+        {
+          rv[tid] = v2;
+          ri[tid] = i2;
+        }
+      }
+
+      __syncthreads ();
+    }
+
+    const double amax = rv[0];
+    const int piv     = ri[0];
+
+    __syncthreads ();
+
+    if (amax == 0.0)
+    {
+      if (tid == 0)
+        *info = k + 1;
+
+      return;
+    }
+
+    if (piv != k)
+    {
+      for (int j = tid; j < np; j += 1024)
+      {
+        const double t          = S[(size_t) k * np + j];
+        S[(size_t) k * np + j]   = S[(size_t) piv * np + j];
+        S[(size_t) piv * np + j] = t;
+      }
+
+      if (tid == 0)
+      {
+        const double t = b[k];
+
+        b[k]   = b[piv];
+        b[piv] = t;
+      }
+    }
+
+    __syncthreads ();
+
+    const int rows = np - k - 1;
+
+    for (size_t e = tid; e < (size_t) rows * (rows + 1); e += 1024)
+    {
+      const int i    = k + 1 + (int) (e / (rows + 1));
+      const int jj   = (int) (e % (rows + 1));
+      const double l = S[(size_t) i * np + k] / S[(size_t) k * np + k];
+
+      if (jj < rows)
+      {
+        const int j = k + 1 + jj;
+
+        S[(size_t) i * np + j] = S[(size_t) i * np + j] - l * S[(size_t) k * np + j];
+      }
+      else
+      {
+        b[i] = b[i] - l * b[k];
+      }
+    }
+
+    __syncthreads ();
+  }
+
+  for (int i = np - 1; i >= 0; --i)
+  {
+    if (tid == 0)
+    {
+      double t = b[i];
+
+      for (int j = i + 1; j < np; ++j)
+        t = t - S[(size_t) i * np + j] * b[j];
+
+      b[i] = t / S[(size_t) i * np + i];
+    }
+
+    __syncthreads ();
+  }
+
+  if (tid == 0)
+    *info = 0;
+}
+
+hipError_t
+launch_lu_solve (double *S, int np, double *b, int *info, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_lu_solve, dim3 (1), dim3 (1024), 0, s, S, np, b, info);
+
+  return hipGetLastError ();
+}
+
+__global__ void
+k_scatter (const double *__restrict__ sol, const int *__restrict__ P, int np, int n, double *__restrict__ x)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+
+  if (a < n)
+    x[a] = 0.0;
+}
+
+__global__ void
+k_scatter2 (const double *__restrict__ sol, const int *__restrict__ P, int np, double *__restrict__ x)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+
+  if (a < np)
+    x[P[a]] = sol[a];
+}
+
+hipError_t
+launch_scatter (const double *sol, const int *P, int np, int n, double *x, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_scatter, dim3 ((n + 255) / 256), dim3 (256), 0, s, sol, P, np, n, x);
+  hipLaunchKernelGGL (k_scatter2, dim3 ((np + 255) / 256), dim3 (256), 0, s, sol, P, np, x);
+
+  return hipGetLastError ();
+}
+
+// w <- w (1 - s) / sum(w) + s / n  (ncm_stats_dist.c:1087-1093), serial sum
+__global__ void
+k_shrink (double *__restrict__ w, int n, double shrink, int *__restrict__ status)
+{
+  __shared__ double total;
+
+  if (threadIdx.x == 0)
+  {
+    double t = 0.0;
+
+    for (int i = 0; i < n; ++i)
+      t = __dadd_rn (t, w[i]);
+
+    total = t;
+
+    if (!(t > 0.0))
+      status[0] = 3;
+  }
+
+  __syncthreads ();
+
+  const double sc = (1.0 - shrink) / total;
+  const double ad = shrink / n;
+
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    w[i] = __dadd_rn (__dmul_rn (w[i], sc), ad);
+}
+
+hipError_t
+launch_shrink (double *w, int n, double shrink, int *status, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_shrink, dim3 (1), dim3 (1024), 0, s, w, n, shrink, status);
+
+  return hipGetLastError ();
+}
+
+} // namespace nck

@@ -1,0 +1,895 @@
+// k_prepare.hip -- FP64 prepare kernels: covariance, batched Cholesky with
+// nearPD/diagonal fallback, whitening, exact kNN, evaluation factors.
+//
+// These kernels keep the reference's operation order per element (no FMA
+// contraction, IEEE division and square root), so the local covariances, the
+// Cholesky factors and the neighbour sets equal the CPU restatement's bit for
+// bit.  Parallelism comes from independent matrices, matrix entries and
+// queries, never from reassociating a sum.
+#include "nckde_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace nck
+{
+
+__device__ inline void
+pair_of (int p, int d, int &a, int &b)
+{
+  int aa = 0, rem = p;
+
+  while (rem >= d - aa)
+  {
+    rem -= d - aa;
+    ++aa;
+  }
+
+  a = aa;
+  b = aa + rem;
+}
+
+// --------------------------------------------------------------------------
+// NcmStatsVec covariance (ncm_stats_vec.c:510-565, :2376-2399).  One thread
+// per (a <= b) entry follows the one-pass West/Welford recurrence over the
+// rows in their insertion order; the per-entry recurrence needs only the
+// running means of columns a and b, which the thread tracks itself.
+// --------------------------------------------------------------------------
+__global__ void
+k_welford_cov (const double *__restrict__ X, int d, int nrows, const int *__restrict__ rows,
+               int rows_stride, int nmats, double *__restrict__ cov_out)
+{
+  const int npairs = d * (d + 1) / 2;
+  const long mat   = blockIdx.y + (long) blockIdx.z * gridDim.y;
+  const int p      = blockIdx.x * blockDim.x + threadIdx.x;
+
+  if (mat >= nmats || p >= npairs)
+    return;
+
+  int a, b;
+
+  pair_of (p, d, a, b);
+
+  const int *rw  = rows ? rows + mat * (size_t) rows_stride : nullptr;
+  const double w = 1.0;
+  double mean_a = 0.0, mean_b = 0.0, acc = 0.0, weight = 0.0, weight2 = 0.0;
+
+  for (int t = 0; t < nrows; ++t)
+  {
+    const size_t r         = rw ? (size_t) rw[t] : (size_t) t;
+    const double xa        = X[r * d + a];
+    const double xb        = X[r * d + b];
+    const double curweight = weight + w;
+    const double delta_a   = xa - mean_a;
+    const double R_a       = delta_a * w / curweight;
+
+    if (a == b)
+    {
+      const double dvar = weight * delta_a * R_a;
+
+      acc    = acc + dvar;
+      mean_a = mean_a + R_a;
+    }
+    else
+    {
+      const double mean_a_new = mean_a + R_a;
+      const double dC         = w * (xa - mean_a_new) * (xb - mean_b);
+      const double delta_b    = xb - mean_b;
+      const double R_b        = delta_b * w / curweight;
+
+      acc    = acc + dC;
+      mean_a = mean_a_new;
+      mean_b = mean_b + R_b;
+    }
+
+    weight  = curweight;
+    weight2 = weight2 + w * w;
+  }
+
+  {
+    const double bias_wt = 1.0 / (weight - weight2 / weight);
+    double *C            = cov_out + mat * (size_t) d * d;
+    const double v       = acc * bias_wt;
+
+    C[a * d + b] = v;
+    C[b * d + a] = v;
+  }
+}
+
+hipError_t
+launch_welford_cov (const double *X, int d, int nrows, const int *rows, int rows_stride, int nmats,
+                    double *cov, hipStream_t s)
+{
+  const int npairs = d * (d + 1) / 2;
+  const int bs     = 64;
+  const int gy     = nmats < 32768 ? nmats : 32768;
+  const int gz     = (nmats + gy - 1) / gy;
+  dim3 grid ((npairs + bs - 1) / bs, gy, gz);
+
+  hipLaunchKernelGGL (k_welford_cov, grid, dim3 (bs), 0, s, X, d, nrows, rows, rows_stride, nmats, cov);
+
+  return hipGetLastError ();
+}
+
+// --------------------------------------------------------------------------
+// log-determinant with the frexp guard (ncm_matrix.c:1157-1186) and kernel
+// log-normalisation (ncm_stats_dist_kernel_gauss.c:201-207,
+// ncm_stats_dist_kernel_st.c:241-253).  For the Student-t kernel the host
+// passes lg + nc = lgamma(nu/2) - lgamma((nu+d)/2) + (d/2)(ln pi + ln nu).
+// --------------------------------------------------------------------------
+__device__ double
+chol_lndet (const double *U, int d, int lda)
+{
+  const double lb = 1.0e-200;
+  const double ub = 1.0e+200;
+  double detL     = 1.0;
+  long exponent   = 0;
+
+  for (int i = 0; i < d; i++)
+  {
+    const double Lii   = fabs (U[(size_t) i * lda + i]);
+    const double ndetL = detL * Lii;
+
+    if ((ndetL < lb) || (ndetL > ub))
+    {
+      int e = 0;
+
+      detL      = frexp (ndetL, &e);
+      exponent += e;
+    }
+    else
+    {
+      detL = ndetL;
+    }
+  }
+
+  return 2.0 * (log (detL) + exponent * 0.69314718055994530942);
+}
+
+__device__ double
+kernel_lnnorm (const double *U, int d, int lda, int kernel, double lnnorm_const)
+{
+  const double lndet = chol_lndet (U, d, lda);
+
+  if (kernel == NCKDE_KERNEL_GAUSS)
+    return 0.5 * ((double) (unsigned) d * LN2PI + lndet);
+
+  return lnnorm_const + 0.5 * lndet;
+}
+
+// --------------------------------------------------------------------------
+// Batched Cholesky, dpotrf on the row-major upper triangle (ncm_lapack.c:
+// 215-237), one 64-thread block per matrix held in LDS.  Column j: the pivot
+// by one lane, then the row j entries right of the pivot in parallel; each
+// entry's dot product runs in the serial order.  The strict lower triangle is
+// left as the input had it, as LAPACK does.
+// --------------------------------------------------------------------------
+__global__ void
+k_chol_batched (const double *__restrict__ cov, double *__restrict__ Uout, int *__restrict__ info,
+                double *__restrict__ lnnorm, int d, int nmats, int kernel, double lnnorm_const)
+{
+  extern __shared__ double S[];
+  __shared__ int s_info;
+  const long mat = blockIdx.x + (long) blockIdx.y * gridDim.x;
+  const int tid  = threadIdx.x;
+
+  if (mat >= nmats)
+    return;
+
+  const double *A = cov + mat * (size_t) d * d;
+
+  for (int i = tid; i < d * d; i += blockDim.x)
+    S[i] = A[i];
+
+  if (tid == 0)
+    s_info = 0;
+
+  __syncthreads ();
+
+  for (int j = 0; j < d; ++j)
+  {
+    if (tid == 0)
+    {
+      double s = S[j * d + j];
+
+      for (int k = 0; k < j; ++k)
+      {
+        const double ukj = S[k * d + j];
+
+        s = s - ukj * ukj;
+      }
+
+      if (!(s > 0.0))
+      {
+        S[j * d + j] = s;
+        s_info       = j + 1;
+      }
+      else
+      {
+        S[j * d + j] = sqrt (s);
+      }
+    }
+
+    __syncthreads ();
+
+    if (s_info != 0)
+      break;
+
+    {
+      const double ujj = S[j * d + j];
+
+      for (int i = j + 1 + tid; i < d; i += blockDim.x)
+      {
+        double t = S[j * d + i];
+
+        for (int k = 0; k < j; ++k)
+          t = t - S[k * d + j] * S[k * d + i];
+
+        S[j * d + i] = t / ujj;
+      }
+    }
+
+    __syncthreads ();
+  }
+
+  __syncthreads ();
+
+  double *U = Uout + mat * (size_t) d * d;
+
+  for (int i = tid; i < d * d; i += blockDim.x)
+    U[i] = S[i];
+
+  if (tid == 0)
+  {
+    info[mat] = s_info;
+
+    if (s_info == 0)
+      lnnorm[mat] = kernel_lnnorm (S, d, d, kernel, lnnorm_const);
+  }
+}
+
+hipError_t
+launch_chol_batched (const double *cov, double *U, int *info, double *lnnorm, int d, int nmats, int kernel,
+                     double lnnorm_const, hipStream_t s)
+{
+  const int gx = nmats < 32768 ? nmats : 32768;
+  const int gy = (nmats + gx - 1) / gx;
+
+  hipLaunchKernelGGL (k_chol_batched, dim3 (gx, gy), dim3 (64), sizeof (double) * d * d, s, cov, U, info,
+                      lnnorm, d, nmats, kernel, lnnorm_const);
+
+  return hipGetLastError ();
+}
+
+// --------------------------------------------------------------------------
+// nearPD (ncm_matrix.c:1247-1344) and the diagonal fallback
+// (ncm_stats_dist_vkde.c:337-360) for the matrices whose Cholesky failed.
+// One lane per failing matrix runs the serial algorithm on global scratch:
+// cyclic Jacobi for the symmetric eigenproblem, clamp of the negative
+// eigenvalues to min_pos * DBL_EPSILON, rank-1 re-assembly of the upper
+// triangle, Dykstra correction, diagonal restored, until the Cholesky
+// succeeds or maxiter + 2 projections have run.
+// --------------------------------------------------------------------------
+__device__ int
+dev_cholesky_upper (int d, double *A)
+{
+  for (int j = 0; j < d; j++)
+  {
+    double s = A[j * d + j];
+
+    for (int k = 0; k < j; k++)
+    {
+      const double ukj = A[k * d + j];
+
+      s = s - ukj * ukj;
+    }
+
+    if (!(s > 0.0))
+    {
+      A[j * d + j] = s;
+      return j + 1;
+    }
+
+    s            = sqrt (s);
+    A[j * d + j] = s;
+
+    for (int i = j + 1; i < d; i++)
+    {
+      double t = A[j * d + i];
+
+      for (int k = 0; k < j; k++)
+        t = t - A[k * d + j] * A[k * d + i];
+
+      A[j * d + i] = t / s;
+    }
+  }
+
+  return 0;
+}
+
+__device__ void
+dev_jacobi_eigen (int d, const double *A, double *eva, double *eve, double *a, double *v, int *ord)
+{
+  for (int i = 0; i < d; i++)
+    for (int j = 0; j < d; j++)
+    {
+      a[i * d + j] = (j >= i) ? A[i * d + j] : A[j * d + i];
+      v[i * d + j] = (i == j) ? 1.0 : 0.0;
+    }
+
+  for (int sweep = 0; sweep < 100; sweep++)
+  {
+    double off = 0.0;
+
+    for (int p = 0; p < d; p++)
+      for (int q = p + 1; q < d; q++)
+        off += a[p * d + q] * a[p * d + q];
+
+    if (off == 0.0)
+      break;
+
+    for (int p = 0; p < d; p++)
+    {
+      for (int q = p + 1; q < d; q++)
+      {
+        const double apq = a[p * d + q];
+        const double app = a[p * d + p];
+        const double aqq = a[q * d + q];
+
+        if (fabs (apq) <= 2.220446049250313e-16 * 1.0e-3 * sqrt (fabs (app * aqq)) || apq == 0.0)
+        {
+          a[p * d + q] = 0.0;
+          a[q * d + p] = 0.0;
+          continue;
+        }
+
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t     = (theta >= 0.0 ? 1.0 : -1.0) / (fabs (theta) + sqrt (theta * theta + 1.0));
+        const double c     = 1.0 / sqrt (t * t + 1.0);
+        const double s     = t * c;
+
+        for (int k = 0; k < d; k++)
+        {
+          const double akp = a[k * d + p];
+          const double akq = a[k * d + q];
+
+          a[k * d + p] = c * akp - s * akq;
+          a[k * d + q] = s * akp + c * akq;
+        }
+
+        for (int k = 0; k < d; k++)
+        {
+          const double apk = a[p * d + k];
+          const double aqk = a[q * d + k];
+
+          a[p * d + k] = c * apk - s * aqk;
+          a[q * d + k] = s * apk + c * aqk;
+        }
+
+        for (int k = 0; k < d; k++)
+        {
+          const double vkp = v[k * d + p];
+          const double vkq = v[k * d + q];
+
+          v[k * d + p] = c * vkp - s * vkq;
+          v[k * d + q] = s * vkp + c * vkq;
+        }
+      }
+    }
+  }
+
+  for (int i = 0; i < d; i++)
+    ord[i] = i;
+
+  for (int i = 1; i < d; i++)
+  {
+    const int oi = ord[i];
+    int j        = i - 1;
+
+    while (j >= 0 && a[ord[j] * d + ord[j]] > a[oi * d + oi])
+    {
+      ord[j + 1] = ord[j];
+      j--;
+    }
+
+    ord[j + 1] = oi;
+  }
+
+  for (int i = 0; i < d; i++)
+  {
+    eva[i] = a[ord[i] * d + ord[i]];
+
+    for (int j = 0; j < d; j++)
+      eve[i * d + j] = v[j * d + ord[i]];
+  }
+}
+
+__global__ void
+k_nearpd_fix (const int *__restrict__ which, int nfix, const double *__restrict__ cov, double *__restrict__ Uout,
+              double *__restrict__ lnnorm, double *__restrict__ scratch, int d, int maxiter, int kernel,
+              double lnnorm_const, int *__restrict__ status)
+{
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+
+  if (f >= nfix)
+    return;
+
+  const int mat   = which[f];
+  const size_t dd = (size_t) d * d;
+  double *ws      = scratch + (size_t) f * (6 * dd + 3 * d);
+  double *cm      = ws;
+  double *D_S     = ws + dd;
+  double *R       = ws + 2 * dd;
+  double *eve     = ws + 3 * dd;
+  double *a       = ws + 4 * dd;
+  double *v       = ws + 5 * dd;
+  double *eva     = ws + 6 * dd;
+  double *diag    = eva + d;
+  int *ord        = (int *) (diag + d);
+  const double *C = cov + (size_t) mat * dd;
+  double *U       = Uout + (size_t) mat * dd;
+  int ret         = 0;
+  int iter        = 0;
+
+  for (size_t i = 0; i < dd; i++)
+  {
+    cm[i]  = C[i];
+    D_S[i] = 0.0;
+  }
+
+  for (int i = 0; i < d; i++)
+    diag[i] = cm[i * d + i];
+
+  while (true)
+  {
+    double min_pos_ev = INFINITY;
+
+    for (size_t i = 0; i < dd; i++)
+      cm[i] = cm[i] - D_S[i];
+
+    for (size_t i = 0; i < dd; i++)
+      R[i] = cm[i];
+
+    dev_jacobi_eigen (d, cm, eva, eve, a, v, ord);
+
+    for (int i = 0; i < d; i++)
+      if (eva[i] > 0.0)
+        min_pos_ev = fmin (eva[i], min_pos_ev);
+
+    for (int i = 0; i < d; i++)
+    {
+      const double sc = (eva[i] < 0.0) ? sqrt (min_pos_ev * 2.220446049250313e-16) : sqrt (eva[i]);
+
+      if (eva[i] < 0.0)
+        eva[i] = min_pos_ev * 2.220446049250313e-16;
+
+      for (int j = 0; j < d; j++)
+        eve[i * d + j] = sc * eve[i * d + j];
+    }
+
+    for (int i = 0; i < d; i++)
+      for (int j = i; j < d; j++)
+      {
+        double s = 0.0;
+
+        for (int k = 0; k < d; k++)
+          s = s + eve[k * d + i] * eve[k * d + j];
+
+        cm[i * d + j] = s;
+      }
+
+    for (size_t i = 0; i < dd; i++)
+      D_S[i] = cm[i] - R[i];
+
+    for (int i = 0; i < d; i++)
+      cm[i * d + i] = diag[i];
+
+    for (size_t i = 0; i < dd; i++)
+      R[i] = cm[i];
+
+    if ((ret = dev_cholesky_upper (d, R)) == 0)
+      break;
+
+    if (iter > maxiter)
+      break;
+
+    iter++;
+  }
+
+  if (ret == 0)
+  {
+    for (size_t i = 0; i < dd; i++)
+      U[i] = R[i];
+  }
+  else
+  {
+    for (size_t i = 0; i < dd; i++)
+      U[i] = 0.0;
+
+    for (int i = 0; i < d; i++)
+      U[i * d + i] = C[i * d + i];
+
+    if (dev_cholesky_upper (d, U) != 0)
+      status[0] = 1;
+  }
+
+  lnnorm[mat] = kernel_lnnorm (U, d, d, kernel, lnnorm_const);
+}
+
+hipError_t
+launch_nearpd_fix (const int *which, int nfix, const double *cov, double *U, double *lnnorm, double *scratch,
+                   int d, int maxiter, int kernel, double lnnorm_const, int *status, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_nearpd_fix, dim3 ((nfix + 63) / 64), dim3 (64), 0, s, which, nfix, cov, U, lnnorm,
+                      scratch, d, maxiter, kernel, lnnorm_const, status);
+
+  return hipGetLastError ();
+}
+
+// --------------------------------------------------------------------------
+// invUsample = X U^{-1} (gsl_blas_dtrsm Right/Upper/NoTrans,
+// ncm_stats_dist_kde.c:477-480): one lane per row, forward substitution.
+// --------------------------------------------------------------------------
+__global__ void
+k_whiten (const double *__restrict__ X, const double *__restrict__ Ug, double *__restrict__ Y, int n, int d)
+{
+  const long j = blockIdx.x * (long) blockDim.x + threadIdx.x;
+
+  if (j >= n)
+    return;
+
+  const double *x = X + j * d;
+  double *y       = Y + j * d;
+
+  for (int l = 0; l < d; l++)
+  {
+    double t = x[l];
+
+    for (int m = 0; m < l; m++)
+      t = t - y[m] * Ug[m * d + l];
+
+    y[l] = t / Ug[l * d + l];
+  }
+}
+
+hipError_t
+launch_whiten (const double *X, const double *Ug, double *Y, int n, int d, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_whiten, dim3 ((n + 255) / 256), dim3 (256), 0, s, X, Ug, Y, n, d);
+
+  return hipGetLastError ();
+}
+
+// --------------------------------------------------------------------------
+// Exact k nearest neighbours in whitened space (kdtree.c:27-43 distance,
+// :192-320 search; rb_knn_list.c:31-40 order).  One 256-thread block per
+// query:
+//   1. squared distances to every point, summed over coordinates in order,
+//      stored as their IEEE bit patterns (monotone for non-negative doubles);
+//   2. radix select (8 passes of 8 bits) of the k-th smallest key;
+//   3. collect keys below it, then the smallest-index points equal to it;
+//   4. bitonic sort of the k (key, index) pairs; write the indices.
+// --------------------------------------------------------------------------
+constexpr int KNN_THREADS = 256;
+constexpr int KNN_KMAX    = 4096;
+
+size_t
+knn_max_k ()
+{
+  return KNN_KMAX;
+}
+
+__global__ void __launch_bounds__ (KNN_THREADS)
+k_knn (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, uint64_t *__restrict__ keys_all,
+       int *__restrict__ out, int *__restrict__ status)
+{
+  __shared__ uint64_t skey[KNN_KMAX];
+  __shared__ int sidx[KNN_KMAX];
+  __shared__ unsigned hist[256];
+  __shared__ uint64_t s_prefix;
+  __shared__ int s_kk;
+  __shared__ int s_nlt;
+  __shared__ int s_eq_taken;
+  __shared__ int s_scan[KNN_THREADS];
+
+  const int qb  = blockIdx.x;
+  const int tid = threadIdx.x;
+
+  if (qb >= nq)
+    return;
+
+  const int q       = q0 + qb;
+  uint64_t *keys    = keys_all + (size_t) qb * n;
+  const double *yq  = Y + (size_t) q * d;
+
+  // 1. distances
+  for (int j = tid; j < n; j += KNN_THREADS)
+  {
+    const double *yj = Y + (size_t) j * d;
+    double dist      = 0.0;
+
+    for (int l = 0; l < d; l++)
+    {
+      const double t = yj[l] - yq[l];
+
+      dist = dist + t * t;
+    }
+
+    keys[j] = (uint64_t) __double_as_longlong (dist);
+  }
+
+  if (tid == 0)
+  {
+    s_prefix = 0;
+    s_kk     = k;
+  }
+
+  __syncthreads ();
+
+  // 2. radix select of the k-th smallest key (1-based rank s_kk)
+  uint64_t mask = 0;
+
+  for (int pass = 0; pass < 8; ++pass)
+  {
+    const int shift = 56 - 8 * pass;
+
+    hist[tid] = 0;
+    __syncthreads ();
+
+    const uint64_t prefix = s_prefix;
+
+    for (int j = tid; j < n; j += KNN_THREADS)
+    {
+      const uint64_t key = keys[j];
+
+      if ((key & mask) == prefix)
+        atomicAdd (&hist[(key >> shift) & 255u], 1u);
+    }
+
+    __syncthreads ();
+
+    if (tid == 0)
+    {
+      int kk       = s_kk;
+      unsigned cum = 0;
+      int dg       = 0;
+
+      for (dg = 0; dg < 256; ++dg)
+      {
+        if (cum + hist[dg] >= (unsigned) kk)
+          break;
+
+        cum += hist[dg];
+      }
+
+      s_kk     = kk - (int) cum;
+      s_prefix = prefix | ((uint64_t) dg << shift);
+    }
+
+    mask |= (uint64_t) 255 << shift;
+    __syncthreads ();
+  }
+
+  const uint64_t T = s_prefix;
+  const int need_eq = s_kk;   // number of points equal to T to take
+
+  // 3. collect: all keys < T (any order), then the need_eq smallest indices
+  //    with key == T, scanning the points in index order chunk by chunk
+  if (tid == 0)
+  {
+    s_nlt      = 0;
+    s_eq_taken = 0;
+  }
+
+  __syncthreads ();
+
+  for (int j = tid; j < n; j += KNN_THREADS)
+  {
+    const uint64_t key = keys[j];
+
+    if (key < T)
+    {
+      const int slot = atomicAdd (&s_nlt, 1);
+
+      if (slot < KNN_KMAX)
+      {
+        skey[slot] = key;
+        sidx[slot] = j;
+      }
+    }
+  }
+
+  __syncthreads ();
+
+  const int nlt = s_nlt;
+
+  for (int base = 0; base < n; base += KNN_THREADS)
+  {
+    if (s_eq_taken >= need_eq)
+      break;
+
+    const int j    = base + tid;
+    const int flag = (j < n && keys[j] == T) ? 1 : 0;
+
+    s_scan[tid] = flag;
+    __syncthreads ();
+
+    for (int off = 1; off < KNN_THREADS; off <<= 1)
+    {
+      const int v = (tid >= off) ? s_scan[tid - off] : 0;
+
+      __syncthreads ();
+      s_scan[tid] += v;
+      __syncthreads ();
+    }
+
+    const int taken = s_eq_taken;
+    const int rank  = s_scan[tid] - flag;   // exclusive
+
+    if (flag && taken + rank < need_eq)
+    {
+      const int slot = nlt + taken + rank;
+
+      if (slot < KNN_KMAX)
+      {
+        skey[slot] = T;
+        sidx[slot] = j;
+      }
+    }
+
+    __syncthreads ();
+
+    if (tid == KNN_THREADS - 1)
+      s_eq_taken = taken + s_scan[KNN_THREADS - 1];
+
+    __syncthreads ();
+  }
+
+  if (nlt + need_eq != k && tid == 0)
+    status[0] = 2;
+
+  // 4. bitonic sort of (key, idx), padded to a power of two
+  int npow = 1;
+
+  while (npow < k)
+    npow <<= 1;
+
+  for (int i = k + tid; i < npow; i += KNN_THREADS)
+  {
+    skey[i] = ~(uint64_t) 0;
+    sidx[i] = 0x7fffffff;
+  }
+
+  __syncthreads ();
+
+  for (int size = 2; size <= npow; size <<= 1)
+  {
+    for (int stride = size >> 1; stride > 0; stride >>= 1)
+    {
+      for (int i = tid; i < npow; i += KNN_THREADS)
+      {
+        const int pj = i ^ stride;
+
+        if (pj > i)
+        {
+          const bool up      = ((i & size) == 0);
+          const uint64_t ka  = skey[i], kb = skey[pj];
+          const int ia       = sidx[i], ib = sidx[pj];
+          const bool a_gt_b  = (ka > kb) || (ka == kb && ia > ib);
+
+          if (a_gt_b == up)
+          {
+            skey[i]  = kb;
+            skey[pj] = ka;
+            sidx[i]  = ib;
+            sidx[pj] = ia;
+          }
+        }
+      }
+
+      __syncthreads ();
+    }
+  }
+
+  int *o = out + (size_t) q * k;
+
+  for (int i = tid; i < k; i += KNN_THREADS)
+    o[i] = sidx[i];
+}
+
+hipError_t
+launch_knn (const double *Y, int n, int d, int q0, int nq, int k, uint64_t *keys, int *out, int *status,
+            hipStream_t s)
+{
+  hipLaunchKernelGGL (k_knn, dim3 (nq), dim3 (KNN_THREADS), 0, s, Y, n, d, q0, nq, k, keys, out, status);
+
+  return hipGetLastError ();
+}
+
+// --------------------------------------------------------------------------
+// Evaluation factors.  VKDE: W_i = U_i^{-T} (lower triangular), packed by
+// rows and zero-padded to D, and the padded centres.  KDE: padded whitened
+// centres.  Each column of W_i solves U_i^T w = e_c by forward substitution.
+// --------------------------------------------------------------------------
+__global__ void
+k_vkde_factors (const double *__restrict__ Uall, const double *__restrict__ X, int nk, int d, int D,
+                double *__restrict__ Wp, double *__restrict__ Th)
+{
+  const long i = blockIdx.x * (long) blockDim.x + threadIdx.x;
+
+  if (i >= nk)
+    return;
+
+  const double *U = Uall + i * (size_t) d * d;
+  double *W       = Wp + i * (size_t) (D * (D + 1) / 2);
+  double *th      = Th + i * (size_t) D;
+
+  for (int l = 0; l < D * (D + 1) / 2; ++l)
+    W[l] = 0.0;
+
+  for (int c = 0; c < d; ++c)
+  {
+    // w_l for l >= c; w_l = 0 for l < c
+    for (int l = c; l < d; ++l)
+    {
+      double t = (l == c) ? 1.0 : 0.0;
+
+      for (int m = c; m < l; ++m)
+        t = t - U[m * d + l] * W[m * (m + 1) / 2 + c];
+
+      W[l * (l + 1) / 2 + c] = t / U[l * d + l];
+    }
+  }
+
+  for (int l = 0; l < D; ++l)
+    th[l] = (l < d) ? X[i * d + l] : 0.0;
+}
+
+hipError_t
+launch_vkde_factors (const double *U, const double *X, int nk, int d, int D, double *Wp, double *Th, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_vkde_factors, dim3 ((nk + 127) / 128), dim3 (128), 0, s, U, X, nk, d, D, Wp, Th);
+
+  return hipGetLastError ();
+}
+
+__global__ void
+k_kde_factors (const double *__restrict__ Y, int nk, int d, int D, double *__restrict__ Th)
+{
+  const long i = blockIdx.x * (long) blockDim.x + threadIdx.x;
+
+  if (i >= nk)
+    return;
+
+  for (int l = 0; l < D; ++l)
+    Th[i * D + l] = (l < d) ? Y[i * d + l] : 0.0;
+}
+
+hipError_t
+launch_kde_factors (const double *Y, int nk, int d, int D, double *Th, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_kde_factors, dim3 ((nk + 255) / 256), dim3 (256), 0, s, Y, nk, d, D, Th);
+
+  return hipGetLastError ();
+}
+
+// b_i = ln w_i - lnnorm_i (the per-kernel constant of sum0/sum1)
+__global__ void
+k_bvec (const double *__restrict__ w, const double *__restrict__ lnnorm, int nk, double *__restrict__ b)
+{
+  const long i = blockIdx.x * (long) blockDim.x + threadIdx.x;
+
+  if (i >= nk)
+    return;
+
+  b[i] = log (w[i]) - (lnnorm ? lnnorm[i] : 0.0);
+}
+
+hipError_t
+launch_bvec (const double *w, const double *lnnorm, int nk, double *b, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_bvec, dim3 ((nk + 255) / 256), dim3 (256), 0, s, w, lnnorm, nk, b);
+
+  return hipGetLastError ();
+}
+
+} // namespace nck

@@ -1,0 +1,150 @@
+// nckde_internal.h -- shared declarations of the gfx950 kernel-density engine.
+//
+// Device layout of one prepared handle (all row-major):
+//   X      n_obs x d        FP64  sample theta_j
+//   Y      n_obs x d        FP64  whitened sample y_j (U_g^T y_j = theta_j)
+//   U      n_k x d x d      FP64  per-kernel Cholesky factors (cov_i = U_i^T U_i)
+//   Wp     n_k x P(D)       FP64  packed lower W_i = U_i^{-T}, D = padded dim,
+//                                 P(D) = D(D+1)/2 (VKDE) / none (KDE)
+//   Th     n_k x D          FP64  kernel centres padded to D
+//   lnnorm n_k              FP64  kernel log-normalisation (without d ln h)
+//   w      n_k              FP64  weights;  b_i = ln w_i - lnnorm_i
+//   Mixed-precision factors (VKDE): A_i = s_i W_i U_g^T / h split in FP16
+//   hi/lo (n_k x 2 x DP x DP), c_i = A_i yc_i in FP32 (n_k x DP) and per-kernel
+//   FP32 coefficients; yc = centred whitened coordinates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/nckde.h"
+
+namespace nck
+{
+
+constexpr double LN2PI = 1.8378770664093454835606594728112353;
+constexpr double LNPI  = 1.1447298858494001741434273513530587;
+constexpr double LOG_DBL_EPSILON = -3.6043653389117154e+01;
+
+void set_error (const char *fmt, ...);
+
+// Device buffer with grow-only capacity.
+struct DBuf
+{
+  void *p      = nullptr;
+  size_t bytes = 0;
+
+  DBuf () = default;
+  DBuf (const DBuf &) = delete;
+  DBuf &operator= (const DBuf &) = delete;
+  ~DBuf () { release (); }
+
+  hipError_t ensure (size_t nbytes);
+  void release ();
+
+  template <class T> T *as () const { return static_cast<T *> (p); }
+};
+
+// Padded dimension used by the templated per-pair kernels.
+int padded_dim (int d);
+// Padded dimension of the MFMA tile (multiples of 4 up to 64, see k_eval_mixed.hip).
+int mfma_dim (int d);
+
+// ---- prepare kernels (k_prepare.hip) ------------------------------------
+hipError_t launch_welford_cov (const double *X, int d, int nrows, const int *rows, int rows_stride,
+                               int nmats, double *cov, hipStream_t s);
+hipError_t launch_chol_batched (const double *cov, double *U, int *info, double *lnnorm, int d,
+                                int nmats, int kernel, double lnnorm_const, hipStream_t s);
+hipError_t launch_nearpd_fix (const int *which, int nfix, const double *cov, double *U, double *lnnorm,
+                              double *scratch, int d, int maxiter, int kernel, double lnnorm_const,
+                              int *status, hipStream_t s);
+hipError_t launch_whiten (const double *X, const double *Ug, double *Y, int n, int d, hipStream_t s);
+hipError_t launch_knn (const double *Y, int n, int d, int q0, int nq, int k, uint64_t *keys,
+                       int *out, int *status, hipStream_t s);
+size_t knn_max_k ();
+hipError_t launch_vkde_factors (const double *U, const double *X, int nk, int d, int D, double *Wp,
+                                double *Th, hipStream_t s);
+hipError_t launch_kde_factors (const double *Y, int nk, int d, int D, double *Th, hipStream_t s);
+hipError_t launch_bvec (const double *w, const double *lnnorm, int nk, double *b, hipStream_t s);
+
+// ---- evaluation (k_eval.hip) --------------------------------------------
+struct EvalArgs
+{
+  int kind, kernel, d, D, n, m;
+  double nu, kappa, inv_h2, lnh_d, lnnorm_shared;
+  const double *Th, *Wp, *b;       // kernel factors
+  const double *Ug;                // KDE: global factor (d x d)
+  const double *X;                 // points, m x d
+  double *partM, *partS;           // nsplit x m
+  int nsplit;
+  double *out;                     // m
+  int linear;                      // 0: m2lnp, 1: density
+};
+hipError_t launch_eval_f64 (const EvalArgs &a, hipStream_t s);
+hipError_t launch_eval_merge (const EvalArgs &a, hipStream_t s);
+
+// ---- mixed-precision evaluation (k_eval_mixed.hip) ------------------------
+struct MixedFactors
+{
+  int DP;                 // padded MFMA dimension
+  const void *A;          // n_k x 2 x DP x DP fp16 (hi, lo), row-major per kernel
+  const float *c;         // n_k x DP
+  const float *coef;      // n_k x 4: (q_i, b2_i, -, -)
+};
+struct MixedArgs
+{
+  int kernel, d, n, m, nsplit;
+  double nu, kappa, lnh_d;
+  MixedFactors f;
+  const double *X;        // points m x d (FP64)
+  const double *mu;       // centring (d)
+  const double *G;        // d x d: yc = G (x - mu) with G = U_g^{-T} as lower dense
+  double ysc;             // power-of-two scale applied to yc before splitting
+  void *Yh;               // m x DP fp16 hi/lo interleaved workspace (2 x m x DP halves)
+  float *partM, *partS;   // nsplit x m
+  double *out;
+  int linear;
+};
+hipError_t launch_mixed_prepare_points (const MixedArgs &a, hipStream_t s);
+hipError_t launch_eval_mixed (const MixedArgs &a, hipStream_t s);
+hipError_t launch_mixed_merge (const MixedArgs &a, hipStream_t s);
+hipError_t launch_mixed_factors (const double *Wp, const double *Th, const double *Ug, const double *mu,
+                                 const double *b, int nk, int d, int D, int DP, double inv_h, double ysc,
+                                 int kernel, double nu, void *A, float *c, float *coef, hipStream_t s);
+
+// ---- interpolation matrix & linear algebra (k_linalg.hip) ----------------
+struct IMArgs
+{
+  int kind, kernel, d, D, n_obs, n_k;
+  double nu, inv_h2, h2, lnnorm_href, kde_scale;
+  const double *X, *Y, *Th, *Wp, *lnnorm;
+  const double *m2lnp;     // NULL: no row scaling
+  double min_m2lnp;
+  double *AT;              // n_k x n_obs (column-major A)
+};
+hipError_t launch_build_IM (const IMArgs &a, hipStream_t s);
+hipError_t launch_transpose (const double *in, int rows, int cols, double *out, hipStream_t s);
+hipError_t launch_gram_nt (const double *AT, int n, int K, double *M, hipStream_t s);
+hipError_t launch_row_dot (const double *AT, int n, int K, const double *v, double *out, hipStream_t s);
+hipError_t launch_residual (const double *AT, int nk, int nobs, const double *x, const double *f,
+                            double *r, double *partial, double *rnorm2, hipStream_t s);
+hipError_t launch_gather_sym (const double *M, int n, const int *P, int np, double *S,
+                              const double *b, double *bs, hipStream_t s);
+hipError_t launch_potrf (double *S, int np, int *info, hipStream_t s);
+hipError_t launch_trsv_pair (const double *S, int np, double *b, hipStream_t s);
+hipError_t launch_lu_solve (double *S, int np, double *b, int *info, hipStream_t s);
+hipError_t launch_scatter (const double *sol, const int *P, int np, int n, double *x, hipStream_t s);
+hipError_t launch_shrink (double *w, int n, double shrink, int *status, hipStream_t s);
+
+// ---- sampling (k_sample.hip) ----------------------------------------------
+hipError_t launch_wcum (const double *w, int n, double *wcum, hipStream_t s);
+hipError_t launch_choose (const double *wcum, int n, const double *u, int m, uint32_t *idx, hipStream_t s);
+hipError_t launch_sample (const double *wcum, int n, const double *u, const double *z, const double *chi2,
+                          int m, int d, int kind, int kernel, double nu, double href, const double *U,
+                          const double *X, double *out, uint32_t *idx, hipStream_t s);
+
+} // namespace nck

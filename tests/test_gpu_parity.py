@@ -1,0 +1,253 @@
+"""Parity of the HIP engine (through the C-ABI) with the FP64 CPU oracle.
+
+Bars:
+* integer / index results (neighbour lists, chosen kernels) .... bit-exact
+* prepare factors (covariances, Cholesky factors, whitening) ... bit-exact
+  (same operation order, IEEE division / sqrt, no contraction), lnnorm to
+  1e-14 relative (device log vs glibc log)
+* FP64 eval_m2lnp ........................ |rel err| <= 1e-12
+* prepare_interp weights ................. rel L1 <= 1e-7, KKT of the fit
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from numcosmo_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+GAUSS, ST = 0, 1
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import numcosmo_amd as na
+    return na
+
+
+def _pair(gpu, oracle, kind, kernel, d, X, nu=3.0, **kw):
+    sdk = gpu.StatsDistKernelGauss.new(d) if kernel == GAUSS else gpu.StatsDistKernelST.new(d, nu)
+    sd = (gpu.StatsDistVKDE if kind == 1 else gpu.StatsDistKDE).new(sdk, gpu.StatsDistCV.NONE)
+    od = oracle.StatsDist(kind, kernel, d, nu)
+    for k, v in kw.items():
+        getattr(sd, "set_" + k)(v)
+        getattr(od, "set_" + k)(v)
+    sd.add_obs_array(X)
+    od.add_obs_array(X)
+    return sd, od
+
+
+CASES = [
+    # kind, kernel, d, n, nu
+    (1, GAUSS, 2, 300, 0.0),
+    (1, ST, 3, 600, 3.0),
+    (1, ST, 8, 1200, 3.0),
+    (1, GAUSS, 16, 1000, 0.0),
+    (0, GAUSS, 4, 500, 0.0),
+    (0, ST, 5, 400, 4.0),
+]
+
+
+@pytest.mark.parametrize("kind,kernel,d,n,nu", CASES)
+def test_prepare_factors_bit_exact(gpu, oracle, kind, kernel, d, n, nu):
+    _say(f"[prepare] kind={kind} kernel={kernel} d={d} n={n}")
+    g = np.random.default_rng(1000 + d + n)
+    X = synth.Banana(d).draw(g, n) if d >= 2 else g.normal(size=(n, d))
+    sd, od = _pair(gpu, oracle, kind, kernel, d, X, nu)
+    sd.prepare()
+    od.prepare()
+    assert sd.get_sample_size() == od.get_sample_size() == n
+    assert sd.get_href() == od.get_href()
+    np.testing.assert_array_equal(sd.peek_full_cov(), od.get_full_cov())
+    np.testing.assert_array_equal(np.triu(sd.peek_full_cov_decomp()), np.triu(od.get_full_cov_decomp()))
+    np.testing.assert_array_equal(sd.peek_invUsample(), od.get_invUsample())
+    if kind == 1:
+        np.testing.assert_array_equal(sd.peek_knn(), od.get_knn())
+        for i in range(0, n, max(1, n // 50)):
+            np.testing.assert_array_equal(sd.peek_cov_decomp(i), od.get_cov_decomp(i))
+    for i in range(0, n, max(1, n // 20)):
+        assert sd.get_lnnorm(i) == pytest.approx(od.get_lnnorm(i), rel=1e-14, abs=1e-14)
+    np.testing.assert_array_equal(sd.peek_weights(), od.get_weights())
+
+
+@pytest.mark.parametrize("kind,kernel,d,n,nu", CASES)
+def test_eval_m2lnp_fp64(gpu, oracle, kind, kernel, d, n, nu):
+    _say(f"[eval fp64] kind={kind} kernel={kernel} d={d} n={n}")
+    g = np.random.default_rng(2000 + d + n)
+    tgt = synth.Banana(d) if d >= 2 else None
+    X = tgt.draw(g, n)
+    sd, od = _pair(gpu, oracle, kind, kernel, d, X, nu, over_smooth=0.8)
+    sd.prepare()
+    od.prepare()
+    Xe = tgt.draw(g, 700)
+    Xe[:5] = X[:5]             # points on kernel centres
+    Xe[5] = 40.0               # far outlier: every kernel deep in its tail
+    ref = od.eval_m2lnp_batch(Xe)
+    got = sd.eval_m2lnp_batch(Xe)
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)
+    _say(f"   max rel err {err.max():.3e}")
+    assert err.max() <= 1e-12
+    lin = sd.eval_batch(Xe[:50])
+    lref = od.eval_batch(Xe[:50])
+    assert np.allclose(lin, lref, rtol=1e-11, atol=0)
+    # the single-point entry routes to the same engine
+    assert sd.eval_m2lnp(Xe[7]) == got[7]
+
+
+def _kkt_rel(A, x):
+    g = A.T @ (1.0 - A @ x)
+    scale = np.abs(A.T @ np.ones(A.shape[0])).max()
+    act = x > 0
+    return max(np.abs(g[act]).max(initial=0.0), g[~act].max(initial=0.0)) / scale
+
+
+@pytest.mark.parametrize("kind,kernel,d,n,nu", [
+    (1, ST, 2, 200, 3.0), (1, GAUSS, 3, 400, 0.0), (1, ST, 8, 800, 3.0), (0, GAUSS, 4, 300, 0.0)])
+def test_prepare_interp_weights(gpu, oracle, kind, kernel, d, n, nu):
+    _say(f"[prepare_interp] kind={kind} kernel={kernel} d={d} n={n}")
+    g = np.random.default_rng(3000 + d + n)
+    tgt = synth.Gaussian(d, g)
+    X = tgt.draw(g, n)
+    m2lnp = tgt.m2lnp(X)
+    sd, od = _pair(gpu, oracle, kind, kernel, d, X, nu)
+    sd.prepare_interp(m2lnp)
+    od.prepare_interp(m2lnp)
+    w, wo = sd.peek_weights(), od.get_weights()
+    rel_l1 = np.abs(w - wo).sum() / np.abs(wo).sum()
+    _say(f"   weights rel L1 {rel_l1:.3e}, rnorm {sd.get_rnorm():.6e} vs {od.get_rnorm():.6e}")
+    assert rel_l1 <= 1e-7
+    assert sd.get_rnorm() == pytest.approx(od.get_rnorm(), rel=1e-7, abs=1e-14)
+    # the device interpolation matrix equals the oracle's
+    IM, IMo = sd.compute_IM(), od.compute_IM()
+    assert np.allclose(IM, IMo, rtol=1e-12, atol=1e-300)
+    # KKT of the device solution
+    fvec = np.exp(-0.5 * (m2lnp - m2lnp.min()))
+    A = IMo / fvec[:, None]
+    v = w - 0.01 / n
+    Av = A @ v
+    x = v * (Av.sum() / (Av @ Av))
+    assert _kkt_rel(A, x) < 1e-8
+    # evaluation after the fit
+    Xe = tgt.draw(g, 300)
+    ref = od.eval_m2lnp_batch(Xe)
+    got = sd.eval_m2lnp_batch(Xe)
+    assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)) < 1e-7
+
+
+def test_prepare_interp_cut_and_fallback(gpu, oracle):
+    _say("[prepare_interp cut/fallback]")
+    g = np.random.default_rng(3100)
+    d, n = 2, 120
+    X = g.normal(size=(n, d))
+    # fallback: more than half beyond the window
+    m2 = np.zeros(n)
+    m2[:70] = 400.0 + g.uniform(size=70)
+    sd, od = _pair(gpu, oracle, 1, GAUSS, d, X, local_frac=0.1)
+    sd.prepare_interp(m2)
+    od.prepare_interp(m2)
+    np.testing.assert_array_equal(sd.peek_weights(), od.get_weights())
+    # cut: a tenth beyond the window, recursion on the rest
+    m2 = np.sum(X * X, 1)
+    m2[::10] += 1000.0
+    sd, od = _pair(gpu, oracle, 1, GAUSS, d, X, local_frac=0.1)
+    sd.prepare_interp(m2)
+    od.prepare_interp(m2)
+    assert sd.get_n_samples() == od.get_n_samples() == 108
+    np.testing.assert_array_equal(sd.peek_sample_array(), od.get_sample())
+    assert np.abs(sd.peek_weights() - od.get_weights()).sum() < 1e-7
+
+
+@pytest.mark.parametrize("kind,kernel,nu", [(1, GAUSS, 0.0), (1, ST, 3.0), (0, ST, 5.0)])
+def test_kernel_choose_and_sample_exact(gpu, oracle, kind, kernel, nu):
+    _say(f"[sample] kind={kind} kernel={kernel}")
+    g = np.random.default_rng(4000 + kernel)
+    d, n, m = 3, 500, 20000
+    X = synth.Banana(d).draw(g, n)
+    sd, od = _pair(gpu, oracle, kind, kernel, d, X, nu)
+    sd.prepare()
+    od.prepare()
+    u = g.uniform(size=m)
+    u[:3] = [0.0, np.nextafter(1.0, 0.0), 0.5]
+    idx = sd.kernel_choose_batch(u)
+    ref = np.array([od.kernel_choose(p) for p in u])
+    np.testing.assert_array_equal(idx, ref)
+    z = g.normal(size=(200, d))
+    c2 = g.chisquare(nu if kernel == ST else 1.0, size=200)
+    out, ii = sd.sample_batch(u[:200], z, c2 if kernel == ST else None)
+    for p in range(200):
+        i, x = od.sample(u[p], z[p], c2[p])
+        assert ii[p] == i
+        np.testing.assert_allclose(out[p], x, rtol=1e-15, atol=1e-15)
+
+
+def test_kernel_choose_after_fit(gpu, oracle):
+    """Selection from the fitted weights is the reference's bisection on the
+    serial prefix sum of those weights."""
+    g = np.random.default_rng(4100)
+    d, n = 2, 300
+    tgt = synth.Gaussian(d, g)
+    X = tgt.draw(g, n)
+    sd, _ = _pair(gpu, oracle, 1, ST, d, X, 3.0)
+    sd.prepare_interp(tgt.m2lnp(X))
+    w = sd.peek_weights()
+    wcum = np.concatenate([[0.0], np.cumsum(w)])
+    wcum = wcum * (1.0 / wcum[-1])
+    u = g.uniform(size=50000)
+    ref = np.maximum(np.searchsorted(wcum[:n], u, side="right") - 1, 0)
+    np.testing.assert_array_equal(sd.kernel_choose_batch(u), ref)
+
+
+def test_errors_match_reference_text(gpu):
+    sdk = gpu.StatsDistKernelGauss.new(3)
+    sd = gpu.StatsDistKDE.new(sdk, gpu.StatsDistCV.NONE)
+    sd.add_obs_array(np.eye(3))
+    with pytest.raises(gpu.NckdeError, match="the sample is too small"):
+        sd.prepare()
+    sd = gpu.StatsDistVKDE.new(sdk, gpu.StatsDistCV.NONE)
+    sd.add_obs_array(np.random.default_rng(0).normal(size=(20, 3)))
+    with pytest.raises(gpu.NckdeError, match="Too few observations"):
+        sd.prepare()
+    with pytest.raises(gpu.NckdeError, match="not prepared"):
+        gpu.StatsDistVKDE.new(sdk).eval_m2lnp(np.zeros(3))
+
+
+def test_kde_fixed_covariance(gpu, oracle):
+    g = np.random.default_rng(4200)
+    d, n = 3, 200
+    cov = synth.spd(g, d)
+    X = g.normal(size=(n, d)) @ np.linalg.cholesky(cov).T
+    sd, od = _pair(gpu, oracle, 0, GAUSS, d, X)
+    for s in (sd, od):
+        s.set_cov_fixed(cov)
+        s.set_cov_type(1)
+    sd.prepare()
+    od.prepare()
+    Xe = g.normal(size=(100, d))
+    assert np.allclose(sd.eval_m2lnp_batch(Xe), od.eval_m2lnp_batch(Xe), rtol=1e-12)
+    bad = -np.eye(d)
+    sd2, _ = _pair(gpu, oracle, 0, GAUSS, d, X)
+    sd2.set_cov_type(1)
+    with pytest.raises(gpu.NckdeError, match="not positive definite"):
+        sd2.set_cov_fixed(bad)
+
+
+def test_nearpd_path_small_k(gpu, oracle):
+    """k = 2 neighbours give rank-1 local covariances: the nearPD / diagonal
+    fallback runs on the device for every kernel."""
+    g = np.random.default_rng(4300)
+    d, n = 3, 40
+    X = g.normal(size=(n, d))
+    sd, od = _pair(gpu, oracle, 1, GAUSS, d, X, local_frac=0.05)
+    sd.prepare()
+    od.prepare()
+    for i in range(n):
+        Ug, Uo = np.triu(sd.peek_cov_decomp(i)), np.triu(od.get_cov_decomp(i))
+        assert np.allclose(Ug.T @ Ug, Uo.T @ Uo, rtol=1e-9, atol=1e-12)
+    Xe = g.normal(size=(50, d))
+    assert np.allclose(sd.eval_m2lnp_batch(Xe), od.eval_m2lnp_batch(Xe), rtol=1e-9)
