@@ -106,6 +106,10 @@ struct nckde
   // prepared device state
   DBuf dX, dY, dcovg, dUg, dlng, dcov, dU, dinfo, dlnnorm, dw, db, dWp, dTh, dknn, dkeys, dscratch, dstatus,
        dwcum, dwhich;
+  // mixed-precision factors (built at the first mixed eval after a prepare)
+  DBuf dmu, dYc, dymax, dA16, dc32, dcoef, dYw, dpartMf;
+  bool mixed_ready = false, mixed_b_ready = false;
+  double ysc = 1.0, bref = 0.0;
   // work buffers
   DBuf dXe, dout, dpartM, dpartS, du, dz, dchi2, didx;
   DBuf dAT, dM, dS, dbv, dbs, dx, dxt, dr, drt, dg, dP, dpart, dscal, dm2lnp, dlu;
@@ -355,7 +359,8 @@ refresh_bvec (nckde *h)
   HIPCHK (h->db.ensure (sizeof (double) * h->n_k));
   HIPCHK (nck::launch_bvec (h->dw.as<double> (), (h->kind == NCKDE_VKDE) ? h->dlnnorm.as<double> () : nullptr,
                             (int) h->n_k, h->db.as<double> (), h->stream));
-  h->wcum_ready = false;
+  h->wcum_ready    = false;
+  h->mixed_b_ready = false;
 
   return NCKDE_OK;
 }
@@ -429,8 +434,9 @@ do_prepare (nckde *h)
   HIPCHK (h->dwcum.ensure (sizeof (double) * (h->n_k + 1)));
   HIPCHK (hipStreamSynchronize (h->stream));
 
-  h->rnorm    = 0.0;
-  h->prepared = true;
+  h->rnorm       = 0.0;
+  h->prepared    = true;
+  h->mixed_ready = false;
 
   return NCKDE_OK;
 }
@@ -801,6 +807,122 @@ do_prepare_interp (nckde *h, const double *m2lnp, size_t len)
   return NCKDE_OK;
 }
 
+// largest b_i = ln w_i - lnnorm_i (reference point of the base-2 sums)
+static int
+max_b (nckde *h, double &bref)
+{
+  std::vector<double> b (h->n_k);
+
+  HIPCHK (hipMemcpyAsync (b.data (), h->db.p, sizeof (double) * h->n_k, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK (hipStreamSynchronize (h->stream));
+  bref = -INFINITY;
+
+  for (double v : b)
+    bref = std::max (bref, v);
+
+  if (!std::isfinite (bref))
+  {
+    set_error ("nckde: no kernel has a positive weight");
+    return NCKDE_ENUMERIC;
+  }
+
+  return NCKDE_OK;
+}
+
+static int
+ensure_mixed (nckde *h)
+{
+  const int d   = h->d;
+  const int DP  = nck::mfma_dim (d);
+  const int kpt = (DP >= 32) ? 1 : 32 / DP;
+  const size_t n_pad = ((h->n_k + kpt - 1) / kpt) * kpt;
+
+  if (!h->mixed_ready)
+  {
+    TRY (max_b (h, h->bref));
+    HIPCHK (h->dmu.ensure (sizeof (double) * d));
+    HIPCHK (h->dYc.ensure (sizeof (double) * h->n_k * d));
+    HIPCHK (h->dymax.ensure (sizeof (unsigned long long)));
+    HIPCHK (h->dA16.ensure (sizeof (_Float16) * n_pad * 2 * DP * DP));
+    HIPCHK (h->dc32.ensure (sizeof (float) * n_pad * DP));
+    HIPCHK (h->dcoef.ensure (sizeof (float) * 4 * n_pad));
+    HIPCHK (nck::mixed_build (h->dX.as<double> (), (int) h->n_k, d, h->D, DP, h->dWp.as<double> (),
+                              h->dUg.as<double> (), h->db.as<double> (), h->bref, 1.0 / h->href, h->kernel, h->nu,
+                              h->dmu.as<double> (), h->dYc.as<double> (),
+                              h->dymax.as<unsigned long long> (), &h->ysc, h->dA16.as<_Float16> (),
+                              h->dc32.as<float> (), h->dcoef.as<float> (), h->stream));
+    h->mixed_ready   = true;
+    h->mixed_b_ready = true;
+  }
+  else if (!h->mixed_b_ready)
+  {
+    TRY (max_b (h, h->bref));
+    HIPCHK (nck::mixed_refresh_b (h->db.as<double> (), h->bref, (int) h->n_k, h->dcoef.as<float> (), h->stream));
+    h->mixed_b_ready = true;
+  }
+
+  return NCKDE_OK;
+}
+
+static int
+eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s)
+{
+  if (h->kind != NCKDE_VKDE)
+  {
+    set_error ("nckde: the mixed-precision eval is implemented for NcmStatsDistVKDE; use NCKDE_PREC_FP64 for KDE");
+    return NCKDE_EUNSUPPORTED;
+  }
+
+  TRY (ensure_mixed (h));
+
+  const int DP      = nck::mfma_dim (h->d);
+  const int kpt     = (DP >= 32) ? 1 : 32 / DP;
+  const int ngroups = (int) ((h->n_k + kpt - 1) / kpt);
+  const int bx      = (int) ((m + nck::mixed_points_per_block (DP) - 1) / nck::mixed_points_per_block (DP));
+  int nsplit        = (1024 + bx - 1) / bx;
+
+  nsplit = std::max (1, std::min (nsplit, std::max (1, ngroups / 8)));
+
+  if (h->last_stream != nullptr && h->last_stream != s)
+    HIPCHK (hipStreamSynchronize (h->last_stream));
+
+  h->last_stream = s;
+
+  HIPCHK (h->dYw.ensure (m * (2 * sizeof (_Float16) * DP + sizeof (float) * 2)));
+  HIPCHK (h->dpartMf.ensure (sizeof (float) * m * nsplit));
+  HIPCHK (h->dpartS.ensure (sizeof (double) * m * nsplit));
+
+  nck::MixedArgs a;
+
+  a.kernel = h->kernel;
+  a.d      = h->d;
+  a.n      = (int) h->n_k;
+  a.m      = (int) m;
+  a.nsplit = nsplit;
+  a.nu     = h->nu;
+  a.kappa  = -0.5 * (h->nu + (double) (unsigned) h->d);
+  a.lnh_d  = h->d * std::log (h->href);
+  a.f.DP   = DP;
+  a.f.A    = h->dA16.p;
+  a.f.c    = h->dc32.as<float> ();
+  a.f.coef = h->dcoef.as<float> ();
+  a.X      = dX;
+  a.mu     = h->dmu.as<double> ();
+  a.G      = h->dUg.as<double> ();
+  a.ysc    = h->ysc;
+  a.Yh     = h->dYw.p;
+  a.partM  = h->dpartMf.as<float> ();
+  a.partS  = h->dpartS.as<double> ();
+  a.out    = dout;
+  a.linear = linear;
+
+  HIPCHK (nck::launch_mixed_points (a, s));
+  HIPCHK (nck::launch_eval_mixed (a, s));
+  HIPCHK (nck::launch_mixed_merge (a, h->bref, s));
+
+  return NCKDE_OK;
+}
+
 static int
 eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, int linear, hipStream_t s)
 {
@@ -808,10 +930,7 @@ eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, 
     return NCKDE_OK;
 
   if (precision == NCKDE_PREC_MIXED)
-  {
-    set_error ("nckde: the mixed-precision path is not built in this library version");
-    return NCKDE_EUNSUPPORTED;
-  }
+    return eval_mixed (h, dX, m, dout, linear, s);
 
   if (precision != NCKDE_PREC_FP64)
   {

@@ -1,0 +1,772 @@
+// k_eval_mixed.hip -- mixed-precision VKDE eval_m2lnp on the MFMA units.
+//
+// For kernel i and point x (natural log units)
+//   chi2_i(x) = | U_i^{-T} (x - theta_i) |^2 / h^2 = | B_i (y(x) - y_i) |^2,
+//   y(x) = G (x - mu), G = U_g^{-T} (global whitening), B_i = U_i^{-T} U_g^T / h.
+// With power-of-two scales ysc (points) and s_i (kernel),
+//   z'_i(x) = A_i (ysc y(x)) - c_i,  A_i = (s_i / ysc) B_i,  c_i = A_i (ysc y_i),
+//   chi2_i = |z'_i|^2 / s_i^2.
+// A_i and ysc y(x) are split into FP16 hi + lo parts (22 significant bits);
+// z' is accumulated in FP32 by v_mfma_f32_32x32x16_f16 from three products
+// (A_hi y_hi + A_hi y_lo + A_lo y_hi), with -c_i as the accumulator input.
+// Each 32-row MFMA tile holds 32/DP kernels (DP = padded dimension 4, 8, 16)
+// or one kernel over 1 or 2 row tiles (DP = 32, 64); for DP < 16 the hi/lo
+// operands are packed along K so no K lanes are wasted.
+//
+// The kernel sum runs on the VALU in base 2: lnt2 = log2(e) lnt - b_ref with
+//   Gauss  lnt2 = q_i |z'|^2 + b2_i,               q_i = -log2(e) / (2 s_i^2)
+//   ST     lnt2 = kappa log2(1 + r_i |z'|^2) + b2_i, r_i = 1 / (nu s_i^2)
+// and an online maximum M (FP32) with the scaled sum S in FP64.  Partial
+// (M, S) per kernel-range split are merged in order by k_mixed_merge.
+//
+// Reference: ncm_stats_dist_vkde.c:681-723 with the sum0 reduction of
+// ncm_stats_dist_kernel_gauss.c:246-290 / ncm_stats_dist_kernel_st.c:295-341.
+#include "nckde_internal.h"
+
+#include <cstring>
+
+namespace nck
+{
+
+typedef _Float16 half8 __attribute__ ((ext_vector_type (8)));
+typedef _Float16 half4 __attribute__ ((ext_vector_type (4)));
+typedef float floatx16 __attribute__ ((ext_vector_type (16)));
+
+constexpr double LOG2E = 1.4426950408889634073599246810018921;
+constexpr double LN2   = 0.69314718055994530941723212145817657;
+
+template <int DP>
+struct Plan
+{
+  static constexpr int KPT    = (DP >= 32) ? 1 : 32 / DP;  // kernels per 32-row tile
+  static constexpr int NRT    = (DP == 64) ? 2 : 1;        // row tiles per kernel group
+  static constexpr int NKB    = (DP >= 32) ? DP / 16 : 1;  // 16-wide K blocks
+  static constexpr int NLOCAL = (DP <= 8) ? 4 : (DP == 16 ? 2 : 1);
+};
+
+int
+mfma_dim (int d)
+{
+  if (d <= 4)
+    return 4;
+
+  if (d <= 8)
+    return 8;
+
+  if (d <= 16)
+    return 16;
+
+  if (d <= 32)
+    return 32;
+
+  if (d <= 64)
+    return 64;
+
+  return -1;
+}
+
+// --------------------------------------------------------------------------
+// Points: y' = U_g^{-T} (x - mu) in FP64, scaled by ysc (and, for points
+// beyond the representable range, by a per-point power of two t_p), split
+// into FP16 hi/lo.  Layout: Yh[p][DP], Yl[p][DP]; tp[p] = (t_p, 1/t_p^2).
+// --------------------------------------------------------------------------
+__global__ void
+k_mixed_points (const double *__restrict__ X, int m, int d, int DP, const double *__restrict__ mu,
+                const double *__restrict__ Ug, double ysc, _Float16 *__restrict__ Yh, _Float16 *__restrict__ Yl,
+                float2 *__restrict__ tp)
+{
+  const long p = blockIdx.x * (long) blockDim.x + threadIdx.x;
+
+  if (p >= m)
+    return;
+
+  double y[64];
+  double ymax = 0.0;
+
+  for (int l = 0; l < d; ++l)
+  {
+    double t = X[p * d + l] - mu[l];
+
+    for (int q = 0; q < l; ++q)
+      t = fma (-y[q], Ug[q * d + l], t);
+
+    y[l] = t / Ug[l * d + l];
+    ymax = fmax (ymax, fabs (y[l]));
+  }
+
+  double sc = ysc;
+
+  // keep |sc y| below 2^14 (FP16 max is 65504); bounded for inf/NaN input
+  for (int it = 0; it < 80 && ymax * sc >= 16384.0; ++it)
+    sc *= 0.0625;
+
+  const double t_p = sc / ysc;
+
+  for (int l = 0; l < DP; ++l)
+  {
+    const double v     = (l < d) ? y[l] * sc : 0.0;
+    const _Float16 hi  = (_Float16) v;
+    const _Float16 lo  = (_Float16) (v - (double) hi);
+
+    Yh[p * DP + l] = hi;
+    Yl[p * DP + l] = lo;
+  }
+
+  tp[p] = make_float2 ((float) t_p, (float) (1.0 / (t_p * t_p)));
+}
+
+hipError_t
+launch_mixed_points (const MixedArgs &a, hipStream_t s)
+{
+  _Float16 *Yh = (_Float16 *) a.Yh;
+  _Float16 *Yl = Yh + (size_t) a.m * a.f.DP;
+  float2 *tp   = (float2 *) (Yl + (size_t) a.m * a.f.DP);
+
+  hipLaunchKernelGGL (k_mixed_points, dim3 ((a.m + 127) / 128), dim3 (128), 0, s, a.X, a.m, a.d, a.f.DP, a.mu, a.G,
+                      a.ysc, Yh, Yl, tp);
+
+  return hipGetLastError ();
+}
+
+// --------------------------------------------------------------------------
+// Kernel factors (FP64 -> split), one lane per kernel slot (padded to a whole
+// number of tiles; padding kernels get A = 0, c = 0, b2 = -inf).
+//   A (half): [n_pad][2][DP][DP] (hi, lo);  c (float): [n_pad][DP];
+//   coef (float4): (q_i or r_i, b2_i, 1/s_i^2, 0).
+// Wp holds W_i = U_i^{-T} packed by rows with padded dimension D; Yc holds
+// y'_i = U_g^{-T} (theta_i - mu).
+// --------------------------------------------------------------------------
+__global__ void
+k_mixed_factors (const double *__restrict__ Wp, int D, const double *__restrict__ Yc, const double *__restrict__ Ug,
+                 const double *__restrict__ b, double bref, int n, int n_pad, int d, int DP, double inv_h,
+                 double ysc, int kernel, double nu, _Float16 *__restrict__ A, float *__restrict__ c,
+                 float4 *__restrict__ coef)
+{
+  const long i = blockIdx.x * (long) blockDim.x + threadIdx.x;
+
+  if (i >= n_pad)
+    return;
+
+  _Float16 *Ah = A + i * 2 * (size_t) DP * DP;
+  _Float16 *Al = Ah + (size_t) DP * DP;
+  float *ci    = c + i * (size_t) DP;
+
+  if (i >= n)
+  {
+    for (int e = 0; e < DP * DP; ++e)
+    {
+      Ah[e] = (_Float16) 0.0f;
+      Al[e] = (_Float16) 0.0f;
+    }
+
+    for (int r = 0; r < DP; ++r)
+      ci[r] = 0.0f;
+
+    coef[i] = make_float4 (0.0f, -INFINITY, 0.0f, 0.0f);
+
+    return;
+  }
+
+  const double *W  = Wp + i * (size_t) (D * (D + 1) / 2);
+  const double *yi = Yc + i * (size_t) d;
+
+  // B[r][q] = sum_{k=q..r} W[r][k] U_g[q][k] / h
+  double bmax = 0.0;
+
+  for (int r = 0; r < d; ++r)
+    for (int q = 0; q < d; ++q)
+    {
+      double t = 0.0;
+
+      for (int k = q; k <= r; ++k)
+        t = fma (W[r * (r + 1) / 2 + k], Ug[q * d + k], t);
+
+      bmax = fmax (bmax, fabs (t * inv_h));
+    }
+
+  // s_i / ysc brings max |A_i| into [2^13, 2^14)
+  const int e      = (bmax > 0.0) ? ilogb (bmax / ysc) : 0;
+  const double s_i = ldexp (1.0, 13 - e);
+  const double sy  = s_i / ysc;
+
+  for (int r = 0; r < DP; ++r)
+  {
+    double cr = 0.0;
+
+    for (int q = 0; q < DP; ++q)
+    {
+      double v = 0.0;
+
+      if (r < d && q < d)
+      {
+        double t = 0.0;
+
+        for (int k = q; k <= r; ++k)
+          t = fma (W[r * (r + 1) / 2 + k], Ug[q * d + k], t);
+
+        v  = t * inv_h * sy;
+        cr = fma (v, yi[q] * ysc, cr);
+      }
+
+      const _Float16 hi = (_Float16) v;
+      const _Float16 lo = (_Float16) (v - (double) hi);
+
+      Ah[r * DP + q] = hi;
+      Al[r * DP + q] = lo;
+    }
+
+    ci[r] = (float) cr;
+  }
+
+  const double inv_s2 = 1.0 / (s_i * s_i);
+  const double qr     = (kernel == NCKDE_KERNEL_GAUSS) ? -0.5 * LOG2E * inv_s2 : inv_s2 / nu;
+
+  coef[i] = make_float4 ((float) qr, (float) ((b[i] - bref) * LOG2E), (float) inv_s2, 0.0f);
+}
+
+// b2_i refresh after the weights change
+__global__ void
+k_mixed_coef_b (const double *__restrict__ b, double bref, int n, float4 *__restrict__ coef)
+{
+  const long i = blockIdx.x * (long) blockDim.x + threadIdx.x;
+
+  if (i < n)
+    coef[i].y = (float) ((b[i] - bref) * LOG2E);
+}
+
+// y'_i = U_g^{-T} (theta_i - mu) for the kernel centres, and max |y'|
+__global__ void
+k_mixed_centres (const double *__restrict__ X, int n, int d, const double *__restrict__ mu,
+                 const double *__restrict__ Ug, double *__restrict__ Yc, unsigned long long *__restrict__ ymax_bits)
+{
+  const long i = blockIdx.x * (long) blockDim.x + threadIdx.x;
+
+  if (i >= n)
+    return;
+
+  double ym = 0.0;
+
+  for (int l = 0; l < d; ++l)
+  {
+    double t = X[i * d + l] - mu[l];
+
+    for (int q = 0; q < l; ++q)
+      t = fma (-Yc[i * d + q], Ug[q * d + l], t);
+
+    t            = t / Ug[l * d + l];
+    Yc[i * d + l] = t;
+    ym           = fmax (ym, fabs (t));
+  }
+
+  atomicMax (ymax_bits, (unsigned long long) __double_as_longlong (ym));
+}
+
+// column means of the sample (serial per column)
+__global__ void
+k_colmean (const double *__restrict__ X, int n, int d, double *__restrict__ mu)
+{
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+
+  if (l >= d)
+    return;
+
+  double s = 0.0;
+
+  for (int i = 0; i < n; ++i)
+    s += X[(size_t) i * d + l];
+
+  mu[l] = s / n;
+}
+
+// --------------------------------------------------------------------------
+// The evaluation kernel.  256 threads = 4 waves; each wave owns NPT tiles of
+// 32 points whose B fragments stay in registers; every wave of the grid walks
+// a range of kernel groups.
+// --------------------------------------------------------------------------
+template <int DP>
+__device__ inline half8
+load_a_frag (const _Float16 *__restrict__ A, int tg, int rt, int kb, int lohi, int lane)
+{
+  using P       = Plan<DP>;
+  const int g   = lane >> 5;
+  const int R   = lane & 31;
+  half8 v;
+
+  if (DP >= 16)
+  {
+    const int kern = (DP == 64) ? tg : tg * P::KPT + R / DP;
+    const int row  = (DP == 64) ? rt * 32 + R : R % DP;
+    const _Float16 *base = A + ((size_t) kern * 2 + lohi) * DP * DP + (size_t) row * DP + kb * 16 + g * 8;
+
+    v = *(const half8 *) base;
+  }
+  else if (DP == 8)
+  {
+    const int kern = tg * P::KPT + R / 8;
+    const int row  = R % 8;
+    const _Float16 *base = A + ((size_t) kern * 2 + g) * 64 + row * 8;  // g = 0: hi, 1: lo
+
+    v = *(const half8 *) base;
+  }
+  else
+  {
+    const int kern = tg * P::KPT + R / 4;
+    const int row  = R % 4;
+    const _Float16 *bh = A + (size_t) kern * 2 * 16 + row * 4;
+    const half4 h      = *(const half4 *) bh;
+    const half4 l      = *(const half4 *) (bh + 16);
+
+    v = (half8) {h[0], h[1], h[2], h[3], l[0], l[1], l[2], l[3]};
+  }
+
+  return v;
+}
+
+template <int DP>
+__device__ inline half8
+load_b_frag (const _Float16 *__restrict__ Yh, const _Float16 *__restrict__ Yl, int p, int kb, int lohi, int lane)
+{
+  const int g = lane >> 5;
+  half8 v;
+
+  if (DP >= 16)
+  {
+    const _Float16 *src = (lohi ? Yl : Yh) + (size_t) p * DP + kb * 16 + g * 8;
+
+    v = *(const half8 *) src;
+  }
+  else if (DP == 8)
+  {
+    v = *(const half8 *) ((lohi ? Yl : Yh) + (size_t) p * 8);
+  }
+  else
+  {
+    const half4 q = *(const half4 *) ((g ? Yl : Yh) + (size_t) p * 4);
+
+    v = (half8) {q[0], q[1], q[2], q[3], q[0], q[1], q[2], q[3]};
+  }
+
+  return v;
+}
+
+template <int KERNEL>
+__device__ inline void
+lse_push (float c2, float4 cf, float kappa, float &M, double &S)
+{
+  float lnt;
+
+  if (KERNEL == NCKDE_KERNEL_GAUSS)
+    lnt = fmaf (cf.x, c2, cf.y);
+  else
+    lnt = fmaf (kappa, __builtin_amdgcn_logf (fmaf (c2, cf.x, 1.0f)), cf.y);
+
+  if (lnt > M)
+  {
+    S = S * (double) __builtin_amdgcn_exp2f (M - lnt);
+    M = lnt;
+  }
+
+  const float e = __builtin_amdgcn_exp2f (lnt - M);
+
+  S += (lnt > -INFINITY) ? (double) e : 0.0;
+}
+
+template <int DP, int KERNEL, int NPT>
+__global__ void __launch_bounds__ (256)
+k_eval_mixed (MixedArgs a)
+{
+  using P        = Plan<DP>;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g    = lane >> 5;
+  const int R    = lane & 31;
+
+  const _Float16 *Yh = (const _Float16 *) a.Yh;
+  const _Float16 *Yl = Yh + (size_t) a.m * DP;
+  const float2 *tpv  = (const float2 *) (Yl + (size_t) a.m * DP);
+  const _Float16 *A  = (const _Float16 *) a.f.A;
+  const float *cv    = a.f.c;
+  const float4 *coef = (const float4 *) a.f.coef;
+  const float kappa  = (float) a.kappa;
+
+  const int tile0   = (blockIdx.x * 4 + wave) * NPT;
+  const int ngroups = (a.n + P::KPT - 1) / P::KPT;
+  const int chunk   = (ngroups + a.nsplit - 1) / a.nsplit;
+  const int g0      = blockIdx.y * chunk;
+  const int g1      = min (ngroups, g0 + chunk);
+
+  // B fragments (points) and per-point scales
+  half8 Bh[NPT][P::NKB], Bl[NPT][P::NKB];
+  float tscale[NPT], it2[NPT];
+  bool scaled = false;
+
+#pragma unroll
+  for (int t = 0; t < NPT; ++t)
+  {
+    int p = (tile0 + t) * 32 + R;
+
+    p = (p < a.m) ? p : a.m - 1;
+
+#pragma unroll
+    for (int kb = 0; kb < P::NKB; ++kb)
+    {
+      Bh[t][kb] = load_b_frag<DP> (Yh, Yl, p, kb, 0, lane);
+      Bl[t][kb] = load_b_frag<DP> (Yh, Yl, p, kb, 1, lane);
+    }
+
+    const float2 tv = tpv[p];
+
+    tscale[t] = tv.x;
+    it2[t]    = tv.y;
+    scaled    = scaled || (tv.x != 1.0f);
+  }
+
+  scaled = __any (scaled);
+
+  float M[NPT];
+  double S[NPT];
+
+#pragma unroll
+  for (int t = 0; t < NPT; ++t)
+  {
+    M[t] = -INFINITY;
+    S[t] = 0.0;
+  }
+
+  float tot_prev = 0.0f;
+
+  for (int tg = g0; tg < g1; ++tg)
+  {
+    half8 Ah[P::NRT][P::NKB], Al[P::NRT][P::NKB];
+    floatx16 cinit[P::NRT];
+
+#pragma unroll
+    for (int rt = 0; rt < P::NRT; ++rt)
+    {
+#pragma unroll
+      for (int kb = 0; kb < P::NKB; ++kb)
+      {
+        Ah[rt][kb] = load_a_frag<DP> (A, tg, rt, kb, 0, lane);
+
+        if (DP >= 16)
+          Al[rt][kb] = load_a_frag<DP> (A, tg, rt, kb, 1, lane);
+      }
+
+      // -c for the 16 rows held by this lane: rows 8 j + 4 g + r
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+      {
+        const int Rr   = 8 * j + 4 * g;
+        const int kern = (DP == 64) ? tg : tg * P::KPT + Rr / DP;
+        const int row  = (DP == 64) ? rt * 32 + Rr : Rr % DP;
+        const float4 c4 = *(const float4 *) (cv + (size_t) kern * DP + row);
+
+        cinit[rt][4 * j + 0] = -c4.x;
+        cinit[rt][4 * j + 1] = -c4.y;
+        cinit[rt][4 * j + 2] = -c4.z;
+        cinit[rt][4 * j + 3] = -c4.w;
+      }
+    }
+
+    // coefficients of the slots this lane owns
+    float4 cf[4];
+
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+      int slot = 0;
+
+      if (DP == 4)
+        slot = 2 * q + g;
+      else if (DP == 8)
+        slot = 2 * (q & 1) + g;
+      else if (DP == 16)
+        slot = g;
+
+      cf[q] = coef[(size_t) tg * P::KPT + slot];
+    }
+
+#pragma unroll
+    for (int t = 0; t < NPT; ++t)
+    {
+      floatx16 acc[P::NRT];
+
+#pragma unroll
+      for (int rt = 0; rt < P::NRT; ++rt)
+      {
+        floatx16 c0 = cinit[rt];
+
+        if (scaled)
+        {
+#pragma unroll
+          for (int r = 0; r < 16; ++r)
+            c0[r] *= tscale[t];
+        }
+
+        if (DP == 4)
+        {
+          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (Ah[rt][0], Bh[t][0], c0, 0, 0, 0);
+        }
+        else if (DP == 8)
+        {
+          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (Ah[rt][0], Bh[t][0], c0, 0, 0, 0);
+          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (Ah[rt][0], Bl[t][0], acc[rt], 0, 0, 0);
+        }
+        else
+        {
+          acc[rt] = c0;
+
+#pragma unroll
+          for (int kb = 0; kb < P::NKB; ++kb)
+          {
+            acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (Ah[rt][kb], Bh[t][kb], acc[rt], 0, 0, 0);
+            acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (Ah[rt][kb], Bl[t][kb], acc[rt], 0, 0, 0);
+            acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (Al[rt][kb], Bh[t][kb], acc[rt], 0, 0, 0);
+          }
+        }
+      }
+
+      // squared norms per kernel slot held by this lane
+      float part[P::NLOCAL];
+
+#pragma unroll
+      for (int j = 0; j < P::NLOCAL; ++j)
+        part[j] = 0.0f;
+
+#pragma unroll
+      for (int rt = 0; rt < P::NRT; ++rt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+        {
+          const int j = (DP <= 8) ? r / 4 : ((DP == 16) ? r / 8 : 0);
+
+          part[j] = fmaf (acc[rt][r], acc[rt][r], part[j]);
+        }
+
+      if (DP == 4)
+      {
+        // each kernel's 4 rows sit in one lane: slots 2 j + g
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          lse_push<KERNEL> (part[j] * (scaled ? it2[t] : 1.0f), cf[j], kappa, M[t], S[t]);
+      }
+      else if (DP == 8)
+      {
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+        {
+          const float send = g ? part[2 * q] : part[2 * q + 1];
+          const float keep = g ? part[2 * q + 1] : part[2 * q];
+          const float tot  = keep + __shfl_xor (send, 32, 64);
+
+          lse_push<KERNEL> (tot * (scaled ? it2[t] : 1.0f), cf[q], kappa, M[t], S[t]);
+        }
+      }
+      else if (DP == 16)
+      {
+        const float send = g ? part[0] : part[1];
+        const float keep = g ? part[1] : part[0];
+        const float tot  = keep + __shfl_xor (send, 32, 64);
+
+        lse_push<KERNEL> (tot * (scaled ? it2[t] : 1.0f), cf[0], kappa, M[t], S[t]);
+      }
+      else
+      {
+        // one kernel per tile group: both halves get the total; the lower
+        // half keeps even point tiles, the upper half odd ones, so the state
+        // M[t/2] of a lane belongs to its tile 2 (t/2) + g
+        const float tot = part[0] + __shfl_xor (part[0], 32, 64);
+
+        if (t & 1)
+        {
+          // tot_prev holds tile t-1: the lower half takes it, the upper half t
+          const float mine = g ? tot * (scaled ? it2[t] : 1.0f) : tot_prev;
+
+          lse_push<KERNEL> (mine, cf[0], kappa, M[t / 2], S[t / 2]);
+        }
+        else
+        {
+          tot_prev = tot * (scaled ? it2[t] : 1.0f);
+        }
+      }
+    }
+  }
+
+  if (DP >= 32)
+  {
+#pragma unroll
+    for (int t2 = 0; t2 < NPT / 2; ++t2)
+    {
+      const int p = (tile0 + 2 * t2 + g) * 32 + R;
+
+      if (p < a.m)
+      {
+        a.partM[(size_t) blockIdx.y * a.m + p] = M[t2];
+        a.partS[(size_t) blockIdx.y * a.m + p] = S[t2];
+      }
+    }
+
+    return;
+  }
+
+  // merge the two half-waves (same points, disjoint kernel slots)
+#pragma unroll
+  for (int t = 0; t < NPT; ++t)
+  {
+    const float Mo  = __shfl_xor (M[t], 32, 64);
+    const double So = __shfl_xor (S[t], 32, 64);
+    const float Mm  = fmaxf (M[t], Mo);
+    double Sm       = 0.0;
+
+    if (M[t] > -INFINITY)
+      Sm += S[t] * (double) __builtin_amdgcn_exp2f (M[t] - Mm);
+
+    if (Mo > -INFINITY)
+      Sm += So * (double) __builtin_amdgcn_exp2f (Mo - Mm);
+
+    const int p = (tile0 + t) * 32 + R;
+
+    if (g == 0 && p < a.m)
+    {
+      a.partM[(size_t) blockIdx.y * a.m + p] = Mm;
+      a.partS[(size_t) blockIdx.y * a.m + p] = Sm;
+    }
+  }
+}
+
+// log sum = b_ref + ln2 (M + log2 S); m2lnp = -2 (log sum - d ln h)
+__global__ void
+k_mixed_merge (MixedArgs a, double bref)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+
+  if (p >= a.m)
+    return;
+
+  const double *pS = a.partS;
+  float Mx         = -INFINITY;
+
+  for (int s = 0; s < a.nsplit; ++s)
+    Mx = fmaxf (Mx, a.partM[(size_t) s * a.m + p]);
+
+  double S = 0.0;
+
+  for (int s = 0; s < a.nsplit; ++s)
+  {
+    const float Ms = a.partM[(size_t) s * a.m + p];
+
+    if (Ms > -INFINITY)
+      S += pS[(size_t) s * a.m + p] * exp2 ((double) Ms - (double) Mx);
+  }
+
+  const double lnsum = bref + LN2 * ((double) Mx + log2 (S));
+  const double m2lnp = -2.0 * (lnsum - a.lnh_d);
+
+  a.out[p] = a.linear ? exp (-0.5 * m2lnp) : m2lnp;
+}
+
+template <int DP, int KERNEL>
+static hipError_t
+eval_t (const MixedArgs &a, int npt, hipStream_t s)
+{
+  constexpr int NPT = (DP == 64) ? 2 : 4;
+  const int pts     = 4 * NPT * 32;
+  dim3 grid ((a.m + pts - 1) / pts, a.nsplit);
+
+  (void) npt;
+  hipLaunchKernelGGL ((k_eval_mixed<DP, KERNEL, NPT>), grid, dim3 (256), 0, s, a);
+
+  return hipGetLastError ();
+}
+
+template <int DP>
+static hipError_t
+eval_d (const MixedArgs &a, hipStream_t s)
+{
+  return (a.kernel == NCKDE_KERNEL_GAUSS) ? eval_t<DP, NCKDE_KERNEL_GAUSS> (a, 0, s)
+                                          : eval_t<DP, NCKDE_KERNEL_ST> (a, 0, s);
+}
+
+int
+mixed_points_per_block (int DP)
+{
+  return 4 * ((DP == 64) ? 2 : 4) * 32;
+}
+
+hipError_t
+launch_eval_mixed (const MixedArgs &a, hipStream_t s)
+{
+  switch (a.f.DP)
+  {
+    case 4: return eval_d<4> (a, s);
+    case 8: return eval_d<8> (a, s);
+    case 16: return eval_d<16> (a, s);
+    case 32: return eval_d<32> (a, s);
+    case 64: return eval_d<64> (a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t
+launch_mixed_merge (const MixedArgs &a, double bref, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_mixed_merge, dim3 ((a.m + 255) / 256), dim3 (256), 0, s, a, bref);
+
+  return hipGetLastError ();
+}
+
+// factor build pipeline (called by the host at the first mixed eval after a
+// prepare, and when the weights change)
+hipError_t
+mixed_build (const double *X, int n, int d, int D, int DP, const double *Wp, const double *Ug, const double *b,
+             double bref, double inv_h, int kernel, double nu, double *mu, double *Yc,
+             unsigned long long *ymax_bits, double *ysc_out, _Float16 *A, float *c, float *coef, hipStream_t s)
+{
+  hipError_t e;
+
+  hipLaunchKernelGGL (k_colmean, dim3 (1), dim3 (64), 0, s, X, n, d, mu);
+  e = hipMemsetAsync (ymax_bits, 0, sizeof (unsigned long long), s);
+
+  if (e != hipSuccess)
+    return e;
+
+  hipLaunchKernelGGL (k_mixed_centres, dim3 ((n + 127) / 128), dim3 (128), 0, s, X, n, d, mu, Ug, Yc, ymax_bits);
+
+  unsigned long long bits = 0;
+
+  e = hipMemcpyAsync (&bits, ymax_bits, sizeof (bits), hipMemcpyDeviceToHost, s);
+
+  if (e != hipSuccess)
+    return e;
+
+  e = hipStreamSynchronize (s);
+
+  if (e != hipSuccess)
+    return e;
+
+  double ymax;
+
+  memcpy (&ymax, &bits, sizeof (double));
+
+  // the sample's largest |y'| maps to about 2^11: eval points up to 8x
+  // farther out need no per-point rescaling
+  const double ysc = (ymax > 0.0) ? ldexp (1.0, 11 - ilogb (ymax) - 1) : 1.0;
+  const int kpt    = (DP >= 32) ? 1 : 32 / DP;
+  const int n_pad  = ((n + kpt - 1) / kpt) * kpt;
+
+  *ysc_out = ysc;
+  hipLaunchKernelGGL (k_mixed_factors, dim3 ((n_pad + 63) / 64), dim3 (64), 0, s, Wp, D, Yc, Ug, b, bref, n, n_pad,
+                      d, DP, inv_h, ysc, kernel, nu, A, c, (float4 *) coef);
+
+  return hipGetLastError ();
+}
+
+hipError_t
+mixed_refresh_b (const double *b, double bref, int n, float *coef, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_mixed_coef_b, dim3 ((n + 255) / 256), dim3 (256), 0, s, b, bref, n, (float4 *) coef);
+
+  return hipGetLastError ();
+}
+
+} // namespace nck

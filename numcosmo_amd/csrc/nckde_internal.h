@@ -90,31 +90,35 @@ hipError_t launch_eval_merge (const EvalArgs &a, hipStream_t s);
 // ---- mixed-precision evaluation (k_eval_mixed.hip) ------------------------
 struct MixedFactors
 {
-  int DP;                 // padded MFMA dimension
-  const void *A;          // n_k x 2 x DP x DP fp16 (hi, lo), row-major per kernel
-  const float *c;         // n_k x DP
-  const float *coef;      // n_k x 4: (q_i, b2_i, -, -)
+  int DP;                 // padded MFMA dimension (4, 8, 16, 32, 64)
+  const void *A;          // n_pad x 2 x DP x DP fp16 (hi, lo), row-major per kernel
+  const float *c;         // n_pad x DP
+  const float *coef;      // n_pad x 4: (q_i or r_i, b2_i, 1/s_i^2, 0)
 };
 struct MixedArgs
 {
   int kernel, d, n, m, nsplit;
   double nu, kappa, lnh_d;
   MixedFactors f;
-  const double *X;        // points m x d (FP64)
+  const double *X;        // points m x d (FP64, device)
   const double *mu;       // centring (d)
-  const double *G;        // d x d: yc = G (x - mu) with G = U_g^{-T} as lower dense
-  double ysc;             // power-of-two scale applied to yc before splitting
-  void *Yh;               // m x DP fp16 hi/lo interleaved workspace (2 x m x DP halves)
-  float *partM, *partS;   // nsplit x m
+  const double *G;        // U_g (d x d, upper): y' = U_g^{-T} (x - mu)
+  double ysc;             // power-of-two scale of the whitened coordinates
+  void *Yh;               // workspace: Yh, Yl (m x DP fp16 each), then m float2
+  float *partM;           // nsplit x m
+  double *partS;          // nsplit x m
   double *out;
   int linear;
 };
-hipError_t launch_mixed_prepare_points (const MixedArgs &a, hipStream_t s);
+int mixed_points_per_block (int DP);
+hipError_t launch_mixed_points (const MixedArgs &a, hipStream_t s);
 hipError_t launch_eval_mixed (const MixedArgs &a, hipStream_t s);
-hipError_t launch_mixed_merge (const MixedArgs &a, hipStream_t s);
-hipError_t launch_mixed_factors (const double *Wp, const double *Th, const double *Ug, const double *mu,
-                                 const double *b, int nk, int d, int D, int DP, double inv_h, double ysc,
-                                 int kernel, double nu, void *A, float *c, float *coef, hipStream_t s);
+hipError_t launch_mixed_merge (const MixedArgs &a, double bref, hipStream_t s);
+hipError_t mixed_build (const double *X, int n, int d, int D, int DP, const double *Wp, const double *Ug,
+                        const double *b, double bref, double inv_h, int kernel, double nu, double *mu, double *Yc,
+                        unsigned long long *ymax_bits, double *ysc_out, _Float16 *A, float *c, float *coef,
+                        hipStream_t s);
+hipError_t mixed_refresh_b (const double *b, double bref, int n, float *coef, hipStream_t s);
 
 // ---- interpolation matrix & linear algebra (k_linalg.hip) ----------------
 struct IMArgs

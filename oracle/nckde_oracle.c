@@ -1974,6 +1974,13 @@ ok_sample (ok_sd *sd, double u, const double *z, double chi2, double *x)
   return i;
 }
 
+void
+ok_set_weights (ok_sd *sd, const double *w)
+{
+  memcpy (sd->weights, w, sizeof (double) * sd->n_kernels);
+  sd->wcum_ready = 0;
+}
+
 int ok_get_dim (ok_sd *sd) { return sd->d; }
 int ok_get_sample_size (ok_sd *sd) { return sd->n_obs; }
 int ok_get_n_kernels (ok_sd *sd) { return sd->n_kernels; }

@@ -74,6 +74,9 @@ void   ok_eval_m2lnp_batch (ok_sd *sd, const double *X, int m, double *out);
 void   ok_eval_batch (ok_sd *sd, const double *X, int m, double *out);
 int    ok_kernel_choose (ok_sd *sd, double u);
 int    ok_sample (ok_sd *sd, double u, const double *z, double chi2, double *x);
+/* replaces the weights of a prepared object (checker hook: evaluate with
+ * weights fitted elsewhere) */
+void   ok_set_weights (ok_sd *sd, const double *w);
 
 /* peeks (valid after prepare) */
 int    ok_get_dim (ok_sd *sd);

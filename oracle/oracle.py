@@ -76,6 +76,7 @@ def lib():
             "ok_eval_batch": (None, [C.c_void_p, _dp, C.c_int, _dp]),
             "ok_kernel_choose": (C.c_int, [C.c_void_p, C.c_double]),
             "ok_sample": (C.c_int, [C.c_void_p, C.c_double, _dp, C.c_double, _dp]),
+            "ok_set_weights": (None, [C.c_void_p, _dp]),
             "ok_get_dim": (C.c_int, [C.c_void_p]),
             "ok_get_sample_size": (C.c_int, [C.c_void_p]),
             "ok_get_n_kernels": (C.c_int, [C.c_void_p]),
@@ -271,6 +272,9 @@ class StatsDist:
         x = np.empty(self.d)
         i = lib().ok_sample(self._h, float(u), _f64(z), float(chi2), x)
         return i, x
+
+    def set_weights(self, w):
+        lib().ok_set_weights(self._h, _f64(w))
 
     # peeks
     def get_dim(self): return lib().ok_get_dim(self._h)

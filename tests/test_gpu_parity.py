@@ -251,3 +251,59 @@ def test_nearpd_path_small_k(gpu, oracle):
         assert np.allclose(Ug.T @ Ug, Uo.T @ Uo, rtol=1e-9, atol=1e-12)
     Xe = g.normal(size=(50, d))
     assert np.allclose(sd.eval_m2lnp_batch(Xe), od.eval_m2lnp_batch(Xe), rtol=1e-9)
+
+
+MIXED_CASES = [
+    # kernel, d (padded MFMA dim 4, 8, 16, 32, 64), n, nu
+    (GAUSS, 3, 700, 0.0),
+    (ST, 8, 1500, 3.0),
+    (ST, 12, 900, 1.0),
+    (GAUSS, 16, 2000, 0.0),
+    (ST, 24, 1200, 5.0),
+    (ST, 40, 1300, 5.0),
+]
+
+
+@pytest.mark.parametrize("kernel,d,n,nu", MIXED_CASES)
+def test_eval_m2lnp_mixed_precision(gpu, oracle, kernel, d, n, nu):
+    """FP16-split MFMA distances with FP32 kernel sums: relative m2lnp error
+    |dm2lnp| / max(|m2lnp|, 1) <= 1e-6 against the FP64 oracle."""
+    _say(f"[eval mixed] kernel={kernel} d={d} n={n}")
+    g = np.random.default_rng(5000 + d)
+    tgt = synth.Gaussian(d, g)
+    X = tgt.draw(g, n)
+    sd, od = _pair(gpu, oracle, 1, kernel, d, X, nu)
+    sd.prepare_interp(tgt.m2lnp(X)) if d <= 16 else sd.prepare()
+    od.prepare()
+    od.set_weights(sd.peek_weights())
+    Xe = tgt.draw(g, 1000)
+    Xe[:5] = X[:5]                 # on kernel centres
+    Xe[5] = 50.0 * Xe[5]           # far outlier: per-point rescaled path
+    Xe[6] = 1.0e4 * Xe[6]
+    ref = od.eval_m2lnp_batch(Xe)
+    got = sd.eval_m2lnp_batch(Xe, gpu.Precision.MIXED)
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)
+    _say(f"   mixed max rel err {err.max():.3e} p99 {np.quantile(err, 0.99):.3e}")
+    assert np.all(np.isfinite(got))
+    assert err.max() <= 1e-6
+    # batch composition does not change a point's result beyond the tolerance
+    alone = sd.eval_m2lnp_batch(Xe[10:11], gpu.Precision.MIXED)
+    assert abs(alone[0] - got[10]) <= 1e-6 * max(abs(ref[10]), 1.0)
+
+
+def test_mixed_after_refit_uses_new_weights(gpu, oracle):
+    g = np.random.default_rng(5100)
+    d, n = 8, 800
+    tgt = synth.Banana(d)
+    X = tgt.draw(g, n)
+    sd, od = _pair(gpu, oracle, 1, ST, d, X, 3.0)
+    sd.prepare()
+    Xe = tgt.draw(g, 300)
+    a = sd.eval_m2lnp_batch(Xe, gpu.Precision.MIXED)
+    sd.prepare_interp(tgt.m2lnp(X))
+    b = sd.eval_m2lnp_batch(Xe, gpu.Precision.MIXED)
+    od.prepare()
+    od.set_weights(sd.peek_weights())
+    ref = od.eval_m2lnp_batch(Xe)
+    assert np.max(np.abs(b - ref) / np.maximum(np.abs(ref), 1.0)) <= 1e-6
+    assert np.max(np.abs(a - b)) > 1e-3
