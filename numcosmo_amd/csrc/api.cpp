@@ -107,7 +107,7 @@ struct nckde
   DBuf dX, dY, dcovg, dUg, dlng, dcov, dU, dinfo, dlnnorm, dw, db, dWp, dTh, dknn, dkeys, dscratch, dstatus,
        dwcum, dwhich;
   // mixed-precision factors (built at the first mixed eval after a prepare)
-  DBuf dmu, dYc, dymax, dA16, dc32, dcoef, dYw, dpartMf;
+  DBuf dmu, dYc, dymax, dA16, dc32, dcoef, dYw, dpartMf, dflag;
   bool mixed_ready = false, mixed_b_ready = false;
   double ysc = 1.0, bref = 0.0;
   // work buffers
@@ -879,7 +879,7 @@ eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipS
   const int kpt     = (DP >= 32) ? 1 : 32 / DP;
   const int ngroups = (int) ((h->n_k + kpt - 1) / kpt);
   const int bx      = (int) ((m + nck::mixed_points_per_block (DP) - 1) / nck::mixed_points_per_block (DP));
-  int nsplit        = (1024 + bx - 1) / bx;
+  int nsplit        = (2048 + bx - 1) / bx;
 
   nsplit = std::max (1, std::min (nsplit, std::max (1, ngroups / 8)));
 
@@ -889,6 +889,7 @@ eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipS
   h->last_stream = s;
 
   HIPCHK (h->dYw.ensure (m * (2 * sizeof (_Float16) * DP + sizeof (float) * 2)));
+  HIPCHK (h->dflag.ensure (m * (sizeof (int) + sizeof (float))));
   HIPCHK (h->dpartMf.ensure (sizeof (float) * m * nsplit));
   HIPCHK (h->dpartS.ensure (sizeof (double) * m * nsplit));
 
@@ -913,12 +914,14 @@ eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipS
   a.Yh     = h->dYw.p;
   a.partM  = h->dpartMf.as<float> ();
   a.partS  = h->dpartS.as<double> ();
+  a.flag   = h->dflag.as<int> ();
+  a.M0     = (float *) (h->dflag.as<int> () + m);
+  a.bref   = h->bref;
   a.out    = dout;
   a.linear = linear;
 
   HIPCHK (nck::launch_mixed_points (a, s));
   HIPCHK (nck::launch_eval_mixed (a, s));
-  HIPCHK (nck::launch_mixed_merge (a, h->bref, s));
 
   return NCKDE_OK;
 }

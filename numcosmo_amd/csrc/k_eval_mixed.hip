@@ -349,9 +349,26 @@ load_b_frag (const _Float16 *__restrict__ Yh, const _Float16 *__restrict__ Yl, i
   return v;
 }
 
-template <int KERNEL>
+// Pair reduction across the half-waves with one v_permlane32_swap (it swaps
+// lanes 32-63 of its first operand with lanes 0-31 of its second).  With
+// (a, b) the partial sums of two kernel slots, lanes 0-31 get a[L] + a[L+32]
+// and lanes 32-63 get b[L] + b[L-32]: each half-wave receives the total of
+// the slot it owns.
+__device__ inline float
+halfwave_pair_sum (float a, float b)
+{
+  const auto r = __builtin_amdgcn_permlane32_swap (__float_as_uint (a), __float_as_uint (b), false, false);
+
+  return __uint_as_float (r[0]) + __uint_as_float (r[1]);
+}
+
+// All lnt2 <= 0: b2_i is taken relative to max_j b_j and the kernel factor is
+// <= 1.  The sum therefore runs against a fixed reference M0 (0 in the main
+// pass) with no rescaling; Mb tracks the largest term so that points whose
+// terms all fall below 2^-100 can be recomputed against their own maximum.
+template <int KERNEL, bool OFFSET>
 __device__ inline void
-lse_push (float c2, float4 cf, float kappa, float &M, double &S)
+lse_push (float c2, float4 cf, float kappa, float M0, float &Mb, float &S32)
 {
   float lnt;
 
@@ -360,22 +377,258 @@ lse_push (float c2, float4 cf, float kappa, float &M, double &S)
   else
     lnt = fmaf (kappa, __builtin_amdgcn_logf (fmaf (c2, cf.x, 1.0f)), cf.y);
 
-  if (lnt > M)
-  {
-    S = S * (double) __builtin_amdgcn_exp2f (M - lnt);
-    M = lnt;
-  }
+  Mb = fmaxf (Mb, lnt);
 
-  const float e = __builtin_amdgcn_exp2f (lnt - M);
+  if (OFFSET)
+    lnt -= M0;
 
-  S += (lnt > -INFINITY) ? (double) e : 0.0;
+  S32 += __builtin_amdgcn_exp2f (lnt);
 }
 
-template <int DP, int KERNEL, int NPT>
+template <int DP>
+struct GroupData
+{
+  half8 Ah[Plan<DP>::NRT][Plan<DP>::NKB];
+  half8 Al[Plan<DP>::NRT][Plan<DP>::NKB];
+  floatx16 cinit[Plan<DP>::NRT];
+  float4 cf[4];
+};
+
+template <int DP>
+__device__ inline void
+load_group (GroupData<DP> &G, const _Float16 *__restrict__ A, const float *__restrict__ cv,
+            const float4 *__restrict__ coef, int tg, int lane)
+{
+  using P     = Plan<DP>;
+  const int g = lane >> 5;
+
+#pragma unroll
+  for (int rt = 0; rt < P::NRT; ++rt)
+  {
+#pragma unroll
+    for (int kb = 0; kb < P::NKB; ++kb)
+    {
+      G.Ah[rt][kb] = load_a_frag<DP> (A, tg, rt, kb, 0, lane);
+
+      if (DP >= 16)
+        G.Al[rt][kb] = load_a_frag<DP> (A, tg, rt, kb, 1, lane);
+    }
+
+    // -c for the 16 rows held by this lane: rows 8 j + 4 g + r
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+    {
+      const int Rr    = 8 * j + 4 * g;
+      const int kern  = (DP == 64) ? tg : tg * P::KPT + Rr / DP;
+      const int row   = (DP == 64) ? rt * 32 + Rr : Rr % DP;
+      const float4 c4 = *(const float4 *) (cv + (size_t) kern * DP + row);
+
+      G.cinit[rt][4 * j + 0] = -c4.x;
+      G.cinit[rt][4 * j + 1] = -c4.y;
+      G.cinit[rt][4 * j + 2] = -c4.z;
+      G.cinit[rt][4 * j + 3] = -c4.w;
+    }
+  }
+
+  // coefficients of the kernel slots this lane owns
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+  {
+    int slot = 0;
+
+    if (DP == 4)
+      slot = 2 * q + g;
+    else if (DP == 8)
+      slot = 2 * (q & 1) + g;
+    else if (DP == 16)
+      slot = g;
+
+    G.cf[q] = coef[(size_t) tg * P::KPT + slot];
+  }
+}
+
+// One kernel group against the NPT point tiles of a wave: all MFMA chains
+// first (separate accumulators), then the VALU reduction and sums, so the
+// matrix cores of tile t+1 overlap the vector work of tile t.
+template <int DP, int KERNEL, int NPT, bool OFFSET, bool SCALED, int NST>
+__device__ inline void
+group_step (const GroupData<DP> &cur, const half8 (&Bh)[NPT][Plan<DP>::NKB], const half8 (&Bl)[NPT][Plan<DP>::NKB],
+            const float (&tscale)[NPT], const float (&it2)[NPT], const float (&M0)[NST], float (&Mb)[NST],
+            float (&S32)[NST], float kappa, int g)
+{
+  using P = Plan<DP>;
+  floatx16 acc[NPT][P::NRT];
+
+#pragma unroll
+  for (int t = 0; t < NPT; ++t)
+  {
+#pragma unroll
+    for (int rt = 0; rt < P::NRT; ++rt)
+    {
+      floatx16 c0 = cur.cinit[rt];
+
+      if (SCALED)
+      {
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+          c0[r] *= tscale[t];
+      }
+
+      if (DP == 4)
+      {
+        acc[t][rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (cur.Ah[rt][0], Bh[t][0], c0, 0, 0, 0);
+      }
+      else if (DP == 8)
+      {
+        acc[t][rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (cur.Ah[rt][0], Bh[t][0], c0, 0, 0, 0);
+        acc[t][rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (cur.Ah[rt][0], Bl[t][0], acc[t][rt], 0, 0, 0);
+      }
+      else
+      {
+        acc[t][rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (cur.Ah[rt][0], Bh[t][0], c0, 0, 0, 0);
+        acc[t][rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (cur.Ah[rt][0], Bl[t][0], acc[t][rt], 0, 0, 0);
+        acc[t][rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (cur.Al[rt][0], Bh[t][0], acc[t][rt], 0, 0, 0);
+
+#pragma unroll
+        for (int kb = 1; kb < P::NKB; ++kb)
+        {
+          acc[t][rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (cur.Ah[rt][kb], Bh[t][kb], acc[t][rt], 0, 0, 0);
+          acc[t][rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (cur.Ah[rt][kb], Bl[t][kb], acc[t][rt], 0, 0, 0);
+          acc[t][rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (cur.Al[rt][kb], Bh[t][kb], acc[t][rt], 0, 0, 0);
+        }
+      }
+    }
+  }
+
+  float tot_prev = 0.0f;
+
+#pragma unroll
+  for (int t = 0; t < NPT; ++t)
+  {
+    // squared norms per kernel slot held by this lane
+    float part[P::NLOCAL];
+
+#pragma unroll
+    for (int j = 0; j < P::NLOCAL; ++j)
+      part[j] = 0.0f;
+
+#pragma unroll
+    for (int rt = 0; rt < P::NRT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+      {
+        const int j = (DP <= 8) ? r / 4 : ((DP == 16) ? r / 8 : 0);
+
+        part[j] = fmaf (acc[t][rt][r], acc[t][rt][r], part[j]);
+      }
+
+    const float sc2 = SCALED ? it2[t] : 1.0f;
+
+    if (DP == 4)
+    {
+      // each kernel's 4 rows sit in one lane: slots 2 j + g
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        lse_push<KERNEL, OFFSET> (part[j] * sc2, cur.cf[j], kappa, M0[t], Mb[t], S32[t]);
+    }
+    else if (DP == 8)
+    {
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+      {
+        const float tot = halfwave_pair_sum (part[2 * q], part[2 * q + 1]);
+
+        lse_push<KERNEL, OFFSET> (tot * sc2, cur.cf[q], kappa, M0[t], Mb[t], S32[t]);
+      }
+    }
+    else if (DP == 16)
+    {
+      const float tot = halfwave_pair_sum (part[0], part[1]);
+
+      lse_push<KERNEL, OFFSET> (tot * sc2, cur.cf[0], kappa, M0[t], Mb[t], S32[t]);
+    }
+    else
+    {
+      // one kernel per group: the lower half keeps even point tiles, the
+      // upper half odd ones; state q of a lane is its tile 2 q + g
+      const float tot = halfwave_pair_sum (part[0], part[0]) * sc2;
+
+      if (t & 1)
+      {
+        const float mine = g ? tot : tot_prev;
+
+        lse_push<KERNEL, OFFSET> (mine, cur.cf[0], kappa, M0[t / 2], Mb[t / 2], S32[t / 2]);
+      }
+      else
+      {
+        tot_prev = tot;
+      }
+    }
+  }
+}
+
+template <int DP, int KERNEL, int NPT, bool OFFSET, bool SCALED, int NST>
+__device__ inline void
+group_loop (int g0, int g1, const _Float16 *__restrict__ A, const float *__restrict__ cv,
+            const float4 *__restrict__ coef, const half8 (&Bh)[NPT][Plan<DP>::NKB],
+            const half8 (&Bl)[NPT][Plan<DP>::NKB], const float (&tscale)[NPT], const float (&it2)[NPT],
+            const float (&M0)[NST], float (&Mb)[NST], double (&S64)[NST], float kappa, int lane)
+{
+  const int g = lane >> 5;
+  float S32[NST];
+
+#pragma unroll
+  for (int q = 0; q < NST; ++q)
+    S32[q] = 0.0f;
+
+  // two register sets alternate roles: while one group is computed the next
+  // one is loaded into the other set (no register copies)
+  GroupData<DP> ga, gb;
+  int tg = g0;
+
+  if (tg < g1)
+    load_group<DP> (ga, A, cv, coef, tg, lane);
+
+  while (tg < g1)
+  {
+    if (tg + 1 < g1)
+      load_group<DP> (gb, A, cv, coef, tg + 1, lane);
+
+    group_step<DP, KERNEL, NPT, OFFSET, SCALED, NST> (ga, Bh, Bl, tscale, it2, M0, Mb, S32, kappa, g);
+    ++tg;
+
+    if (tg >= g1)
+      break;
+
+    if (tg + 1 < g1)
+      load_group<DP> (ga, A, cv, coef, tg + 1, lane);
+
+    group_step<DP, KERNEL, NPT, OFFSET, SCALED, NST> (gb, Bh, Bl, tscale, it2, M0, Mb, S32, kappa, g);
+    ++tg;
+
+    // fold the FP32 partial sums into FP64 every 16 groups
+    if (((tg - g0) & 15) == 0)
+    {
+#pragma unroll
+      for (int q = 0; q < NST; ++q)
+      {
+        S64[q] += (double) S32[q];
+        S32[q] = 0.0f;
+      }
+    }
+  }
+
+#pragma unroll
+  for (int q = 0; q < NST; ++q)
+    S64[q] += (double) S32[q];
+}
+
+template <int DP, int KERNEL, int NPT, bool OFFSET>
 __global__ void __launch_bounds__ (256)
 k_eval_mixed (MixedArgs a)
 {
   using P        = Plan<DP>;
+  constexpr int NST = (DP >= 32) ? NPT / 2 : NPT;   // LSE states per lane
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g    = lane >> 5;
@@ -394,6 +647,39 @@ k_eval_mixed (MixedArgs a)
   const int chunk   = (ngroups + a.nsplit - 1) / a.nsplit;
   const int g0      = blockIdx.y * chunk;
   const int g1      = min (ngroups, g0 + chunk);
+
+  // second pass: only waves holding a flagged point do any work
+  float M0[NST];
+
+#pragma unroll
+  for (int q = 0; q < NST; ++q)
+    M0[q] = 0.0f;
+
+  if (OFFSET)
+  {
+    int any = 0;
+
+#pragma unroll
+    for (int t = 0; t < NPT; ++t)
+    {
+      const int p = (tile0 + t) * 32 + R;
+
+      if (p < a.m)
+        any |= a.flag[p];
+    }
+
+    if (!__any (any))
+      return;
+
+#pragma unroll
+    for (int q = 0; q < NST; ++q)
+    {
+      const int t = (DP >= 32) ? 2 * q + g : q;
+      const int p = min ((tile0 + t) * 32 + R, a.m - 1);
+
+      M0[q] = a.M0[p];
+    }
+  }
 
   // B fragments (points) and per-point scales
   half8 Bh[NPT][P::NKB], Bl[NPT][P::NKB];
@@ -421,260 +707,108 @@ k_eval_mixed (MixedArgs a)
     scaled    = scaled || (tv.x != 1.0f);
   }
 
-  scaled = __any (scaled);
-
-  float M[NPT];
-  double S[NPT];
+  float Mb[NST];
+  double S64[NST];
 
 #pragma unroll
-  for (int t = 0; t < NPT; ++t)
+  for (int q = 0; q < NST; ++q)
   {
-    M[t] = -INFINITY;
-    S[t] = 0.0;
+    Mb[q]  = -INFINITY;
+    S64[q] = 0.0;
   }
 
-  float tot_prev = 0.0f;
-
-  for (int tg = g0; tg < g1; ++tg)
-  {
-    half8 Ah[P::NRT][P::NKB], Al[P::NRT][P::NKB];
-    floatx16 cinit[P::NRT];
-
-#pragma unroll
-    for (int rt = 0; rt < P::NRT; ++rt)
-    {
-#pragma unroll
-      for (int kb = 0; kb < P::NKB; ++kb)
-      {
-        Ah[rt][kb] = load_a_frag<DP> (A, tg, rt, kb, 0, lane);
-
-        if (DP >= 16)
-          Al[rt][kb] = load_a_frag<DP> (A, tg, rt, kb, 1, lane);
-      }
-
-      // -c for the 16 rows held by this lane: rows 8 j + 4 g + r
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-      {
-        const int Rr   = 8 * j + 4 * g;
-        const int kern = (DP == 64) ? tg : tg * P::KPT + Rr / DP;
-        const int row  = (DP == 64) ? rt * 32 + Rr : Rr % DP;
-        const float4 c4 = *(const float4 *) (cv + (size_t) kern * DP + row);
-
-        cinit[rt][4 * j + 0] = -c4.x;
-        cinit[rt][4 * j + 1] = -c4.y;
-        cinit[rt][4 * j + 2] = -c4.z;
-        cinit[rt][4 * j + 3] = -c4.w;
-      }
-    }
-
-    // coefficients of the slots this lane owns
-    float4 cf[4];
-
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-    {
-      int slot = 0;
-
-      if (DP == 4)
-        slot = 2 * q + g;
-      else if (DP == 8)
-        slot = 2 * (q & 1) + g;
-      else if (DP == 16)
-        slot = g;
-
-      cf[q] = coef[(size_t) tg * P::KPT + slot];
-    }
-
-#pragma unroll
-    for (int t = 0; t < NPT; ++t)
-    {
-      floatx16 acc[P::NRT];
-
-#pragma unroll
-      for (int rt = 0; rt < P::NRT; ++rt)
-      {
-        floatx16 c0 = cinit[rt];
-
-        if (scaled)
-        {
-#pragma unroll
-          for (int r = 0; r < 16; ++r)
-            c0[r] *= tscale[t];
-        }
-
-        if (DP == 4)
-        {
-          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (Ah[rt][0], Bh[t][0], c0, 0, 0, 0);
-        }
-        else if (DP == 8)
-        {
-          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (Ah[rt][0], Bh[t][0], c0, 0, 0, 0);
-          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (Ah[rt][0], Bl[t][0], acc[rt], 0, 0, 0);
-        }
-        else
-        {
-          acc[rt] = c0;
-
-#pragma unroll
-          for (int kb = 0; kb < P::NKB; ++kb)
-          {
-            acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (Ah[rt][kb], Bh[t][kb], acc[rt], 0, 0, 0);
-            acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (Ah[rt][kb], Bl[t][kb], acc[rt], 0, 0, 0);
-            acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16 (Al[rt][kb], Bh[t][kb], acc[rt], 0, 0, 0);
-          }
-        }
-      }
-
-      // squared norms per kernel slot held by this lane
-      float part[P::NLOCAL];
-
-#pragma unroll
-      for (int j = 0; j < P::NLOCAL; ++j)
-        part[j] = 0.0f;
-
-#pragma unroll
-      for (int rt = 0; rt < P::NRT; ++rt)
-#pragma unroll
-        for (int r = 0; r < 16; ++r)
-        {
-          const int j = (DP <= 8) ? r / 4 : ((DP == 16) ? r / 8 : 0);
-
-          part[j] = fmaf (acc[rt][r], acc[rt][r], part[j]);
-        }
-
-      if (DP == 4)
-      {
-        // each kernel's 4 rows sit in one lane: slots 2 j + g
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          lse_push<KERNEL> (part[j] * (scaled ? it2[t] : 1.0f), cf[j], kappa, M[t], S[t]);
-      }
-      else if (DP == 8)
-      {
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-        {
-          const float send = g ? part[2 * q] : part[2 * q + 1];
-          const float keep = g ? part[2 * q + 1] : part[2 * q];
-          const float tot  = keep + __shfl_xor (send, 32, 64);
-
-          lse_push<KERNEL> (tot * (scaled ? it2[t] : 1.0f), cf[q], kappa, M[t], S[t]);
-        }
-      }
-      else if (DP == 16)
-      {
-        const float send = g ? part[0] : part[1];
-        const float keep = g ? part[1] : part[0];
-        const float tot  = keep + __shfl_xor (send, 32, 64);
-
-        lse_push<KERNEL> (tot * (scaled ? it2[t] : 1.0f), cf[0], kappa, M[t], S[t]);
-      }
-      else
-      {
-        // one kernel per tile group: both halves get the total; the lower
-        // half keeps even point tiles, the upper half odd ones, so the state
-        // M[t/2] of a lane belongs to its tile 2 (t/2) + g
-        const float tot = part[0] + __shfl_xor (part[0], 32, 64);
-
-        if (t & 1)
-        {
-          // tot_prev holds tile t-1: the lower half takes it, the upper half t
-          const float mine = g ? tot * (scaled ? it2[t] : 1.0f) : tot_prev;
-
-          lse_push<KERNEL> (mine, cf[0], kappa, M[t / 2], S[t / 2]);
-        }
-        else
-        {
-          tot_prev = tot * (scaled ? it2[t] : 1.0f);
-        }
-      }
-    }
-  }
+  if (__any (scaled))
+    group_loop<DP, KERNEL, NPT, OFFSET, true, NST> (g0, g1, A, cv, coef, Bh, Bl, tscale, it2, M0, Mb, S64, kappa,
+                                                    lane);
+  else
+    group_loop<DP, KERNEL, NPT, OFFSET, false, NST> (g0, g1, A, cv, coef, Bh, Bl, tscale, it2, M0, Mb, S64, kappa,
+                                                     lane);
 
   if (DP >= 32)
   {
 #pragma unroll
-    for (int t2 = 0; t2 < NPT / 2; ++t2)
+    for (int q = 0; q < NST; ++q)
     {
-      const int p = (tile0 + 2 * t2 + g) * 32 + R;
+      const int p = (tile0 + 2 * q + g) * 32 + R;
 
       if (p < a.m)
       {
-        a.partM[(size_t) blockIdx.y * a.m + p] = M[t2];
-        a.partS[(size_t) blockIdx.y * a.m + p] = S[t2];
+        a.partM[(size_t) blockIdx.y * a.m + p] = Mb[q];
+        a.partS[(size_t) blockIdx.y * a.m + p] = S64[q];
       }
     }
 
     return;
   }
 
-  // merge the two half-waves (same points, disjoint kernel slots)
+  // the two half-waves hold the same points (disjoint kernel slots) and the
+  // same reference: add the sums, take the larger maximum
 #pragma unroll
   for (int t = 0; t < NPT; ++t)
   {
-    const float Mo  = __shfl_xor (M[t], 32, 64);
-    const double So = __shfl_xor (S[t], 32, 64);
-    const float Mm  = fmaxf (M[t], Mo);
-    double Sm       = 0.0;
-
-    if (M[t] > -INFINITY)
-      Sm += S[t] * (double) __builtin_amdgcn_exp2f (M[t] - Mm);
-
-    if (Mo > -INFINITY)
-      Sm += So * (double) __builtin_amdgcn_exp2f (Mo - Mm);
-
-    const int p = (tile0 + t) * 32 + R;
+    const float Mo  = __shfl_xor (Mb[t], 32, 64);
+    const double So = __shfl_xor (S64[t], 32, 64);
+    const int p     = (tile0 + t) * 32 + R;
 
     if (g == 0 && p < a.m)
     {
-      a.partM[(size_t) blockIdx.y * a.m + p] = Mm;
-      a.partS[(size_t) blockIdx.y * a.m + p] = Sm;
+      a.partM[(size_t) blockIdx.y * a.m + p] = fmaxf (Mb[t], Mo);
+      a.partS[(size_t) blockIdx.y * a.m + p] = S64[t] + So;
     }
   }
 }
 
-// log sum = b_ref + ln2 (M + log2 S); m2lnp = -2 (log sum - d ln h)
+// Below 2^-100 the FP32 terms lose range: such points are recomputed against
+// their own largest term.
+constexpr float MIXED_FLOOR = -100.0f;
+
+// pass 1: log sum = b_ref + ln2 (M0 + log2 S); flag points that underflowed
 __global__ void
-k_mixed_merge (MixedArgs a, double bref)
+k_mixed_merge (MixedArgs a, double bref, int second)
 {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
 
   if (p >= a.m)
     return;
 
+  if (second && !a.flag[p])
+    return;
+
   const double *pS = a.partS;
   float Mx         = -INFINITY;
-
-  for (int s = 0; s < a.nsplit; ++s)
-    Mx = fmaxf (Mx, a.partM[(size_t) s * a.m + p]);
-
-  double S = 0.0;
+  double S         = 0.0;
 
   for (int s = 0; s < a.nsplit; ++s)
   {
-    const float Ms = a.partM[(size_t) s * a.m + p];
-
-    if (Ms > -INFINITY)
-      S += pS[(size_t) s * a.m + p] * exp2 ((double) Ms - (double) Mx);
+    Mx = fmaxf (Mx, a.partM[(size_t) s * a.m + p]);
+    S += pS[(size_t) s * a.m + p];
   }
 
-  const double lnsum = bref + LN2 * ((double) Mx + log2 (S));
+  const float M0 = second ? a.M0[p] : 0.0f;
+
+  if (!second)
+  {
+    const int bad = (Mx < MIXED_FLOOR) || !(S > 0.0);
+
+    a.flag[p] = bad;
+    a.M0[p]   = (Mx > -INFINITY) ? Mx : 0.0f;
+  }
+
+  const double lnsum = bref + LN2 * ((double) M0 + log2 (S));
   const double m2lnp = -2.0 * (lnsum - a.lnh_d);
 
   a.out[p] = a.linear ? exp (-0.5 * m2lnp) : m2lnp;
 }
 
-template <int DP, int KERNEL>
+template <int DP, int KERNEL, bool OFFSET>
 static hipError_t
-eval_t (const MixedArgs &a, int npt, hipStream_t s)
+eval_t (const MixedArgs &a, hipStream_t s)
 {
   constexpr int NPT = (DP == 64) ? 2 : 4;
   const int pts     = 4 * NPT * 32;
   dim3 grid ((a.m + pts - 1) / pts, a.nsplit);
 
-  (void) npt;
-  hipLaunchKernelGGL ((k_eval_mixed<DP, KERNEL, NPT>), grid, dim3 (256), 0, s, a);
+  hipLaunchKernelGGL ((k_eval_mixed<DP, KERNEL, NPT, OFFSET>), grid, dim3 (256), 0, s, a);
 
   return hipGetLastError ();
 }
@@ -683,8 +817,33 @@ template <int DP>
 static hipError_t
 eval_d (const MixedArgs &a, hipStream_t s)
 {
-  return (a.kernel == NCKDE_KERNEL_GAUSS) ? eval_t<DP, NCKDE_KERNEL_GAUSS> (a, 0, s)
-                                          : eval_t<DP, NCKDE_KERNEL_ST> (a, 0, s);
+  hipError_t e;
+
+  if (a.kernel == NCKDE_KERNEL_GAUSS)
+  {
+    e = eval_t<DP, NCKDE_KERNEL_GAUSS, false> (a, s);
+
+    if (e == hipSuccess)
+      e = launch_mixed_merge (a, a.bref, 0, s);
+
+    if (e == hipSuccess)
+      e = eval_t<DP, NCKDE_KERNEL_GAUSS, true> (a, s);
+  }
+  else
+  {
+    e = eval_t<DP, NCKDE_KERNEL_ST, false> (a, s);
+
+    if (e == hipSuccess)
+      e = launch_mixed_merge (a, a.bref, 0, s);
+
+    if (e == hipSuccess)
+      e = eval_t<DP, NCKDE_KERNEL_ST, true> (a, s);
+  }
+
+  if (e == hipSuccess)
+    e = launch_mixed_merge (a, a.bref, 1, s);
+
+  return e;
 }
 
 int
@@ -708,9 +867,9 @@ launch_eval_mixed (const MixedArgs &a, hipStream_t s)
 }
 
 hipError_t
-launch_mixed_merge (const MixedArgs &a, double bref, hipStream_t s)
+launch_mixed_merge (const MixedArgs &a, double bref, int second, hipStream_t s)
 {
-  hipLaunchKernelGGL (k_mixed_merge, dim3 ((a.m + 255) / 256), dim3 (256), 0, s, a, bref);
+  hipLaunchKernelGGL (k_mixed_merge, dim3 ((a.m + 255) / 256), dim3 (256), 0, s, a, bref, second);
 
   return hipGetLastError ();
 }

@@ -105,15 +105,18 @@ struct MixedArgs
   const double *G;        // U_g (d x d, upper): y' = U_g^{-T} (x - mu)
   double ysc;             // power-of-two scale of the whitened coordinates
   void *Yh;               // workspace: Yh, Yl (m x DP fp16 each), then m float2
-  float *partM;           // nsplit x m
-  double *partS;          // nsplit x m
+  float *partM;           // nsplit x m: largest base-2 term
+  double *partS;          // nsplit x m: sum of 2^(lnt2 - M0)
+  int *flag;              // m: 1 where the terms underflowed (second pass)
+  float *M0;              // m: reference of the second pass
+  double bref;            // max_i b_i
   double *out;
   int linear;
 };
 int mixed_points_per_block (int DP);
 hipError_t launch_mixed_points (const MixedArgs &a, hipStream_t s);
 hipError_t launch_eval_mixed (const MixedArgs &a, hipStream_t s);
-hipError_t launch_mixed_merge (const MixedArgs &a, double bref, hipStream_t s);
+hipError_t launch_mixed_merge (const MixedArgs &a, double bref, int second, hipStream_t s);
 hipError_t mixed_build (const double *X, int n, int d, int D, int DP, const double *Wp, const double *Ug,
                         const double *b, double bref, double inv_h, int kernel, double nu, double *mu, double *Yc,
                         unsigned long long *ymax_bits, double *ysc_out, _Float16 *A, float *c, float *coef,
