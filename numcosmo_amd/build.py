@@ -32,7 +32,7 @@ COMMON = [
     "-I", os.path.join(HERE, "..", "include"),
 ]
 
-SOURCES = ["api.cpp", "k_prepare.hip", "k_eval.hip", "k_eval_mixed.hip", "k_linalg.hip", "k_sample.hip"]
+SOURCES = ["api.cpp", "api_apes.cpp", "k_prepare.hip", "k_eval.hip", "k_eval_mixed.hip", "k_linalg.hip", "k_sample.hip"]
 
 
 def _compile(src: str) -> str:
@@ -52,7 +52,9 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
         objs = list(ex.map(_compile, srcs))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    # the offload linker leaves per-target intermediates in its working
+    # directory: keep them in the build folder
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=BUILD_DIR)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
