@@ -9,7 +9,10 @@ Workload (one step = one eval_m2lnp pass over the evaluation points):
 
 Also measured: the same eval at d=16, n=20000, m=200000 (Gaussian target)
 in both precisions, which is where the mixed-precision path is required to
-beat the plain FP64 path by 4x at relative m2lnp error <= 1e-6.
+beat the plain FP64 path by 4x at relative m2lnp error <= 1e-6; and a served
+stream (512 requests, Poisson arrivals, eval/sample/prepare_interp 60/30/10
+over VKDE sessions with n log-uniform on [500, 20000], d in {4, 8, 16, 32},
+m = 10 n) whose p50/p99 request latency goes into "latency".
 
 Timing: W warm-up steps, then K steps bracketed by torch.cuda.synchronize();
 the dominant kernel's duration comes from HIP events recorded on the stream
@@ -70,7 +73,20 @@ def timed_eval(sd, Xd, outd, prec, steps, warmup, stream):
     e1.record(stream)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3 / steps
-    return e0.elapsed_time(e1) / steps, wall
+    step_ms = e0.elapsed_time(e1) / steps
+
+    # the dominant kernel alone: HIP events the engine records around its
+    # main evaluation kernel on the same stream, one step at a time
+    check(L.nckde_set_profiling(sd._h, 1))
+    kms = []
+    f = C.c_float()
+    for _ in range(steps):
+        step()
+        check(L.nckde_get_kernel_ms(sd._h, C.byref(f)))
+        kms.append(f.value)
+    check(L.nckde_set_profiling(sd._h, 0))
+    torch.cuda.synchronize()
+    return step_ms, wall, float(np.mean(kms))
 
 
 def rel_err(got, ref):
@@ -136,16 +152,140 @@ def run_case(name, kind_target, d, n, m, nu, steps, warmup, fit, check_pts, conf
         if not supports(sd, prec, Xd, outd):
             log(f"[{name}] {key}: not available")
             continue
-        ms, wall = timed_eval(sd, Xd, outd, prec, steps, warmup, stream)
+        ms, wall, kms = timed_eval(sd, Xd, outd, prec, steps, warmup, stream)
         got = outd[:check_pts].cpu().numpy()
         emax, e99 = rel_err(got, ref)
         pairs = n * m
         res[key] = {"ms_per_step": ms, "wall_ms_per_step": wall, "pairs_per_s": pairs / (ms * 1e-3),
-                    "rel_err_max": emax, "rel_err_p99": e99,
-                    "tflops": pairs * flop_per_pair(d) / (ms * 1e-3) / 1e12}
-        log(f"[{name}] {key}: {ms:.3f} ms/step, {pairs / (ms * 1e-3):.3e} pairs/s, "
+                    "rel_err_max": emax, "rel_err_p99": e99, "kernel_ms": kms,
+                    "tflops": pairs * flop_per_pair(d) / (ms * 1e-3) / 1e12,
+                    "kernel_tflops": pairs * flop_per_pair(d) / (kms * 1e-3) / 1e12}
+        log(f"[{name}] {key}: {ms:.3f} ms/step (main kernel {kms:.3f} ms), {pairs / (ms * 1e-3):.3e} pairs/s, "
             f"rel err max {emax:.2e} p99 {e99:.2e}")
     return res
+
+
+def serve_case(n_req, rate, n_sessions, slots, seed=None):
+    """Served stream: Poisson arrivals of eval / sample / prepare_interp
+    requests (60/30/10) over VKDE sessions with n log-uniform on [500, 20000]
+    and d in {4, 8, 16, 32}, m = 10 n points or draws per request; p50/p99 of
+    the server-side latency (submit to results in host memory)."""
+    import threading
+
+    import numcosmo_amd as na
+    from numcosmo_amd import synth
+    from numcosmo_amd.serve import Server
+
+    stream = synth.ServedStream(n_req=n_req, rate=rate, n_sessions=n_sessions, seed=seed)
+    log(f"[serve] {n_sessions} sessions, {n_req} requests at {rate}/s, {slots} slots")
+    t0 = time.perf_counter()
+    sds = []
+    for s, S in enumerate(stream.sessions):
+        sd = na.StatsDistVKDE.new(na.StatsDistKernelST.new(S["d"], stream.nu), na.StatsDistCV.NONE)
+        X, m2 = stream.fit_sample(s)
+        sd.add_obs_array(X)
+        sd.prepare_interp(m2)
+        sds.append(sd)
+    setup_s = time.perf_counter() - t0
+    log(f"[serve] sessions fitted in {setup_s:.1f} s")
+    # payloads (generated before the stream starts: a request's inputs exist at its arrival)
+    refits = {}
+    payload = []
+    for r in range(n_req):
+        s = int(stream.session[r])
+        op = stream.OPS[int(stream.op[r])]
+        if op == "prepare_interp":
+            refits[s] = refits.get(s, 0) + 1
+            payload.append(stream.fit_sample(s, refits[s]))
+        else:
+            payload.append(stream.pool(s))
+    # warm-up: one eval and one sample per session (device work buffers)
+    for s, sd in enumerate(sds):
+        Xe, u, z, c2 = stream.pool(s)
+        sd.eval_m2lnp_batch(Xe, precision=na.Precision.MIXED)
+        sd.sample_batch(u, z, c2)
+    log(f"[serve] payloads and warm-up done")
+
+    srv = Server(slots=slots, hbm_budget=64 << 30)
+    tickets = [None] * n_req
+    lateness = np.zeros(n_req)
+
+    def submitter():
+        base = srv.clock()
+        for r in range(n_req):
+            target = base + stream.arrival[r]
+            while True:
+                dt = target - srv.clock()
+                if dt <= 0:
+                    break
+                time.sleep(min(dt, 0.002))
+            s = int(stream.session[r])
+            op = stream.OPS[int(stream.op[r])]
+            if op == "eval":
+                tickets[r] = srv.submit_eval(sds[s], payload[r][0], precision=na.Precision.MIXED)
+            elif op == "sample":
+                _, u, z, c2 = payload[r]
+                tickets[r] = srv.submit_sample(sds[s], u, z, c2)
+            else:
+                X, m2 = payload[r]
+                tickets[r] = srv.submit_prepare_interp(sds[s], m2, X)
+            lateness[r] = srv.clock() - target
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    infos = []
+    for r in range(n_req):
+        while tickets[r] is None:
+            time.sleep(0.0005)
+        _, info = srv.wait(tickets[r])
+        infos.append(info)
+    th.join()
+    srv.close()
+
+    lat = np.array([i["latency"] for i in infos]) * 1e3
+    t_first = min(i["t_submit"] for i in infos)
+    t_last = max(i["t_end"] for i in infos)
+    ops = np.array([stream.OPS[int(o)] for o in stream.op])
+    pairs = sum(10 * stream.sessions[int(stream.session[r])]["n"] ** 2 for r in range(n_req) if ops[r] == "eval")
+    res = {"requests": n_req, "rate_per_s": rate, "sessions": n_sessions, "slots": slots,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+           "requests_per_s": n_req / (t_last - t_first), "eval_pairs_per_s": pairs / (t_last - t_first),
+           "submit_lateness_p99_ms": float(np.percentile(lateness, 99) * 1e3), "setup_s": setup_s}
+    for op in stream.OPS:
+        sel = ops == op
+        if sel.any():
+            res[op] = {"count": int(sel.sum()), "p50_ms": float(np.percentile(lat[sel], 50)),
+                       "p99_ms": float(np.percentile(lat[sel], 99))}
+    log(f"[serve] p50 {res['p50_ms']:.2f} ms p99 {res['p99_ms']:.2f} ms, {res['requests_per_s']:.1f} req/s, "
+        f"per op {[(op, res[op]) for op in stream.OPS if op in res]}")
+    return res
+
+
+def pmc_probe(steps):
+    """Short run for hardware-counter passes: the workload's mixed and FP64
+    evals, then a plain device copy of 256 MiB (the calibration kernel of the
+    FETCH_SIZE / WRITE_SIZE counters)."""
+    import torch
+
+    import numcosmo_amd as na
+    from numcosmo_amd import synth
+
+    tgt, X, m2lnp, Xe = synth.workload("banana", 4000, 100000, 8, 2)
+    sd = na.StatsDistVKDE.new(na.StatsDistKernelST.new(8, 3.0), na.StatsDistCV.NONE)
+    sd.add_obs_array(X)
+    sd.prepare_interp(m2lnp)
+    Xd = torch.tensor(Xe, dtype=torch.float64, device="cuda")
+    outd = torch.empty(Xe.shape[0], dtype=torch.float64, device="cuda")
+    stream = torch.cuda.Stream()
+    for prec in (na.Precision.MIXED, na.Precision.FP64):
+        timed_eval(sd, Xd, outd, prec, steps, 1, stream)
+    src = torch.empty(32 << 20, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("pmc probe done: copy kernel moves 256 MiB read + 256 MiB written per launch")
 
 
 def main(argv=None):
@@ -154,6 +294,12 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-north-star", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc passes")
+    ap.add_argument("--serve-requests", type=int, default=512)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-sessions", type=int, default=48)
+    ap.add_argument("--serve-slots", type=int, default=8)
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -163,9 +309,13 @@ def main(argv=None):
 
     import torch
 
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.pmc_probe:
+        pmc_probe(max(2, min(args.steps, 5)))
+        return 0
+
     from oracle import oracle as O
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
     main_case = run_case("C2", "banana", 8, 4000, 100000, 3.0, args.steps, args.warmup, True, 4000, 2, O, cores)
@@ -174,10 +324,14 @@ def main(argv=None):
         ns = run_case("north-star", "gaussian", 16, 20000, 200000, 3.0, max(3, args.steps // 4), 1, False,
                       200, 4, O, cores)
 
+    latency = None
+    if not args.no_serve:
+        latency = serve_case(args.serve_requests, args.serve_rate, args.serve_sessions, args.serve_slots)
+
     best_key = "mixed" if ("mixed" in main_case and main_case["mixed"]["rel_err_max"] <= 1e-6) else "fp64"
     best = main_case[best_key]
     d = main_case["d"]
-    tflops = best["tflops"]
+    tflops = best["kernel_tflops"]
     peak = PEAK["f16"] if best_key == "mixed" else PEAK["f64"]
     line = {
         "metric": "kernel-pair evals/s, mixed vs FP64, at rel err<=1e-6; p50/p99 request latency",
@@ -203,7 +357,7 @@ def main(argv=None):
         "mixed": main_case.get("mixed"),
         "prepare_interp_s": main_case["prepare_s"],
         "north_star": ns,
-        "latency": None,
+        "latency": latency,
     }
     print(json.dumps(line))
     return 0

@@ -179,6 +179,119 @@ int nckde_get_invUsample (nckde *h, double *Y);
  * (squared distance, index) order */
 int nckde_get_knn (nckde *h, size_t *k, int32_t *idx);
 
+/* ---- measurement --------------------------------------------------------
+ * No counterpart in the reference.  With profiling on, every eval records HIP
+ * events around its main evaluation kernel (k_eval_f64, or the first pass of
+ * the mixed-precision kernel); nckde_get_kernel_ms waits for the last one and
+ * returns its duration. */
+int nckde_set_profiling (nckde *h, int on);
+int nckde_get_kernel_ms (nckde *h, float *ms);
+
+/* ---- APES ensemble sampler ----------------------------------------------
+ *
+ * NcmFitESMCMCWalkerAPES (numcosmo/ncm/fit/ncm_fit_esmcmc_walker_apes.h:
+ * 51-105, .c:509-919) with the ensemble loop of NcmFitESMCMC
+ * (ncm_fit_esmcmc.c:2136-2288).  Each half-ensemble is moved with a VKDE of
+ * the other half: one prepare_interp, one batched sample and one batched eval
+ * of both transition densities per half.  The likelihood is a host callback
+ * over a batch of m points (the reference calls NcmFit per walker).  Draws
+ * come from a seeded host generator in the reference's per-walker order; the
+ * stream differs from GSL's mt19937.
+ */
+typedef struct nckde_apes nckde_apes;
+
+/* NcmFitESMCMCWalkerAPESMethod (ncm_fit_esmcmc_walker_apes.h:51-57) */
+enum { NCKDE_APES_METHOD_KDE = 0, NCKDE_APES_METHOD_VKDE = 1 };
+/* NcmFitESMCMCWalkerAPESKType (ncm_fit_esmcmc_walker_apes.h:68-75) */
+enum { NCKDE_APES_KTYPE_CAUCHY = 0, NCKDE_APES_KTYPE_ST3 = 1, NCKDE_APES_KTYPE_GAUSS = 2 };
+
+/* m2lnL[j] = -2 ln L(theta[j*d .. j*d+d-1]) for j < m; returns 0 on success
+ * (non-finite values reject the proposal, as ncm_fit_esmcmc.c:2183-2200). */
+typedef int (*nckde_m2lnL_batch_fn) (const double *theta, size_t m, unsigned d, double *m2lnL, void *user);
+
+/* ncm_fit_esmcmc_walker_apes_new_full (ncm_fit_esmcmc_walker_apes.h:78) */
+int nckde_apes_create (nckde_apes **out, unsigned nwalkers, unsigned dim, int method, int ktype, uint64_t seed);
+int nckde_apes_destroy (nckde_apes *a);
+/* ncm_fit_esmcmc_walker_apes_set_over_smooth (ncm_fit_esmcmc_walker_apes.h:85) */
+int nckde_apes_set_over_smooth (nckde_apes *a, double over_smooth);
+/* ncm_fit_esmcmc_walker_apes_set_shrink (ncm_fit_esmcmc_walker_apes.h:86) */
+int nckde_apes_set_shrink (nckde_apes *a, double shrink);
+/* ncm_fit_esmcmc_walker_apes_set_local_frac (ncm_fit_esmcmc_walker_apes.h:105) */
+int nckde_apes_set_local_frac (nckde_apes *a, double local_frac);
+/* ncm_fit_esmcmc_walker_apes_set_random_walk_prob (ncm_fit_esmcmc_walker_apes.h:87) */
+int nckde_apes_set_random_walk_prob (nckde_apes *a, double prob);
+/* ncm_fit_esmcmc_walker_apes_use_interp (ncm_fit_esmcmc_walker_apes.h:97) */
+int nckde_apes_set_use_interp (nckde_apes *a, int use_interp);
+/* ncm_fit_esmcmc_walker_apes_set_exploration (ncm_fit_esmcmc_walker_apes.h:110) */
+int nckde_apes_set_exploration (nckde_apes *a, unsigned steps);
+/* eval arithmetic of the transition densities (NCKDE_PREC_*) */
+int nckde_apes_set_precision (nckde_apes *a, int precision);
+/* parameter bounds (ncm_mset_fparam_valid_bounds); default (-inf, inf) */
+int nckde_apes_set_bounds (nckde_apes *a, const double *lb, const double *ub);
+/* initial walkers: theta is nwalkers x dim, m2lnL their -2 ln L */
+int nckde_apes_init (nckde_apes *a, const double *theta, const double *m2lnL);
+/* ncm_fit_esmcmc_run (ncm_fit_esmcmc.h:78): nsteps ensemble steps; chain
+ * (may be NULL) receives nsteps x nwalkers rows of (m2lnL, theta) */
+int nckde_apes_run (nckde_apes *a, unsigned nsteps, nckde_m2lnL_batch_fn like, void *user, double *chain);
+int nckde_apes_get_state (nckde_apes *a, double *theta, double *m2lnL);
+/* acceptance counters and seconds spent in (prepare, sample, eval, likelihood) */
+int nckde_apes_get_stats (nckde_apes *a, uint64_t *accepted, uint64_t *proposed, double *seconds);
+
+/* ---- served requests (continuous batching) -------------------------------
+ *
+ * No counterpart in the reference, which evaluates one object per caller
+ * thread.  A server runs `slots` worker threads; each admitted request runs
+ * on its session's own HIP stream, so requests on different sessions overlap
+ * on the device.  Admission is in arrival order with backfill: a request is
+ * admitted when a slot is free, its session has no request running or
+ * waiting ahead of it (per-session order is arrival order, so results do not
+ * depend on the mix), and the device bytes of the requests running plus its
+ * own stay within hbm_budget (a request larger than the budget runs alone).
+ * A session is an nckde handle; the caller keeps every buffer of a request
+ * alive until nckde_server_wait returns for it.
+ */
+typedef struct nckde_server nckde_server;
+
+enum { NCKDE_REQ_EVAL_M2LNP = 0, NCKDE_REQ_SAMPLE = 1, NCKDE_REQ_PREPARE_INTERP = 2 };
+
+typedef struct
+{
+  int op;                   /* NCKDE_REQ_* */
+  nckde *h;                 /* session */
+  size_t m;                 /* EVAL: points; SAMPLE: draws; PREPARE_INTERP: sample size */
+  const double *X;          /* EVAL: m x d points; PREPARE_INTERP: m x d new sample (NULL: keep the session's) */
+  const double *m2lnp;      /* PREPARE_INTERP: m values of -2 ln p at the sample */
+  const double *u;          /* SAMPLE: as nckde_sample */
+  const double *z;
+  const double *chi2;
+  double *out;              /* EVAL: m values; SAMPLE: m x d; PREPARE_INTERP (may be NULL): the
+                               n_kernels fitted weights, then NaN up to m (the range cut drops points) */
+  uint32_t *idx;            /* SAMPLE: chosen kernels (may be NULL) */
+  int precision;            /* EVAL: NCKDE_PREC_* */
+} nckde_request;
+
+typedef struct
+{
+  int status;               /* NCKDE_OK or the failing call's status */
+  double t_submit;          /* seconds on the server clock */
+  double t_start;           /* admitted */
+  double t_end;             /* results in the caller's buffers */
+  size_t bytes;             /* device bytes charged at admission */
+  char message[256];        /* nckde_last_error of a failing request */
+} nckde_request_info;
+
+int nckde_server_create (nckde_server **out, unsigned slots, size_t hbm_budget);
+/* waits for every submitted request, then stops the workers */
+int nckde_server_destroy (nckde_server *s);
+/* queue a request; returns at once with its ticket */
+int nckde_server_submit (nckde_server *s, const nckde_request *req, uint64_t *ticket);
+/* block until the request is done; each ticket is waited once */
+int nckde_server_wait (nckde_server *s, uint64_t ticket, nckde_request_info *info);
+/* seconds since nckde_server_create (the clock of nckde_request_info) */
+double nckde_server_clock (nckde_server *s);
+/* device bytes a request would be charged at admission */
+int nckde_request_bytes (const nckde_request *req, size_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,7 +59,46 @@ SIGNATURES = {
     "nckde_get_sample": (C.c_int, [_h, _dp]),
     "nckde_get_invUsample": (C.c_int, [_h, _dp]),
     "nckde_get_knn": (C.c_int, [_h, C.POINTER(_sz), C.c_void_p]),
+    "nckde_set_profiling": (C.c_int, [_h, C.c_int]),
+    "nckde_get_kernel_ms": (C.c_int, [_h, C.POINTER(C.c_float)]),
+    "nckde_apes_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint, C.c_uint, C.c_int, C.c_int, C.c_uint64]),
+    "nckde_apes_destroy": (C.c_int, [_h]),
+    "nckde_apes_set_over_smooth": (C.c_int, [_h, C.c_double]),
+    "nckde_apes_set_shrink": (C.c_int, [_h, C.c_double]),
+    "nckde_apes_set_local_frac": (C.c_int, [_h, C.c_double]),
+    "nckde_apes_set_random_walk_prob": (C.c_int, [_h, C.c_double]),
+    "nckde_apes_set_use_interp": (C.c_int, [_h, C.c_int]),
+    "nckde_apes_set_exploration": (C.c_int, [_h, C.c_uint]),
+    "nckde_apes_set_precision": (C.c_int, [_h, C.c_int]),
+    "nckde_apes_set_bounds": (C.c_int, [_h, _dp, _dp]),
+    "nckde_apes_init": (C.c_int, [_h, _dp, _dp]),
+    "nckde_apes_run": (C.c_int, [_h, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "nckde_apes_get_state": (C.c_int, [_h, _dp, _dp]),
+    "nckde_apes_get_stats": (C.c_int, [_h, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _dp]),
+    "nckde_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint, _sz]),
+    "nckde_server_destroy": (C.c_int, [_h]),
+    "nckde_server_submit": (C.c_int, [_h, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "nckde_server_wait": (C.c_int, [_h, C.c_uint64, C.c_void_p]),
+    "nckde_server_clock": (C.c_double, [_h]),
+    "nckde_request_bytes": (C.c_int, [C.c_void_p, C.POINTER(_sz)]),
 }
+
+
+class Request(C.Structure):
+    """nckde_request"""
+    _fields_ = [("op", C.c_int), ("h", C.c_void_p), ("m", C.c_size_t), ("X", C.c_void_p), ("m2lnp", C.c_void_p),
+                ("u", C.c_void_p), ("z", C.c_void_p), ("chi2", C.c_void_p), ("out", C.c_void_p),
+                ("idx", C.c_void_p), ("precision", C.c_int)]
+
+
+class RequestInfo(C.Structure):
+    """nckde_request_info"""
+    _fields_ = [("status", C.c_int), ("t_submit", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double),
+                ("bytes", C.c_size_t), ("message", C.c_char * 256)]
+
+# nckde_m2lnL_batch_fn
+M2LNL_BATCH_FN = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_double), C.c_size_t, C.c_uint, C.POINTER(C.c_double),
+                             C.c_void_p)
 
 
 class NckdeError(RuntimeError):

@@ -101,6 +101,9 @@ struct nckde
 
   hipStream_t stream      = nullptr;
   hipStream_t last_stream = nullptr;  // stream of the last device-form eval
+  bool profiling          = false;    // events around the main eval kernel
+  bool pev_recorded       = false;
+  hipEvent_t pev[2]       = {nullptr, nullptr};
   std::mutex mtx;
 
   // prepared device state
@@ -112,7 +115,7 @@ struct nckde
   double ysc = 1.0, bref = 0.0;
   // work buffers
   DBuf dXe, dout, dpartM, dpartS, du, dz, dchi2, didx;
-  DBuf dAT, dM, dS, dbv, dbs, dx, dxt, dr, drt, dg, dP, dpart, dscal, dm2lnp, dlu;
+  DBuf dAT, dM, dS, dbv, dbs, dx, dxt, dr, drt, dg, dP, dpart, dscal, dm2lnp, dlu, dtv, drw;
 };
 
 #define HIPCHK(call)                                                                 \
@@ -508,7 +511,7 @@ nnls_unconstrained (nckde *h, int n, const std::vector<int> &P, double *dxo)
 
   if (info == 0)
   {
-    HIPCHK (nck::launch_trsv_pair (h->dS.as<double> (), np, h->dbs.as<double> (), s));
+    HIPCHK (nck::launch_trsv_pair (h->dS.as<double> (), np, h->dbs.as<double> (), h->dtv.as<double> (), s));
   }
   else
   {
@@ -582,7 +585,8 @@ nnls_residual (nckde *h, const double *dxv, double *drv, double &rnorm)
   double r2    = 0.0;
 
   HIPCHK (nck::launch_residual (h->dAT.as<double> (), (int) h->n_k, (int) h->n_obs, dxv, nullptr, drv,
-                                h->dpart.as<double> (), h->dpart.as<double> () + nb, h->stream));
+                                h->dpart.as<double> (), h->dpart.as<double> () + nb, h->drw.as<double> (),
+                                h->stream));
   HIPCHK (hipMemcpyAsync (&r2, h->dpart.as<double> () + nb, sizeof (double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK (hipStreamSynchronize (h->stream));
   rnorm = std::sqrt (r2);
@@ -613,6 +617,8 @@ nnls_solve (nckde *h)
   HIPCHK (h->dS.ensure (sizeof (double) * n * n));
   HIPCHK (h->dbv.ensure (sizeof (double) * n));
   HIPCHK (h->dbs.ensure (sizeof (double) * n));
+  HIPCHK (h->dtv.ensure (sizeof (double) * n));
+  HIPCHK (h->drw.ensure (sizeof (double) * nck::residual_work_doubles (n, nobs)));
   HIPCHK (h->dx.ensure (sizeof (double) * n));
   HIPCHK (h->dxt.ensure (sizeof (double) * n));
   HIPCHK (h->dr.ensure (sizeof (double) * nobs));
@@ -920,6 +926,13 @@ eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipS
   a.out    = dout;
   a.linear = linear;
 
+  if (h->profiling)
+  {
+    a.ev0           = h->pev[0];
+    a.ev1           = h->pev[1];
+    h->pev_recorded = true;
+  }
+
   HIPCHK (nck::launch_mixed_points (a, s));
   HIPCHK (nck::launch_eval_mixed (a, s));
 
@@ -981,7 +994,17 @@ eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, 
   a.out           = dout;
   a.linear        = linear;
 
+  if (h->profiling)
+  {
+    HIPCHK (hipEventRecord (h->pev[0], s));
+    h->pev_recorded = true;
+  }
+
   HIPCHK (nck::launch_eval_f64 (a, s));
+
+  if (h->profiling)
+    HIPCHK (hipEventRecord (h->pev[1], s));
+
   HIPCHK (nck::launch_eval_merge (a, s));
 
   return NCKDE_OK;
@@ -1109,6 +1132,10 @@ nckde_destroy (nckde *h)
     (void) hipStreamSynchronize (h->stream);
     (void) hipStreamDestroy (h->stream);
   }
+
+  for (hipEvent_t &e : h->pev)
+    if (e)
+      (void) hipEventDestroy (e);
 
   delete h;
 
@@ -1602,4 +1629,123 @@ nckde_get_knn (nckde *h, size_t *k, int32_t *idx)
   return NCKDE_OK;
 }
 
+int
+nckde_set_profiling (nckde *h, int on)
+{
+  TRY (check_handle (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  if (on && h->pev[0] == nullptr)
+  {
+    HIPCHK (hipEventCreate (&h->pev[0]));
+    HIPCHK (hipEventCreate (&h->pev[1]));
+  }
+
+  h->profiling    = (on != 0);
+  h->pev_recorded = false;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_kernel_ms (nckde *h, float *ms)
+{
+  TRY (check_handle (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  if (ms == nullptr || !h->pev_recorded)
+  {
+    set_error ("nckde_get_kernel_ms: no profiled eval on this handle");
+    return NCKDE_ESTATE;
+  }
+
+  HIPCHK (hipEventSynchronize (h->pev[1]));
+  HIPCHK (hipEventElapsedTime (ms, h->pev[0], h->pev[1]));
+
+  return NCKDE_OK;
+}
+
+int
+nckde_request_bytes (const nckde_request *req, size_t *bytes)
+{
+  if (req == nullptr || bytes == nullptr)
+  {
+    set_error ("nckde_request_bytes: NULL argument");
+    return NCKDE_EINVAL;
+  }
+
+  TRY (check_handle (req->h));
+
+  const nckde *h  = req->h;
+  const size_t d  = (size_t) h->d;
+  const size_t m  = req->m;
+  const size_t D  = (size_t) nck::padded_dim (h->d);
+  const size_t DP = (size_t) nck::mfma_dim (h->d);
+  const size_t nk = std::max<size_t> (h->n_k, 1);
+
+  switch (req->op)
+  {
+    case NCKDE_REQ_EVAL_M2LNP:
+    {
+      // points, output and the split partial sums (at most 2048 blocks of
+      // partials per point block, see eval_mixed / eval_device)
+      const size_t parts = std::min<size_t> ((nk + 63) / 64, 64);
+      size_t b           = m * d * 8 + m * 8 + m * parts * 16;
+
+      if (req->precision == NCKDE_PREC_MIXED)
+        b += m * (4 * DP + 16);
+
+      *bytes = b;
+      break;
+    }
+    case NCKDE_REQ_SAMPLE:
+      *bytes = m * (8 + 8 * d + 8 + 8 * d + 4);
+      break;
+    case NCKDE_REQ_PREPARE_INTERP:
+    {
+      const size_t n  = m;
+      const size_t kk = std::max<size_t> ((size_t) (h->local_frac * n), 2);
+      const size_t nq = std::min<size_t> (n, std::max<size_t> (1, ((size_t) 512 << 20) / (8 * std::max<size_t> (n, 1))));
+      size_t b        = n * d * 8 * 2 + n * d * d * 8 * 2 + n * (D * (D + 1) / 2 + D + 4) * 8;
+
+      if (h->kind == NCKDE_VKDE)
+        b += n * kk * 4 + n * nq * 8;
+
+      b += 3 * n * n * 8 + 12 * n * 8;  // IM^T, M = A^T A, the active-set matrix, vectors
+      *bytes = b;
+      break;
+    }
+    default:
+      set_error ("nckde_request_bytes: unknown op %d", req->op);
+      return NCKDE_EINVAL;
+  }
+
+  return NCKDE_OK;
+}
+
 } // extern "C"
+
+namespace nck
+{
+
+// Frees the transient buffers of prepare / prepare_interp (kNN keys, IM^T,
+// normal matrices); the prepared factors and weights stay.
+int
+trim_work (nckde *h)
+{
+  TRY (check_handle (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  HIPCHK (hipStreamSynchronize (h->stream));
+  h->dkeys.release ();
+  h->dAT.release ();
+  h->dM.release ();
+  h->dS.release ();
+  h->dlu.release ();
+  h->dscratch.release ();
+  h->drw.release ();
+
+  return NCKDE_OK;
+}
+
+} // namespace nck

@@ -819,9 +819,15 @@ eval_d (const MixedArgs &a, hipStream_t s)
 {
   hipError_t e;
 
+  if (a.ev0)
+    (void) hipEventRecord (a.ev0, s);
+
   if (a.kernel == NCKDE_KERNEL_GAUSS)
   {
     e = eval_t<DP, NCKDE_KERNEL_GAUSS, false> (a, s);
+
+    if (a.ev1)
+      (void) hipEventRecord (a.ev1, s);
 
     if (e == hipSuccess)
       e = launch_mixed_merge (a, a.bref, 0, s);
@@ -832,6 +838,9 @@ eval_d (const MixedArgs &a, hipStream_t s)
   else
   {
     e = eval_t<DP, NCKDE_KERNEL_ST, false> (a, s);
+
+    if (a.ev1)
+      (void) hipEventRecord (a.ev1, s);
 
     if (e == hipSuccess)
       e = launch_mixed_merge (a, a.bref, 0, s);

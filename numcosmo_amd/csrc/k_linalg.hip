@@ -188,21 +188,92 @@ launch_transpose (const double *in, int rows, int cols, double *out, hipStream_t
 }
 
 // --------------------------------------------------------------------------
-// Symmetric rank-K products on 64 x 64 tiles of the upper triangle, 256
-// threads per tile, 4 x 4 outputs per thread, K staged through LDS in
-// slices of 16.  mode 0: M = AT AT^T (AT is n x K, row-major; the Gram
-// matrix A^T A of the NNLS normal equations, ncm_matrix.c:1547-1575).
-// mode 1: trailing update of the blocked Cholesky, S[a][b] -= sum_r
-// S[kb+r][a] S[kb+r][b] over the rows kb..kb+K-1, for off <= a <= b.
+// Symmetric rank-K products on the FP64 matrix cores (v_mfma_f64_16x16x4f64)
+// over 128 x 128 tiles of the upper triangle.  256 threads = 4 waves in a
+// 2 x 2 layout, each wave a 64 x 64 block of 4 x 4 MFMA tiles; K staged
+// through LDS in slices of 16, the next slice's global loads in flight
+// during the current slice's MFMAs.
+//   mode 0: M = AT AT^T (AT is n x K row-major): the Gram matrix A^T A of
+//           the NNLS normal equations (cblas_dsyrk, ncm_matrix.c:1547-1575);
+//           both triangles are written.
+//   mode 1: trailing update of the blocked Cholesky, S[a][b] -= sum_r
+//           S[kb+r][a] S[kb+r][b] over the panel rows kb..kb+K-1, for
+//           off <= a <= b (upper triangle only).
+// Tiles are numbered along the rows of the upper triangle; the grid is
+// padded to a multiple of 8 and block b takes tile (b % 8) * (grid / 8) +
+// b / 8, so that each XCD (blocks are dealt round-robin over the 8 XCDs)
+// works on a contiguous band of tiles that shares operand rows in its L2.
 // --------------------------------------------------------------------------
+typedef double f64x4 __attribute__ ((ext_vector_type (4)));
+
+constexpr int SY_BM = 128;
+constexpr int SY_BK = 16;
+
+template <int MODE>
+__device__ inline void
+syrk_load (const double *__restrict__ P, int n, int K, int ld, int kb, int r0, int k0, double v[8])
+{
+  const int tid = threadIdx.x;
+
+  if (MODE == 0)
+  {
+    const int a  = r0 + tid / 2;
+    const int kq = k0 + (tid % 2) * 8;
+
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      v[q] = (a < n && kq + q < K) ? P[(size_t) a * ld + kq + q] : 0.0;
+  }
+  else
+  {
+    const int kq = k0 + tid / 16;
+    const int a  = r0 + (tid % 16) * 8;
+
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      v[q] = (a + q < n && kq < K) ? P[(size_t) (kb + kq) * ld + a + q] : 0.0;
+  }
+}
+
+template <int MODE>
+__device__ inline void
+syrk_store (double (*T)[SY_BM + 4], const double v[8])
+{
+  const int tid = threadIdx.x;
+
+  if (MODE == 0)
+  {
+    const int rr = tid / 2, kk = (tid % 2) * 8;
+
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      T[kk + q][rr] = v[q];
+  }
+  else
+  {
+    const int kk = tid / 16, rr = (tid % 16) * 8;
+
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      T[kk][rr + q] = v[q];
+  }
+}
+
 template <int MODE>
 __global__ void __launch_bounds__ (256)
-k_syrk_tiles (const double *__restrict__ P, int n, int K, int ld, int off, int kb, double *__restrict__ M)
+k_syrk_mfma (const double *__restrict__ P, int n, int K, int ld, int off, int kb, double *__restrict__ M,
+             int ntiles)
 {
-  __shared__ double As[16][65];
-  __shared__ double Bs[16][65];
-  const int nt = (n - off + 63) / 64;
-  int t        = blockIdx.x;
+  __shared__ double As[SY_BK][SY_BM + 4];
+  __shared__ double Bs[SY_BK][SY_BM + 4];
+
+  const int per = gridDim.x / 8;
+  int t         = (blockIdx.x % 8) * per + blockIdx.x / 8;
+
+  if (t >= ntiles)
+    return;
+
+  const int nt = (n - off + SY_BM - 1) / SY_BM;
   int ti       = 0;
 
   while (t >= nt - ti)
@@ -211,99 +282,101 @@ k_syrk_tiles (const double *__restrict__ P, int n, int K, int ld, int off, int k
     ++ti;
   }
 
-  const int tj  = ti + t;
-  const int r0  = off + ti * 64, c0 = off + tj * 64;
-  const int tx  = threadIdx.x % 16, ty = threadIdx.x / 16;
-  double acc[4][4];
+  const int tj   = ti + t;
+  const int r0   = off + ti * SY_BM, c0 = off + tj * SY_BM;
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const int wr   = (w >> 1) * 64, wc = (w & 1) * 64;
+  const int li   = lane % 16, lk = lane / 16;
+
+  f64x4 acc[4][4];
 
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      acc[i][j] = 0.0;
+      acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-  for (int k0 = 0; k0 < K; k0 += 16)
+  double va[8], vb[8];
+
+  syrk_load<MODE> (P, n, K, ld, kb, r0, 0, va);
+  syrk_load<MODE> (P, n, K, ld, kb, c0, 0, vb);
+
+  for (int k0 = 0; k0 < K; k0 += SY_BK)
   {
-    for (int e = threadIdx.x; e < 1024; e += 256)
-    {
-      int rr, kk;
-
-      if (MODE == 0)
-      {
-        rr = e / 16;
-        kk = e % 16;
-        const int ra = r0 + rr, rb = c0 + rr, kc = k0 + kk;
-
-        As[kk][rr] = (ra < n && kc < K) ? P[(size_t) ra * ld + kc] : 0.0;
-        Bs[kk][rr] = (rb < n && kc < K) ? P[(size_t) rb * ld + kc] : 0.0;
-      }
-      else
-      {
-        kk = e / 64;
-        rr = e % 64;
-        const int ca = r0 + rr, cb = c0 + rr, kr = k0 + kk;
-
-        As[kk][rr] = (ca < n && kr < K) ? P[(size_t) (kb + kr) * ld + ca] : 0.0;
-        Bs[kk][rr] = (cb < n && kr < K) ? P[(size_t) (kb + kr) * ld + cb] : 0.0;
-      }
-    }
-
+    __syncthreads ();
+    syrk_store<MODE> (As, va);
+    syrk_store<MODE> (Bs, vb);
     __syncthreads ();
 
-#pragma unroll
-    for (int kk = 0; kk < 16; ++kk)
+    if (k0 + SY_BK < K)
     {
-      double av[4], bv[4];
+      syrk_load<MODE> (P, n, K, ld, kb, r0, k0 + SY_BK, va);
+      syrk_load<MODE> (P, n, K, ld, kb, c0, k0 + SY_BK, vb);
+    }
+
+#pragma unroll
+    for (int ks = 0; ks < SY_BK / 4; ++ks)
+    {
+      const int kq = ks * 4 + lk;
+      double fa[4], fb[4];
 
 #pragma unroll
       for (int i = 0; i < 4; ++i)
       {
-        av[i] = As[kk][ty * 4 + i];
-        bv[i] = Bs[kk][tx * 4 + i];
+        fa[i] = As[kq][wr + i * 16 + li];
+        fb[i] = Bs[kq][wc + i * 16 + li];
       }
 
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          acc[i][j] = fma (av[i], bv[j], acc[i][j]);
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64 (fa[i], fb[j], acc[i][j], 0, 0, 0);
     }
-
-    __syncthreads ();
   }
 
+  // D[i][j] of a 16 x 16 tile: register r of a lane holds i = lane / 16 + 4 r, j = lane % 16
 #pragma unroll
   for (int i = 0; i < 4; ++i)
-  {
-    const int a = r0 + ty * 4 + i;
-
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-    {
-      const int b = c0 + tx * 4 + j;
-
-      if (a < n && b < n && a <= b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
       {
-        if (MODE == 0)
+        const int a = r0 + wr + i * 16 + lk + 4 * r;
+        const int b = c0 + wc + j * 16 + li;
+
+        if (a < n && b < n && a <= b)
         {
-          M[(size_t) a * n + b] = acc[i][j];
-          M[(size_t) b * n + a] = acc[i][j];
-        }
-        else
-        {
-          M[(size_t) a * ld + b] -= acc[i][j];
+          if (MODE == 0)
+          {
+            M[(size_t) a * n + b] = acc[i][j][r];
+            M[(size_t) b * n + a] = acc[i][j][r];
+          }
+          else
+          {
+            M[(size_t) a * ld + b] -= acc[i][j][r];
+          }
         }
       }
-    }
-  }
+}
+
+template <int MODE>
+static void
+launch_syrk_mfma (const double *P, int n, int K, int ld, int off, int kb, double *M, hipStream_t s)
+{
+  const int nt     = (n - off + SY_BM - 1) / SY_BM;
+  const int ntiles = nt * (nt + 1) / 2;
+  const int grid   = (ntiles + 7) / 8 * 8;
+
+  if (ntiles > 0)
+    hipLaunchKernelGGL (k_syrk_mfma<MODE>, dim3 (grid), dim3 (256), 0, s, P, n, K, ld, off, kb, M, ntiles);
 }
 
 hipError_t
 launch_gram_nt (const double *AT, int n, int K, double *M, hipStream_t s)
 {
-  const int nt = (n + 63) / 64;
-
-  hipLaunchKernelGGL (k_syrk_tiles<0>, dim3 (nt * (nt + 1) / 2), dim3 (256), 0, s, AT, n, K, K, 0, 0, M);
+  launch_syrk_mfma<0> (AT, n, K, K, 0, 0, M, s);
 
   return hipGetLastError ();
 }
@@ -339,28 +412,49 @@ launch_row_dot (const double *AT, int n, int K, const double *v, double *out, hi
   return hipGetLastError ();
 }
 
-// r = f - A x (f == NULL: ones), with per-block partial sums of r^2; x is
-// wave-uniform and zero entries (inactive columns) are skipped.
+// r = f - A x (f == NULL: ones) and |r|^2.  Pass 1: block (jb, c) sums the
+// kernels of chunk c (RES_CH kernels, inactive ones skipped: x is uniform
+// over the block) for 256 observations; pass 2 adds the chunks in a fixed
+// order and forms per-block partial sums of r^2; a last pass adds those.
+constexpr int RES_CH = 512;
+
 __global__ void __launch_bounds__ (256)
-k_residual (const double *__restrict__ AT, int nk, int nobs, const double *__restrict__ x,
-            const double *__restrict__ f, double *__restrict__ r, double *__restrict__ partial)
+k_residual_part (const double *__restrict__ AT, int nk, int nobs, const double *__restrict__ x,
+                 double *__restrict__ part)
+{
+  const int j  = blockIdx.x * 256 + threadIdx.x;
+  const int a0 = blockIdx.y * RES_CH;
+  const int a1 = (a0 + RES_CH < nk) ? a0 + RES_CH : nk;
+  double acc   = 0.0;
+
+  if (j < nobs)
+    for (int a = a0; a < a1; ++a)
+    {
+      const double xa = x[a];
+
+      if (xa != 0.0)
+        acc = fma (AT[(size_t) a * nobs + j], xa, acc);
+    }
+
+  if (j < nobs)
+    part[(size_t) blockIdx.y * nobs + j] = acc;
+}
+
+__global__ void __launch_bounds__ (256)
+k_residual (const double *__restrict__ part, int nch, int nobs, const double *__restrict__ f,
+            double *__restrict__ r, double *__restrict__ partial)
 {
   __shared__ double red[256];
   const int j = blockIdx.x * 256 + threadIdx.x;
-  double acc  = 0.0;
-
-  for (int a = 0; a < nk; ++a)
-  {
-    const double xa = x[a];
-
-    if (xa != 0.0 && j < nobs)
-      acc = fma (AT[(size_t) a * nobs + j], xa, acc);
-  }
-
-  double rj = 0.0;
+  double rj   = 0.0;
 
   if (j < nobs)
   {
+    double acc = 0.0;
+
+    for (int c = 0; c < nch; ++c)
+      acc += part[(size_t) c * nobs + j];
+
     rj   = (f ? f[j] : 1.0) - acc;
     r[j] = rj;
   }
@@ -394,13 +488,21 @@ k_sum_partials (const double *__restrict__ partial, int np, double *__restrict__
   }
 }
 
+size_t
+residual_work_doubles (int nk, int nobs)
+{
+  return (size_t) ((nk + RES_CH - 1) / RES_CH) * nobs;
+}
+
 hipError_t
 launch_residual (const double *AT, int nk, int nobs, const double *x, const double *f, double *r, double *partial,
-                 double *rnorm2, hipStream_t s)
+                 double *rnorm2, double *work, hipStream_t s)
 {
-  const int nb = (nobs + 255) / 256;
+  const int nb  = (nobs + 255) / 256;
+  const int nch = (nk + RES_CH - 1) / RES_CH;
 
-  hipLaunchKernelGGL (k_residual, dim3 (nb), dim3 (256), 0, s, AT, nk, nobs, x, f, r, partial);
+  hipLaunchKernelGGL (k_residual_part, dim3 (nb, nch), dim3 (256), 0, s, AT, nk, nobs, x, work);
+  hipLaunchKernelGGL (k_residual, dim3 (nb), dim3 (256), 0, s, work, nch, nobs, f, r, partial);
   hipLaunchKernelGGL (k_sum_partials, dim3 (1), dim3 (64), 0, s, partial, nb, rnorm2);
 
   return hipGetLastError ();
@@ -515,17 +617,18 @@ k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ 
   }
 }
 
-// row panel: for each column c >= kb+nb solve U_kk^T x = S[kb:kb+nb][c]
-__global__ void __launch_bounds__ (256)
+// row panel: for each column c >= kb+nb solve U_kk^T x = S[kb:kb+nb][c],
+// one lane per column (the column's earlier rows are re-read from cache)
+__global__ void __launch_bounds__ (64)
 k_trsm_row (double *__restrict__ S, int np, int kb, int nb, const int *__restrict__ info)
 {
   __shared__ double U[CH_NB][CH_NB + 1];
-  const int c = kb + nb + blockIdx.x * 256 + threadIdx.x;
+  const int c = kb + nb + blockIdx.x * 64 + threadIdx.x;
 
   if (*info != 0)
     return;
 
-  for (int e = threadIdx.x; e < nb * nb; e += 256)
+  for (int e = threadIdx.x; e < nb * nb; e += 64)
   {
     const int r = e / nb, q = e % nb;
 
@@ -571,11 +674,9 @@ launch_potrf (double *S, int np, int *info, hipStream_t s)
     if (kb + nb < np)
     {
       const int rest = np - kb - nb;
-      const int nt   = (rest + 63) / 64;
 
-      hipLaunchKernelGGL (k_trsm_row, dim3 ((rest + 255) / 256), dim3 (256), 0, s, S, np, kb, nb, info);
-      hipLaunchKernelGGL (k_syrk_tiles<1>, dim3 (nt * (nt + 1) / 2), dim3 (256), 0, s, S, np, nb, np,
-                          kb + nb, kb, S);
+      hipLaunchKernelGGL (k_trsm_row, dim3 ((rest + 63) / 64), dim3 (64), 0, s, S, np, kb, nb, info);
+      launch_syrk_mfma<1> (S, np, nb, np, kb + nb, kb, S, s);
     }
   }
 
@@ -584,56 +685,98 @@ launch_potrf (double *S, int np, int *info, hipStream_t s)
   return hipGetLastError ();
 }
 
-// U^T y = b then U x = y, in place in b; one 1024-thread block.  Each step
-// divides by the pivot, then updates the remaining right-hand side with a
-// unit-stride row of U (forward) or of its mirrored transpose (backward).
-__global__ void __launch_bounds__ (1024)
-k_trsv_pair (const double *__restrict__ S, int np, double *__restrict__ b)
+// U^T y = b then U x = y with the factor of launch_potrf (upper triangle =
+// U, lower triangle = U^T), in 64-row panels: one launch per panel and
+// direction.  With T[i][j] = S[(p0+j) np + p0+i], the panel's diagonal
+// block is L = U^T below its diagonal (forward) and U above it (backward),
+// and the coupling of a row i outside the panel to panel column c is
+// S[(p0+c) np + i] in both directions, a unit-stride read over i.  Every
+// block stages T in LDS and solves the panel with one wave (64 serial
+// steps, operands shuffled from the pivot lane); block 0 writes the panel's
+// solution to `out`, and every block updates its 256 rows of the right-hand
+// side outside the panel.  The panel rows of `rhs` are only read in a
+// launch, so the blocks never race.
+constexpr int TS_NB = 64;
+
+template <int FWD>
+__global__ void __launch_bounds__ (256)
+k_trsv_panel (const double *__restrict__ S, int np, int p0, int pb, double *__restrict__ rhs,
+              double *__restrict__ out)
 {
-  __shared__ double yj;
+  __shared__ double T[TS_NB][TS_NB + 1];
+  __shared__ double y[TS_NB];
   const int tid = threadIdx.x;
 
-  for (int j = 0; j < np; ++j)
+  for (int e = tid; e < TS_NB * TS_NB; e += 256)
   {
-    if (tid == 0)
-    {
-      yj   = b[j] / S[(size_t) j * np + j];
-      b[j] = yj;
-    }
+    const int j = e / TS_NB, i = e % TS_NB;
 
-    __syncthreads ();
-
-    const double y = yj;
-
-    for (int i = j + 1 + tid; i < np; i += 1024)
-      b[i] = fma (-S[(size_t) j * np + i], y, b[i]);
-
-    __syncthreads ();
+    T[i][j] = (i < pb && j < pb) ? S[(size_t) (p0 + j) * np + p0 + i] : 0.0;
   }
 
-  for (int j = np - 1; j >= 0; --j)
+  __syncthreads ();
+
+  if (tid < 64)
   {
-    if (tid == 0)
+    const int i = tid;
+    double bi   = (i < pb) ? rhs[p0 + i] : 0.0;
+
+    for (int step = 0; step < pb; ++step)
     {
-      yj   = b[j] / S[(size_t) j * np + j];
-      b[j] = yj;
+      const int j     = FWD ? step : pb - 1 - step;
+      const double bj = __shfl (bi, j, 64);
+      const double yj = bj / T[j][j];
+
+      if (i == j)
+        bi = yj;
+      else if (FWD ? (i > j) : (i < j))
+        bi = fma (-T[i][j], yj, bi);
     }
 
-    __syncthreads ();
+    y[i] = bi;
 
-    const double y = yj;
+    if (blockIdx.x == 0 && i < pb)
+      out[p0 + i] = bi;
+  }
 
-    for (int i = tid; i < j; i += 1024)
-      b[i] = fma (-S[(size_t) j * np + i], y, b[i]);
+  __syncthreads ();
 
-    __syncthreads ();
+  const int row = (FWD ? p0 + pb : 0) + blockIdx.x * 256 + tid;
+  const int hi  = FWD ? np : p0;
+
+  if (row < hi)
+  {
+    double t = rhs[row];
+
+    for (int c = 0; c < pb; ++c)
+      t = fma (-S[(size_t) (p0 + c) * np + row], y[c], t);
+
+    rhs[row] = t;
   }
 }
 
+// solution of (U^T U) x = b returned in b; tmp holds np doubles
 hipError_t
-launch_trsv_pair (const double *S, int np, double *b, hipStream_t s)
+launch_trsv_pair (const double *S, int np, double *b, double *tmp, hipStream_t s)
 {
-  hipLaunchKernelGGL (k_trsv_pair, dim3 (1), dim3 (1024), 0, s, S, np, b);
+  for (int p0 = 0; p0 < np; p0 += TS_NB)
+  {
+    const int pb   = (np - p0 < TS_NB) ? np - p0 : TS_NB;
+    const int rest = np - p0 - pb;
+    const int grid = (rest + 255) / 256 > 0 ? (rest + 255) / 256 : 1;
+
+    hipLaunchKernelGGL (k_trsv_panel<1>, dim3 (grid), dim3 (256), 0, s, S, np, p0, pb, b, tmp);
+  }
+
+  const int last = ((np - 1) / TS_NB) * TS_NB;
+
+  for (int p0 = last; p0 >= 0; p0 -= TS_NB)
+  {
+    const int pb   = (np - p0 < TS_NB) ? np - p0 : TS_NB;
+    const int grid = (p0 + 255) / 256 > 0 ? (p0 + 255) / 256 : 1;
+
+    hipLaunchKernelGGL (k_trsv_panel<0>, dim3 (grid), dim3 (256), 0, s, S, np, p0, pb, tmp, b);
+  }
 
   return hipGetLastError ();
 }

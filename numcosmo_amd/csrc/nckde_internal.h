@@ -49,6 +49,9 @@ struct DBuf
   template <class T> T *as () const { return static_cast<T *> (p); }
 };
 
+// Frees a handle's transient prepare buffers (defined in api.cpp).
+int trim_work (nckde *h);
+
 // Padded dimension used by the templated per-pair kernels.
 int padded_dim (int d);
 // Padded dimension of the MFMA tile (multiples of 4 up to 64, see k_eval_mixed.hip).
@@ -112,6 +115,7 @@ struct MixedArgs
   double bref;            // max_i b_i
   double *out;
   int linear;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // when set: recorded around the main evaluation kernel
 };
 int mixed_points_per_block (int DP);
 hipError_t launch_mixed_points (const MixedArgs &a, hipStream_t s);
@@ -137,12 +141,13 @@ hipError_t launch_build_IM (const IMArgs &a, hipStream_t s);
 hipError_t launch_transpose (const double *in, int rows, int cols, double *out, hipStream_t s);
 hipError_t launch_gram_nt (const double *AT, int n, int K, double *M, hipStream_t s);
 hipError_t launch_row_dot (const double *AT, int n, int K, const double *v, double *out, hipStream_t s);
+size_t residual_work_doubles (int nk, int nobs);
 hipError_t launch_residual (const double *AT, int nk, int nobs, const double *x, const double *f,
-                            double *r, double *partial, double *rnorm2, hipStream_t s);
+                            double *r, double *partial, double *rnorm2, double *work, hipStream_t s);
 hipError_t launch_gather_sym (const double *M, int n, const int *P, int np, double *S,
                               const double *b, double *bs, hipStream_t s);
 hipError_t launch_potrf (double *S, int np, int *info, hipStream_t s);
-hipError_t launch_trsv_pair (const double *S, int np, double *b, hipStream_t s);
+hipError_t launch_trsv_pair (const double *S, int np, double *b, double *tmp, hipStream_t s);
 hipError_t launch_lu_solve (double *S, int np, double *b, int *info, hipStream_t s);
 hipError_t launch_scatter (const double *sol, const int *P, int np, int n, double *x, hipStream_t s);
 hipError_t launch_shrink (double *w, int n, double shrink, int *status, hipStream_t s);

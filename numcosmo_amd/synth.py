@@ -126,3 +126,62 @@ def workload(name: str, n: int, m: int, d: int, config_id: int, seed: int | None
     X = tgt.draw(g, n)
     Xe = tgt.draw(g, m)
     return tgt, X, tgt.m2lnp(X), Xe
+
+
+class ServedStream:
+    """Served request stream: Poisson arrivals over prepared sessions.
+
+    Sessions have n log-uniform on [n_lo, n_hi] and d uniform on ``dims``;
+    each targets a seeded correlated Gaussian (covariance as ``gaussian``)
+    and is fitted with VKDE + prepare_interp on n draws.  Requests pick a
+    session uniformly (so a request's n is log-uniform and its d uniform),
+    with ops eval_m2lnp / sample / prepare_interp in proportions ``mix``;
+    eval and sample requests carry m = m_factor * n points or draws, and a
+    prepare_interp request refits its session on a fresh sample of n.
+    Per-session point and draw pools are generated once and shared by the
+    session's requests.
+    """
+
+    OPS = ("eval", "sample", "prepare_interp")
+
+    def __init__(self, n_req=512, rate=20.0, n_sessions=48, n_lo=500, n_hi=20000, dims=(4, 8, 16, 32),
+                 mix=(0.6, 0.3, 0.1), m_factor=10, seed=None, nu=3.0):
+        g = rng(4, seed)
+        self.nu = nu
+        self.m_factor = m_factor
+        self.sessions = []
+        for s in range(n_sessions):
+            n = int(round(np.exp(g.uniform(np.log(n_lo), np.log(n_hi)))))
+            d = int(g.choice(dims))
+            self.sessions.append({"n": n, "d": d, "seed": int(g.integers(1 << 62))})
+        self.arrival = np.cumsum(g.exponential(1.0 / rate, size=n_req)) if rate > 0 else np.zeros(n_req)
+        self.op = g.choice(3, size=n_req, p=np.asarray(mix) / np.sum(mix))
+        self.session = g.integers(0, n_sessions, size=n_req)
+        self._pools = {}
+
+    def target(self, s):
+        S = self.sessions[s]
+        return Gaussian(S["d"], rng(seed=S["seed"]))
+
+    def fit_sample(self, s, refit=0):
+        """(X, m2lnp) of the session's fit number ``refit`` (0: initial)."""
+        S = self.sessions[s]
+        g = rng(seed=S["seed"] + 1 + refit)
+        tgt = self.target(s)
+        X = tgt.draw(g, S["n"])
+        return X, tgt.m2lnp(X)
+
+    def pool(self, s):
+        """Evaluation points and (u, z, chi2) draws of session s."""
+        if s not in self._pools:
+            S = self.sessions[s]
+            n, d = S["n"], S["d"]
+            m = self.m_factor * n
+            g = rng(seed=S["seed"] + 1_000_003)
+            tgt = self.target(s)
+            Xe = tgt.draw(g, m)
+            u = g.random(m)
+            z = g.standard_normal((m, d))
+            c2 = g.chisquare(self.nu, size=m)
+            self._pools[s] = (Xe, u, z, c2)
+        return self._pools[s]

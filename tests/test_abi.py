@@ -60,3 +60,26 @@ def test_create_without_device_fails_cleanly(libso):
     from numcosmo_amd import StatsDistKernelGauss, StatsDistVKDE, NckdeError
     with pytest.raises(NckdeError):
         StatsDistVKDE.new(StatsDistKernelGauss.new(2))
+
+
+def test_request_struct_layout_matches_the_header(tmp_path):
+    """The ctypes mirrors of nckde_request / nckde_request_info have the C
+    layout (sizes and field offsets as compiled from the header)."""
+    import ctypes as C
+    from numcosmo_amd._lib import Request, RequestInfo
+    src = tmp_path / "layout.c"
+    fields_r = [f for f, _ in Request._fields_]
+    fields_i = [f for f, _ in RequestInfo._fields_]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void){',
+             'printf("%zu\\n", sizeof(nckde_request));']
+    lines += [f'printf("%zu\\n", offsetof(nckde_request, {f}));' for f in fields_r]
+    lines += ['printf("%zu\\n", sizeof(nckde_request_info));']
+    lines += [f'printf("%zu\\n", offsetof(nckde_request_info, {f}));' for f in fields_i]
+    lines += ['return 0;}']
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [C.sizeof(Request)] + [getattr(Request, f).offset for f in fields_r]
+    want += [C.sizeof(RequestInfo)] + [getattr(RequestInfo, f).offset for f in fields_i]
+    assert got == want
