@@ -546,109 +546,153 @@ constexpr int CH_NB = 64;
 __global__ void __launch_bounds__ (256)
 k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ info)
 {
+  // right-looking in LDS: per column j, every thread takes the pivot, the
+  // row j of U is scaled into `row`, then the trailing upper triangle is
+  // updated by all threads (two barriers per column)
   __shared__ double B[CH_NB][CH_NB + 1];
-  __shared__ int s_fail;
+  __shared__ double row[CH_NB];
   const int tid = threadIdx.x;
 
   if (*info != 0)
     return;
 
-  for (int e = tid; e < nb * nb; e += 256)
   {
-    const int r = e / nb, c = e % nb;
+    // all 16 loads of a thread in flight before the LDS stores
+    double v[16];
 
-    B[r][c] = S[(size_t) (kb + r) * np + kb + c];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+    {
+      const int e = tid + q * 256, r = e >> 6, c = e & 63;
+
+      v[q] = (r < nb && c < nb) ? S[(size_t) (kb + r) * np + kb + c] : 0.0;
+    }
+
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+    {
+      const int e = tid + q * 256;
+
+      B[e >> 6][e & 63] = v[q];
+    }
   }
-
-  if (tid == 0)
-    s_fail = 0;
 
   __syncthreads ();
 
+  int fail = 0;
+
   for (int j = 0; j < nb; ++j)
   {
-    if (tid == 0)
+    const double djj = B[j][j];
+
+    if (!(djj > 0.0))
     {
-      double s = B[j][j];
+      fail = kb + j + 1;
+      break;
+    }
 
-      for (int k = 0; k < j; ++k)
-        s = fma (-B[k][j], B[k][j], s);
+    const double ujj = sqrt (djj);
 
-      if (!(s > 0.0))
-        s_fail = kb + j + 1;
-      else
-        B[j][j] = sqrt (s);
+    __syncthreads ();
+
+    if (tid >= j && tid < nb)
+    {
+      const double v = (tid == j) ? ujj : B[j][tid] / ujj;
+
+      row[tid]  = v;
+      B[j][tid] = v;
     }
 
     __syncthreads ();
 
-    if (s_fail)
-      break;
-
-    const double ujj = B[j][j];
-
-    for (int i = j + 1 + tid; i < nb; i += 256)
-    {
-      double t = B[j][i];
-
-      for (int k = 0; k < j; ++k)
-        t = fma (-B[k][j], B[k][i], t);
-
-      B[j][i] = t / ujj;
-    }
+    for (int r = j + 1 + (tid >> 4); r < nb; r += 16)
+      for (int c = j + 1 + (tid & 15); c < nb; c += 16)
+        if (r <= c)
+          B[r][c] = fma (-row[r], row[c], B[r][c]);
 
     __syncthreads ();
   }
 
-  if (s_fail)
+  if (fail)
   {
     if (tid == 0)
-      *info = s_fail;
+      *info = fail;
 
     return;
   }
 
-  for (int e = tid; e < nb * nb; e += 256)
+#pragma unroll
+  for (int q = 0; q < 16; ++q)
   {
-    const int r = e / nb, c = e % nb;
+    const int e = tid + q * 256, r = e >> 6, c = e & 63;
 
-    if (c >= r)
+    if (r < nb && c < nb && c >= r)
       S[(size_t) (kb + r) * np + kb + c] = B[r][c];
   }
 }
 
-// row panel: for each column c >= kb+nb solve U_kk^T x = S[kb:kb+nb][c],
-// one lane per column (the column's earlier rows are re-read from cache)
+// row panel: for each column c >= kb+nb solve U_kk^T x = S[kb:kb+nb][c].
+// A 64-lane block takes 64 columns: the panel block and U_kk sit in LDS and
+// each lane runs the forward substitution on its column (four partial sums
+// per row to shorten the dependency chain).
 __global__ void __launch_bounds__ (64)
 k_trsm_row (double *__restrict__ S, int np, int kb, int nb, const int *__restrict__ info)
 {
   __shared__ double U[CH_NB][CH_NB + 1];
-  const int c = kb + nb + blockIdx.x * 64 + threadIdx.x;
+  __shared__ double Xs[CH_NB][64 + 1];
+  const int lane = threadIdx.x;
+  const int c0   = kb + nb + blockIdx.x * 64;
+  const int c    = c0 + lane;
 
   if (*info != 0)
     return;
 
-  for (int e = threadIdx.x; e < nb * nb; e += 64)
+  // staged 16 rows at a time so that 16 loads per lane are in flight
+  for (int r0 = 0; r0 < CH_NB; r0 += 16)
   {
-    const int r = e / nb, q = e % nb;
+    double vu[16], vx[16];
 
-    U[r][q] = S[(size_t) (kb + r) * np + kb + q];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+    {
+      const int r = r0 + q;
+
+      vu[q] = (r < nb && lane < nb) ? S[(size_t) (kb + r) * np + kb + lane] : (r == lane ? 1.0 : 0.0);
+      vx[q] = (r < nb && c < np) ? S[(size_t) (kb + r) * np + c] : 0.0;
+    }
+
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+    {
+      U[r0 + q][lane]  = vu[q];
+      Xs[r0 + q][lane] = vx[q];
+    }
   }
 
   __syncthreads ();
 
-  if (c >= np)
-    return;
-
   for (int r = 0; r < nb; ++r)
   {
-    double t = S[(size_t) (kb + r) * np + c];
+    double t0 = Xs[r][lane], t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    int q     = 0;
 
-    for (int q = 0; q < r; ++q)
-      t = fma (-U[q][r], S[(size_t) (kb + q) * np + c], t);
+    for (; q + 4 <= r; q += 4)
+    {
+      t0 = fma (-U[q][r], Xs[q][lane], t0);
+      t1 = fma (-U[q + 1][r], Xs[q + 1][lane], t1);
+      t2 = fma (-U[q + 2][r], Xs[q + 2][lane], t2);
+      t3 = fma (-U[q + 3][r], Xs[q + 3][lane], t3);
+    }
 
-    S[(size_t) (kb + r) * np + c] = t / U[r][r];
+    for (; q < r; ++q)
+      t0 = fma (-U[q][r], Xs[q][lane], t0);
+
+    Xs[r][lane] = ((t0 + t1) + (t2 + t3)) / U[r][r];
   }
+
+  if (c < np)
+    for (int r = 0; r < nb; ++r)
+      S[(size_t) (kb + r) * np + c] = Xs[r][lane];
 }
 
 // lower triangle <- transpose of the upper factor (for the backward solve)
@@ -707,11 +751,24 @@ k_trsv_panel (const double *__restrict__ S, int np, int p0, int pb, double *__re
   __shared__ double y[TS_NB];
   const int tid = threadIdx.x;
 
-  for (int e = tid; e < TS_NB * TS_NB; e += 256)
   {
-    const int j = e / TS_NB, i = e % TS_NB;
+    double v[16];
 
-    T[i][j] = (i < pb && j < pb) ? S[(size_t) (p0 + j) * np + p0 + i] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+    {
+      const int e = tid + q * 256, j = e >> 6, i = e & 63;
+
+      v[q] = (i < pb && j < pb) ? S[(size_t) (p0 + j) * np + p0 + i] : 0.0;
+    }
+
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+    {
+      const int e = tid + q * 256;
+
+      T[e & 63][e >> 6] = v[q];
+    }
   }
 
   __syncthreads ();
@@ -746,12 +803,22 @@ k_trsv_panel (const double *__restrict__ S, int np, int p0, int pb, double *__re
 
   if (row < hi)
   {
-    double t = rhs[row];
+    double t[4] = {rhs[row], 0.0, 0.0, 0.0};
 
-    for (int c = 0; c < pb; ++c)
-      t = fma (-S[(size_t) (p0 + c) * np + row], y[c], t);
+    for (int c0 = 0; c0 < pb; c0 += 16)
+    {
+      double v[16];
 
-    rhs[row] = t;
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        v[q] = (c0 + q < pb) ? S[(size_t) (p0 + c0 + q) * np + row] : 0.0;
+
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        t[q & 3] = fma (-v[q], (c0 + q < pb) ? y[c0 + q] : 0.0, t[q & 3]);
+    }
+
+    rhs[row] = (t[0] + t[1]) + (t[2] + t[3]);
   }
 }
 
