@@ -567,6 +567,22 @@ group_step (const GroupData<DP> &cur, const half8 (&Bh)[NPT][Plan<DP>::NKB], con
   }
 }
 
+// Tiling chosen by measurement on MI355X: up to DP = 16 a wave keeps more
+// point tiles and loads each kernel group just before its MFMA chains (the
+// occupancy of the smaller register set hides the loads); at DP >= 32 the next
+// group is prefetched into a second register set.
+constexpr int
+tiles_per_wave (int DP)
+{
+  return (DP == 64) ? 2 : ((DP == 16) ? 8 : 4);
+}
+
+constexpr bool
+prefetch_groups (int DP)
+{
+  return DP >= 32;
+}
+
 template <int DP, int KERNEL, int NPT, bool OFFSET, bool SCALED, int NST>
 __device__ inline void
 group_loop (int g0, int g1, const _Float16 *__restrict__ A, const float *__restrict__ cv,
@@ -583,6 +599,34 @@ group_loop (int g0, int g1, const _Float16 *__restrict__ A, const float *__restr
 
   // two register sets alternate roles: while one group is computed the next
   // one is loaded into the other set (no register copies)
+  if (!prefetch_groups (DP))
+  {
+    for (int tg = g0; tg < g1; ++tg)
+    {
+      GroupData<DP> gd;
+
+      load_group<DP> (gd, A, cv, coef, tg, lane);
+      group_step<DP, KERNEL, NPT, OFFSET, SCALED, NST> (gd, Bh, Bl, tscale, it2, M0, Mb, S32, kappa, g);
+
+      // fold the FP32 partial sums into FP64 every 16 groups
+      if (((tg + 1 - g0) & 15) == 0)
+      {
+#pragma unroll
+        for (int q = 0; q < NST; ++q)
+        {
+          S64[q] += (double) S32[q];
+          S32[q] = 0.0f;
+        }
+      }
+    }
+
+#pragma unroll
+    for (int q = 0; q < NST; ++q)
+      S64[q] += (double) S32[q];
+
+    return;
+  }
+
   GroupData<DP> ga, gb;
   int tg = g0;
 
@@ -804,7 +848,7 @@ template <int DP, int KERNEL, bool OFFSET>
 static hipError_t
 eval_t (const MixedArgs &a, hipStream_t s)
 {
-  constexpr int NPT = (DP == 64) ? 2 : 4;
+  constexpr int NPT = tiles_per_wave (DP);
   const int pts     = 4 * NPT * 32;
   dim3 grid ((a.m + pts - 1) / pts, a.nsplit);
 
@@ -858,7 +902,7 @@ eval_d (const MixedArgs &a, hipStream_t s)
 int
 mixed_points_per_block (int DP)
 {
-  return 4 * ((DP == 64) ? 2 : 4) * 32;
+  return 4 * tiles_per_wave (DP) * 32;
 }
 
 hipError_t
