@@ -336,29 +336,58 @@ k_syrk_mfma (const double *__restrict__ P, int n, int K, int ld, int off, int kb
   }
 
   // D[i][j] of a 16 x 16 tile: register r of a lane holds i = lane / 16 + 4 r, j = lane % 16
+  if (MODE == 0)
+  {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-      {
-        const int a = r0 + wr + i * 16 + lk + 4 * r;
-        const int b = c0 + wc + j * 16 + li;
-
-        if (a < n && b < n && a <= b)
+        for (int r = 0; r < 4; ++r)
         {
-          if (MODE == 0)
+          const int a = r0 + wr + i * 16 + lk + 4 * r;
+          const int b = c0 + wc + j * 16 + li;
+
+          if (a < n && b < n && a <= b)
           {
             M[(size_t) a * n + b] = acc[i][j][r];
             M[(size_t) b * n + a] = acc[i][j][r];
           }
-          else
-          {
-            M[(size_t) a * ld + b] -= acc[i][j][r];
-          }
         }
-      }
+  }
+  else
+  {
+    // read-modify-write of the tile, one 16-column block at a time with all
+    // 16 loads of a lane in flight (out-of-range entries read element 0)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+    {
+      const int b = c0 + wc + j * 16 + li;
+      double old[4][4];
+
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+          const int a  = r0 + wr + i * 16 + lk + 4 * r;
+          const bool v = (a < n && b < n && a <= b);
+
+          old[i][r] = M[v ? (size_t) a * ld + b : 0];
+        }
+
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+          const int a = r0 + wr + i * 16 + lk + 4 * r;
+
+          if (a < n && b < n && a <= b)
+            M[(size_t) a * ld + b] = old[i][r] - acc[i][j][r];
+        }
+    }
+  }
 }
 
 template <int MODE>
@@ -546,18 +575,20 @@ constexpr int CH_NB = 64;
 __global__ void __launch_bounds__ (256)
 k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ info)
 {
-  // right-looking in LDS: per column j, every thread takes the pivot, the
-  // row j of U is scaled into `row`, then the trailing upper triangle is
-  // updated by all threads (two barriers per column)
+  // right-looking in LDS: per column j every thread takes the pivot, row j
+  // of U is scaled into `row`, then the trailing upper triangle is updated
+  // with thread (tr, tc) of a 16 x 16 grid owning entries (j+1+tr+16a,
+  // j+1+tc+16b): its 16 entries and 8 row values are read in one batch and
+  // written in one batch (two barriers per column)
   __shared__ double B[CH_NB][CH_NB + 1];
   __shared__ double row[CH_NB];
   const int tid = threadIdx.x;
+  const int tr  = tid >> 4, tc = tid & 15;
 
   if (*info != 0)
     return;
 
   {
-    // all 16 loads of a thread in flight before the LDS stores
     double v[16];
 
 #pragma unroll
@@ -605,10 +636,38 @@ k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ 
 
     __syncthreads ();
 
-    for (int r = j + 1 + (tid >> 4); r < nb; r += 16)
-      for (int c = j + 1 + (tid & 15); c < nb; c += 16)
-        if (r <= c)
-          B[r][c] = fma (-row[r], row[c], B[r][c]);
+    double ur[4], uc[4], x[4][4];
+
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+    {
+      const int r = j + 1 + tr + 16 * a;
+      const int c = j + 1 + tc + 16 * a;
+
+      ur[a] = (r < nb) ? row[r] : 0.0;
+      uc[a] = (c < nb) ? row[c] : 0.0;
+    }
+
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+      {
+        const int r = j + 1 + tr + 16 * a, c = j + 1 + tc + 16 * b;
+
+        x[a][b] = (r < nb && c < nb) ? B[r][c] : 0.0;
+      }
+
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+      {
+        const int r = j + 1 + tr + 16 * a, c = j + 1 + tc + 16 * b;
+
+        if (r < nb && c < nb && r <= c)
+          B[r][c] = fma (-ur[a], uc[b], x[a][b]);
+      }
 
     __syncthreads ();
   }
@@ -632,30 +691,30 @@ k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ 
 }
 
 // row panel: for each column c >= kb+nb solve U_kk^T x = S[kb:kb+nb][c].
-// A 64-lane block takes 64 columns: the panel block and U_kk sit in LDS and
-// each lane runs the forward substitution on its column (four partial sums
-// per row to shorten the dependency chain).
-__global__ void __launch_bounds__ (64)
+// A block takes 64 columns: all four waves stage the panel block and U_kk
+// in LDS, then wave 0 runs the forward substitution, one lane per column
+// (four partial sums per row to shorten the dependency chain), and all
+// waves write the result back.
+__global__ void __launch_bounds__ (256)
 k_trsm_row (double *__restrict__ S, int np, int kb, int nb, const int *__restrict__ info)
 {
   __shared__ double U[CH_NB][CH_NB + 1];
   __shared__ double Xs[CH_NB][64 + 1];
-  const int lane = threadIdx.x;
+  const int tid  = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
   const int c0   = kb + nb + blockIdx.x * 64;
   const int c    = c0 + lane;
 
   if (*info != 0)
     return;
 
-  // staged 16 rows at a time so that 16 loads per lane are in flight
-  for (int r0 = 0; r0 < CH_NB; r0 += 16)
   {
     double vu[16], vx[16];
 
 #pragma unroll
     for (int q = 0; q < 16; ++q)
     {
-      const int r = r0 + q;
+      const int r = w * 16 + q;
 
       vu[q] = (r < nb && lane < nb) ? S[(size_t) (kb + r) * np + kb + lane] : (r == lane ? 1.0 : 0.0);
       vx[q] = (r < nb && c < np) ? S[(size_t) (kb + r) * np + c] : 0.0;
@@ -664,35 +723,46 @@ k_trsm_row (double *__restrict__ S, int np, int kb, int nb, const int *__restric
 #pragma unroll
     for (int q = 0; q < 16; ++q)
     {
-      U[r0 + q][lane]  = vu[q];
-      Xs[r0 + q][lane] = vx[q];
+      U[w * 16 + q][lane]  = vu[q];
+      Xs[w * 16 + q][lane] = vx[q];
     }
   }
 
   __syncthreads ();
 
-  for (int r = 0; r < nb; ++r)
+  if (w == 0)
   {
-    double t0 = Xs[r][lane], t1 = 0.0, t2 = 0.0, t3 = 0.0;
-    int q     = 0;
-
-    for (; q + 4 <= r; q += 4)
+    for (int r = 0; r < nb; ++r)
     {
-      t0 = fma (-U[q][r], Xs[q][lane], t0);
-      t1 = fma (-U[q + 1][r], Xs[q + 1][lane], t1);
-      t2 = fma (-U[q + 2][r], Xs[q + 2][lane], t2);
-      t3 = fma (-U[q + 3][r], Xs[q + 3][lane], t3);
+      double t0 = Xs[r][lane], t1 = 0.0, t2 = 0.0, t3 = 0.0;
+      int q     = 0;
+
+      for (; q + 4 <= r; q += 4)
+      {
+        t0 = fma (-U[q][r], Xs[q][lane], t0);
+        t1 = fma (-U[q + 1][r], Xs[q + 1][lane], t1);
+        t2 = fma (-U[q + 2][r], Xs[q + 2][lane], t2);
+        t3 = fma (-U[q + 3][r], Xs[q + 3][lane], t3);
+      }
+
+      for (; q < r; ++q)
+        t0 = fma (-U[q][r], Xs[q][lane], t0);
+
+      Xs[r][lane] = ((t0 + t1) + (t2 + t3)) / U[r][r];
     }
-
-    for (; q < r; ++q)
-      t0 = fma (-U[q][r], Xs[q][lane], t0);
-
-    Xs[r][lane] = ((t0 + t1) + (t2 + t3)) / U[r][r];
   }
 
+  __syncthreads ();
+
   if (c < np)
-    for (int r = 0; r < nb; ++r)
-      S[(size_t) (kb + r) * np + c] = Xs[r][lane];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+    {
+      const int r = w * 16 + q;
+
+      if (r < nb)
+        S[(size_t) (kb + r) * np + c] = Xs[r][lane];
+    }
 }
 
 // lower triangle <- transpose of the upper factor (for the backward solve)
@@ -719,7 +789,7 @@ launch_potrf (double *S, int np, int *info, hipStream_t s)
     {
       const int rest = np - kb - nb;
 
-      hipLaunchKernelGGL (k_trsm_row, dim3 ((rest + 63) / 64), dim3 (64), 0, s, S, np, kb, nb, info);
+      hipLaunchKernelGGL (k_trsm_row, dim3 ((rest + 63) / 64), dim3 (256), 0, s, S, np, kb, nb, info);
       launch_syrk_mfma<1> (S, np, nb, np, kb + nb, kb, S, s);
     }
   }
