@@ -297,7 +297,7 @@ def main(argv=None):
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc passes")
     ap.add_argument("--serve-requests", type=int, default=512)
-    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-rate", type=float, default=10.0)
     ap.add_argument("--serve-sessions", type=int, default=48)
     ap.add_argument("--serve-slots", type=int, default=8)
     args = ap.parse_args(argv)
@@ -328,6 +328,15 @@ def main(argv=None):
     if not args.no_serve:
         latency = serve_case(args.serve_requests, args.serve_rate, args.serve_sessions, args.serve_slots)
 
+    # HBM bytes per launch of the dominant kernel, from the rocprofv3 --pmc
+    # FETCH_SIZE / WRITE_SIZE passes (scaled by a 256 MiB copy) recorded in
+    # profiles/eval_m2lnp_vkde_c2_traffic.json
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "eval_m2lnp_vkde_c2_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
     best_key = "mixed" if ("mixed" in main_case and main_case["mixed"]["rel_err_max"] <= 1e-6) else "fp64"
     best = main_case[best_key]
     d = main_case["d"]
@@ -350,7 +359,7 @@ def main(argv=None):
                                "m=100000 eval points in HBM", "n_kernels": 4000, "d": d, "m": 100000,
                    "precision": best_key},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": peak, "unit": "TFLOP/s",
-                     "frac": tflops / peak, "traffic": None},
+                     "frac": tflops / peak, "traffic": traffic if best_key == "mixed" else None},
         "cpu_baseline": {"value": main_case["cpu_pairs_per_s"], "unit": "pairs/s", "cores": cores,
                          "kind": "port", "sample": main_case["cpu_sample"] + " (FP64 oracle eval_m2lnp)"},
         "fp64": main_case.get("fp64"),

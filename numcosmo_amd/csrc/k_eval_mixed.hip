@@ -261,21 +261,31 @@ k_mixed_centres (const double *__restrict__ X, int n, int d, const double *__res
   atomicMax (ymax_bits, (unsigned long long) __double_as_longlong (ym));
 }
 
-// column means of the sample (serial per column)
-__global__ void
+// column means of the sample: one block per column, strided partial sums
+// and a fixed-order tree reduction (deterministic)
+__global__ void __launch_bounds__ (256)
 k_colmean (const double *__restrict__ X, int n, int d, double *__restrict__ mu)
 {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double red[256];
+  const int l = blockIdx.x;
+  double s    = 0.0;
 
-  if (l >= d)
-    return;
-
-  double s = 0.0;
-
-  for (int i = 0; i < n; ++i)
+  for (int i = threadIdx.x; i < n; i += 256)
     s += X[(size_t) i * d + l];
 
-  mu[l] = s / n;
+  red[threadIdx.x] = s;
+  __syncthreads ();
+
+  for (int o = 128; o > 0; o >>= 1)
+  {
+    if (threadIdx.x < o)
+      red[threadIdx.x] += red[threadIdx.x + o];
+
+    __syncthreads ();
+  }
+
+  if (threadIdx.x == 0)
+    mu[l] = red[0] / n;
 }
 
 // --------------------------------------------------------------------------
@@ -936,7 +946,7 @@ mixed_build (const double *X, int n, int d, int D, int DP, const double *Wp, con
 {
   hipError_t e;
 
-  hipLaunchKernelGGL (k_colmean, dim3 (1), dim3 (64), 0, s, X, n, d, mu);
+  hipLaunchKernelGGL (k_colmean, dim3 (d), dim3 (256), 0, s, X, n, d, mu);
   e = hipMemsetAsync (ymax_bits, 0, sizeof (unsigned long long), s);
 
   if (e != hipSuccess)

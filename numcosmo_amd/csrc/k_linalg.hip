@@ -575,46 +575,49 @@ constexpr int CH_NB = 64;
 __global__ void __launch_bounds__ (256)
 k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ info)
 {
-  // right-looking in LDS: per column j every thread takes the pivot, row j
-  // of U is scaled into `row`, then the trailing upper triangle is updated
-  // with thread (tr, tc) of a 16 x 16 grid owning entries (j+1+tr+16a,
-  // j+1+tc+16b): its 16 entries and 8 row values are read in one batch and
-  // written in one batch (two barriers per column)
-  __shared__ double B[CH_NB][CH_NB + 1];
+  // right-looking with the block in registers: thread (tr, tc) of a 16 x 16
+  // grid owns entries (tr + 16 a, tc + 16 b), a, b < 4.  Per column j the
+  // owner of (j, j) publishes the pivot, the owners of row j publish the
+  // scaled row, and every thread updates its own entries from two LDS
+  // vectors (two barriers per column, 8 LDS reads per thread).
   __shared__ double row[CH_NB];
+  __shared__ double piv;
   const int tid = threadIdx.x;
   const int tr  = tid >> 4, tc = tid & 15;
 
   if (*info != 0)
     return;
 
-  {
-    double v[16];
+  double x[4][4];
 
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
-    {
-      const int e = tid + q * 256, r = e >> 6, c = e & 63;
-
-      v[q] = (r < nb && c < nb) ? S[(size_t) (kb + r) * np + kb + c] : 0.0;
-    }
-
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
+    for (int b = 0; b < 4; ++b)
     {
-      const int e = tid + q * 256;
+      const int r = tr + 16 * a, c = tc + 16 * b;
 
-      B[e >> 6][e & 63] = v[q];
+      x[a][b] = (r < nb && c < nb) ? S[(size_t) (kb + r) * np + kb + c] : 0.0;
     }
-  }
-
-  __syncthreads ();
 
   int fail = 0;
 
   for (int j = 0; j < nb; ++j)
   {
-    const double djj = B[j][j];
+    const int ja = j >> 4, jr = j & 15;
+
+    // publish the pivot (entry (j, j) lives in thread (jr, jr), slot (ja, ja))
+    if (tr == jr && tc == jr)
+    {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        if (a == ja)
+          piv = x[a][a];
+    }
+
+    __syncthreads ();
+
+    const double djj = piv;
 
     if (!(djj > 0.0))
     {
@@ -624,52 +627,47 @@ k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ 
 
     const double ujj = sqrt (djj);
 
-    __syncthreads ();
-
-    if (tid >= j && tid < nb)
+    // row j: owners scale their entries (c >= j) and publish them
+    if (tr == jr)
     {
-      const double v = (tid == j) ? ujj : B[j][tid] / ujj;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        if (a == ja)
+        {
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+          {
+            const int c = tc + 16 * b;
 
-      row[tid]  = v;
-      B[j][tid] = v;
+            if (c >= j && c < nb)
+            {
+              const double u = (c == j) ? ujj : x[a][b] / ujj;
+
+              x[a][b] = u;
+              row[c]  = u;
+            }
+          }
+        }
     }
 
     __syncthreads ();
 
-    double ur[4], uc[4], x[4][4];
+    double ur[4], uc[4];
 
 #pragma unroll
     for (int a = 0; a < 4; ++a)
     {
-      const int r = j + 1 + tr + 16 * a;
-      const int c = j + 1 + tc + 16 * a;
+      const int r = tr + 16 * a, c = tc + 16 * a;
 
-      ur[a] = (r < nb) ? row[r] : 0.0;
-      uc[a] = (c < nb) ? row[c] : 0.0;
+      ur[a] = (r > j && r < nb) ? row[r] : 0.0;
+      uc[a] = (c > j && c < nb) ? row[c] : 0.0;
     }
 
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b)
-      {
-        const int r = j + 1 + tr + 16 * a, c = j + 1 + tc + 16 * b;
-
-        x[a][b] = (r < nb && c < nb) ? B[r][c] : 0.0;
-      }
-
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b)
-      {
-        const int r = j + 1 + tr + 16 * a, c = j + 1 + tc + 16 * b;
-
-        if (r < nb && c < nb && r <= c)
-          B[r][c] = fma (-ur[a], uc[b], x[a][b]);
-      }
-
-    __syncthreads ();
+        x[a][b] = fma (-ur[a], uc[b], x[a][b]);
   }
 
   if (fail)
@@ -681,13 +679,15 @@ k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ 
   }
 
 #pragma unroll
-  for (int q = 0; q < 16; ++q)
-  {
-    const int e = tid + q * 256, r = e >> 6, c = e & 63;
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+    {
+      const int r = tr + 16 * a, c = tc + 16 * b;
 
-    if (r < nb && c < nb && c >= r)
-      S[(size_t) (kb + r) * np + kb + c] = B[r][c];
-  }
+      if (r < nb && c < nb && c >= r)
+        S[(size_t) (kb + r) * np + kb + c] = x[a][b];
+    }
 }
 
 // row panel: for each column c >= kb+nb solve U_kk^T x = S[kb:kb+nb][c].
