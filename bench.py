@@ -262,6 +262,73 @@ def serve_case(n_req, rate, n_sessions, slots, seed=None):
     return res
 
 
+def apes_case(steps, precision):
+    """APES ensemble steps: d=16, 2 x 3200 walkers on a seeded 4-component
+    Gaussian mixture (equal weights, means U[-5,5]^16, random SPD
+    covariances), VKDE with the Cauchy kernel, over_smooth 1; each half-step
+    fits a VKDE of the other half (prepare_interp), samples 3200 proposals and
+    evaluates both transition densities in one batch."""
+    import numcosmo_amd as na
+    from numcosmo_amd import synth
+    from numcosmo_amd.apes import APES, InterpolationKernel
+
+    d, nw = 16, 6400
+    g = synth.rng(3)
+    tgt = synth.Mixture(d, g)
+    x0 = tgt.draw(g, nw)
+    s = APES(nwalkers=nw, ndim=d, log_prob_batch=lambda th: -0.5 * tgt.m2lnp(th),
+             interpolation_kernel=InterpolationKernel.CAUCHY, over_smooth=1.0, precision=int(precision), seed=7)
+    log(f"[apes] d={d} walkers={nw} steps={steps} precision={int(precision)}")
+    t0 = time.perf_counter()
+    s.run_mcmc(x0, steps)
+    wall = time.perf_counter() - t0
+    st = s.get_stats()
+    pairs = steps * 2 * (2 * (nw // 2) * (nw // 2))
+    res = {"d": d, "walkers": nw, "steps": steps, "precision": "mixed" if int(precision) else "fp64",
+           "s_per_step": wall / steps, "acceptance": st["acceptance"],
+           "seconds": {k: v / steps for k, v in st["seconds"].items()},
+           "transition_pairs_per_s": pairs / max(st["seconds"]["eval"], 1e-9)}
+    log(f"[apes] {wall / steps * 1e3:.1f} ms/step, acceptance {st['acceptance']:.3f}, per step "
+        f"{ {k: round(v * 1e3, 2) for k, v in res['seconds'].items()} } ms")
+    return res
+
+
+def c5_case(m, m_fp64):
+    """Large VKDE: d=64, n=50000, Student-t nu=5, Gaussian target with
+    condition number 1e4; prepare_interp (50000^2 interpolation matrix, NNLS
+    weights) then eval_m2lnp at m points in mixed precision and at the first
+    m_fp64 of them in FP64 (the mixed results are checked against those)."""
+    import torch
+
+    import numcosmo_amd as na
+    from numcosmo_amd import synth
+
+    d, n = 64, 50000
+    log(f"[C5] generating d={d} n={n} m={m}")
+    tgt, X, m2lnp, Xe = synth.workload("cond", n, m, d, 5)
+    sd = na.StatsDistVKDE.new(na.StatsDistKernelST.new(d, 5.0), na.StatsDistCV.NONE)
+    sd.add_obs_array(X)
+    t0 = time.perf_counter()
+    sd.prepare_interp(m2lnp)
+    prep = time.perf_counter() - t0
+    log(f"[C5] prepare_interp {prep:.1f} s, {sd.get_n_kernels()} kernels")
+    Xd = torch.tensor(Xe, dtype=torch.float64, device="cuda")
+    outm = torch.empty(m, dtype=torch.float64, device="cuda")
+    out64 = torch.empty(m_fp64, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.Stream()
+    nk = sd.get_n_kernels()
+    ms_m, _, k_m = timed_eval(sd, Xd, outm, na.Precision.MIXED, 3, 1, stream)
+    ms_f, _, k_f = timed_eval(sd, Xd[:m_fp64].contiguous(), out64, na.Precision.FP64, 1, 1, stream)
+    emax, e99 = rel_err(outm[:m_fp64].cpu().numpy(), out64.cpu().numpy())
+    res = {"d": d, "n": n, "m": m, "prepare_interp_s": prep,
+           "mixed": {"ms_per_step": ms_m, "pairs_per_s": nk * m / (ms_m * 1e-3)},
+           "fp64": {"m": m_fp64, "ms_per_step": ms_f, "pairs_per_s": nk * m_fp64 / (ms_f * 1e-3)},
+           "mixed_vs_fp64_rel_err_max": emax, "mixed_vs_fp64_rel_err_p99": e99}
+    log(f"[C5] mixed {ms_m:.1f} ms ({res['mixed']['pairs_per_s']:.3e} pairs/s), fp64 {ms_f:.1f} ms for "
+        f"{m_fp64} points ({res['fp64']['pairs_per_s']:.3e} pairs/s), mixed vs fp64 rel err max {emax:.2e}")
+    return res
+
+
 def pmc_probe(steps):
     """Short run for hardware-counter passes: the workload's mixed and FP64
     evals, then a plain device copy of 256 MiB (the calibration kernel of the
@@ -296,6 +363,9 @@ def main(argv=None):
     ap.add_argument("--no-north-star", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc passes")
+    ap.add_argument("--apes-steps", type=int, default=40, help="APES ensemble steps (0: skip)")
+    ap.add_argument("--c5", action="store_true", help="also run the d=64, n=50000 VKDE (minutes)")
+    ap.add_argument("--only-c5", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=10.0)
     ap.add_argument("--serve-sessions", type=int, default=48)
@@ -314,6 +384,10 @@ def main(argv=None):
         pmc_probe(max(2, min(args.steps, 5)))
         return 0
 
+    if args.only_c5:
+        print(json.dumps({"c5": c5_case(1000000, 100000)}))
+        return 0
+
     from oracle import oracle as O
 
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -323,6 +397,13 @@ def main(argv=None):
     if not args.no_north_star:
         ns = run_case("north-star", "gaussian", 16, 20000, 200000, 3.0, max(3, args.steps // 4), 1, False,
                       200, 4, O, cores)
+
+    c5 = c5_case(1000000, 100000) if args.c5 else None
+
+    apes = None
+    if args.apes_steps > 0:
+        import numcosmo_amd as na
+        apes = apes_case(args.apes_steps, na.Precision.MIXED)
 
     latency = None
     if not args.no_serve:
@@ -367,6 +448,8 @@ def main(argv=None):
         "prepare_interp_s": main_case["prepare_s"],
         "north_star": ns,
         "latency": latency,
+        "apes": apes,
+        "c5": c5,
     }
     print(json.dumps(line))
     return 0

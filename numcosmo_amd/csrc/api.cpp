@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -488,12 +489,31 @@ struct NNLS
 
 // unconstrained solve on the sorted index list P (normal equations:
 // Cholesky, else Gaussian elimination); solution scattered into dxo
+// NCKDE_PROGRESS=1 in the environment: one stderr line per normal-equation
+// solve of the NNLS (long fits report as they go)
+static bool
+progress_enabled ()
+{
+  static const bool on = [] {
+    const char *v = std::getenv ("NCKDE_PROGRESS");
+    return v != nullptr && v[0] != '\0' && v[0] != '0';
+  }();
+
+  return on;
+}
+
 static int
 nnls_unconstrained (nckde *h, int n, const std::vector<int> &P, double *dxo)
 {
   hipStream_t s = h->stream;
   const int np  = (int) P.size ();
   int info      = 0;
+
+  if (progress_enabled ())
+  {
+    fprintf (stderr, "nckde nnls: solve on %d of %d columns\n", np, n);
+    fflush (stderr);
+  }
 
   if (np == 0)
   {
