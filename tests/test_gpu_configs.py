@@ -1,0 +1,118 @@
+"""Parity at the benchmark configurations' own sizes.
+
+* KDE, Gauss kernel, d=4, n=1000, Gaussian target, m=10000.
+  - The oracle runs the whole fit and the eval.
+  - Checks: weights to rel L1 1e-7; eval with the device weights to 1e-12.
+* VKDE, Student-t nu=3, d=8, n=4000, banana target, m=100000.
+* VKDE, Cauchy kernel (nu=1), d=16, n=3200: one APES half-ensemble of the mixture target.
+
+The oracle's active-set fit at n >= 3200 takes many CPU minutes, so the two VKDE cases pin the fit
+differently:
+- the prepared factors against the oracle's prepare;
+- the device weights by the KKT conditions of the NNLS problem on the oracle's interpolation matrix;
+- eval against the oracle holding the device weights (FP64 to 1e-12, mixed precision to 1e-6 on a
+  subset);
+- the mixed path against the FP64 path on all m points.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from numcosmo_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+GAUSS, ST = 0, 1
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+
+
+def _kkt_rel(A, w, shrink, n):
+    """KKT residual of the NNLS problem min |1 - A x|, x >= 0, for the
+    solution behind the shrunk, normalised weights w."""
+    v = w - shrink / n
+    Av = A @ v
+    x = v * (Av.sum() / (Av @ Av))
+    g = A.T @ (1.0 - A @ x)
+    scale = np.abs(A.T @ np.ones(A.shape[0])).max()
+    act = x > 0
+    return max(np.abs(g[act]).max(initial=0.0), g[~act].max(initial=0.0)) / scale
+
+
+def test_kde_gauss_d4_n1000_full_oracle(oracle):
+    import numcosmo_amd as na
+    _say("[config] KDE Gauss d=4 n=1000 m=10000")
+    tgt, X, m2, Xe = synth.workload("gaussian", 1000, 10000, 4, 1)
+    sd = na.StatsDistKDE.new(na.StatsDistKernelGauss.new(4))
+    od = oracle.StatsDist(oracle.KDE, oracle.GAUSS, 4, 0.0)
+    sd.add_obs_array(X)
+    od.add_obs_array(X)
+    sd.prepare_interp(m2)
+    od.prepare_interp(m2)
+    w, wo = sd.peek_weights(), od.get_weights()
+    rel_l1 = np.abs(w - wo).sum() / np.abs(wo).sum()
+    _say(f"   weights rel L1 {rel_l1:.2e}")
+    assert rel_l1 <= 1e-7
+    assert np.max(_rel(sd.eval_m2lnp_batch(Xe), od.eval_m2lnp_batch(Xe))) <= 1e-7
+    od.set_weights(w)
+    err = np.max(_rel(sd.eval_m2lnp_batch(Xe), od.eval_m2lnp_batch(Xe)))
+    _say(f"   FP64 eval rel err (device weights in the oracle) {err:.2e}")
+    assert err <= 1e-12
+
+
+def _vkde_full(oracle, kernel, nu, d, X, m2, Xe, check_pts):
+    import numcosmo_amd as na
+    sdk = na.StatsDistKernelGauss.new(d) if kernel == GAUSS else na.StatsDistKernelST.new(d, nu)
+    sd = na.StatsDistVKDE.new(sdk)
+    sd.add_obs_array(X)
+    sd.prepare_interp(m2)
+    n = X.shape[0]
+    assert sd.get_n_kernels() == n   # no range cut for these targets
+
+    od = oracle.StatsDist(oracle.VKDE, kernel, d, nu)
+    od.add_obs_array(X)
+    od.prepare()
+    # prepared factors
+    for i in (0, n // 2, n - 1):
+        assert np.array_equal(sd.peek_cov_decomp(i), od.get_cov_decomp(i))
+        assert sd.get_lnnorm(i) == pytest.approx(od.get_lnnorm(i), rel=1e-14, abs=1e-14)
+    # the fit: KKT on the oracle's interpolation matrix, scaled by 1/f
+    w = sd.peek_weights()
+    assert np.all(w > 0) and w.sum() == pytest.approx(1.0, rel=1e-12)
+    A = od.compute_IM() / np.exp(-0.5 * (m2 - m2.min()))[:, None]
+    kkt = _kkt_rel(A, w, 0.01, n)
+    _say(f"   KKT residual {kkt:.2e}")
+    assert kkt < 1e-8
+    # eval with the device weights in the oracle
+    od.set_weights(w)
+    sub = Xe[:check_pts]
+    ref = od.eval_m2lnp_batch(sub)
+    e64 = np.max(_rel(sd.eval_m2lnp_batch(sub, precision=na.Precision.FP64), ref))
+    emx = np.max(_rel(sd.eval_m2lnp_batch(sub, precision=na.Precision.MIXED), ref))
+    # mixed vs FP64 on every point
+    full = np.max(_rel(sd.eval_m2lnp_batch(Xe, precision=na.Precision.MIXED),
+                       sd.eval_m2lnp_batch(Xe, precision=na.Precision.FP64)))
+    _say(f"   FP64 vs oracle {e64:.2e}, mixed vs oracle {emx:.2e}, mixed vs FP64 on {len(Xe)} points {full:.2e}")
+    assert e64 <= 1e-12
+    assert emx <= 1e-6
+    assert full <= 1e-6
+
+
+def test_vkde_st3_d8_banana_n4000(oracle):
+    _say("[config] VKDE Student-t nu=3 d=8 banana n=4000 m=100000")
+    tgt, X, m2, Xe = synth.workload("banana", 4000, 100000, 8, 2)
+    _vkde_full(oracle, ST, 3.0, 8, X, m2, Xe, 2000)
+
+
+def test_vkde_cauchy_d16_mixture_half_ensemble(oracle):
+    _say("[config] VKDE Cauchy d=16 mixture n=3200 (APES half-ensemble)")
+    tgt, X, m2, Xe = synth.workload("mixture", 3200, 6400, 16, 3)
+    _vkde_full(oracle, ST, 1.0, 16, X, m2, Xe, 1000)
