@@ -41,7 +41,8 @@ enum { NCKDE_KDE = 0, NCKDE_VKDE = 1 };
 enum { NCKDE_KERNEL_GAUSS = 0, NCKDE_KERNEL_ST = 1 };
 /* NcmStatsDistCV (ncm_stats_dist.h:77-85); only CV_NONE is implemented */
 enum { NCKDE_CV_NONE = 0, NCKDE_CV_SPLIT = 1, NCKDE_CV_SPLIT_NOFIT = 2, NCKDE_CV_LOO = 3 };
-/* NcmStatsDistKDECovType (ncm_stats_dist_kde.h:64-72); SAMPLE and FIXED */
+/* NcmStatsDistKDECovType (ncm_stats_dist_kde.h:64-72): sample, fixed, robust diagonal (Qn scales) and
+ * robust OGK covariances (ncm_stats_vec.c:1834-2064) */
 enum { NCKDE_COV_SAMPLE = 0, NCKDE_COV_FIXED = 1, NCKDE_COV_ROBUST_DIAG = 2, NCKDE_COV_ROBUST = 3 };
 /* arithmetic of the pairwise distance and kernel sums in eval */
 enum { NCKDE_PREC_FP64 = 0, NCKDE_PREC_MIXED = 1 };

@@ -116,7 +116,7 @@ struct nckde
   double ysc = 1.0, bref = 0.0;
   // work buffers
   DBuf dXe, dout, dpartM, dpartS, du, dz, dchi2, didx;
-  DBuf dAT, dM, dS, dbv, dbs, dx, dxt, dr, drt, dg, dP, dpart, dscal, dm2lnp, dlu, dtv, drw;
+  DBuf dAT, dM, dS, dbv, dbs, dx, dxt, dr, drt, dg, dP, dpart, dscal, dm2lnp, dlu, dtv, drw, drobust;
 };
 
 #define HIPCHK(call)                                                                 \
@@ -256,6 +256,30 @@ factor_batch (nckde *h, const double *dcov, double *dU, double *dlnnorm, size_t 
   return NCKDE_OK;
 }
 
+// robust covariance (ncm_stats_vec.c:1834-2064) of nmats row sets of nrows
+// rows each (rows == NULL: the first nrows rows, one set) into dcov
+static int
+robust_cov (nckde *h, int type, int nrows, const int *rows, int nmats, double *dcov)
+{
+  if (nrows < 4)
+  {
+    set_error ("ncm_stats_vec_compute_cov_robust_diag: too few points to estimate the covariance [%d].", nrows);
+    return NCKDE_ESAMPLE;
+  }
+
+  if (h->d > 64)
+  {
+    set_error ("nckde: robust covariances are implemented up to dimension 64");
+    return NCKDE_EUNSUPPORTED;
+  }
+
+  HIPCHK (h->drobust.ensure (sizeof (double) * nck::robust_work_doubles (type, h->d, nrows, nmats)));
+  HIPCHK (nck::launch_robust_cov (type, h->dX.as<double> (), h->d, nrows, rows, nrows, nmats, dcov,
+                                  h->drobust.as<double> (), h->stream));
+
+  return NCKDE_OK;
+}
+
 // KDE prepare_kernel (ncm_stats_dist_kde.c:378-490)
 static int
 kde_prepare_kernel (nckde *h)
@@ -270,9 +294,13 @@ kde_prepare_kernel (nckde *h)
   HIPCHK (h->dlng.ensure (sizeof (double)));
   HIPCHK (hipMemcpyAsync (h->dX.p, h->X.data (), sizeof (double) * h->n_obs * d, hipMemcpyHostToDevice, s));
 
-  HIPCHK (nck::launch_welford_cov (h->dX.as<double> (), d, (int) h->n_k, nullptr, 0, 1, h->dcovg.as<double> (), s));
+  if (h->cov_type == NCKDE_COV_SAMPLE || h->cov_type == NCKDE_COV_FIXED)
+    HIPCHK (nck::launch_welford_cov (h->dX.as<double> (), d, (int) h->n_k, nullptr, 0, 1, h->dcovg.as<double> (),
+                                     s));
+  else
+    TRY (robust_cov (h, h->cov_type, (int) h->n_k, nullptr, 1, h->dcovg.as<double> ()));
 
-  if (h->cov_type == NCKDE_COV_SAMPLE)
+  if (h->cov_type != NCKDE_COV_FIXED)
   {
     TRY (factor_batch (h, h->dcovg.as<double> (), h->dUg.as<double> (), h->dlng.as<double> (), 1));
     HIPCHK (hipMemcpyAsync (&h->kernel_lnnorm, h->dlng.p, sizeof (double), hipMemcpyDeviceToHost, s));
@@ -289,11 +317,6 @@ kde_prepare_kernel (nckde *h)
     HIPCHK (hipMemcpyAsync (h->dcovg.p, h->cov_fixed.data (), sizeof (double) * d * d, hipMemcpyHostToDevice, s));
     HIPCHK (hipMemcpyAsync (h->dUg.p, h->cov_fixed_decomp.data (), sizeof (double) * d * d, hipMemcpyHostToDevice, s));
     h->kernel_lnnorm = h->cov_fixed_lnnorm;
-  }
-  else
-  {
-    set_error ("nckde: robust covariance types are not implemented");
-    return NCKDE_EUNSUPPORTED;
   }
 
   HIPCHK (nck::launch_whiten (h->dX.as<double> (), h->dUg.as<double> (), h->dY.as<double> (), (int) h->n_obs, d, s));
@@ -340,8 +363,13 @@ vkde_build_cov_array (nckde *h)
   HIPCHK (h->dcov.ensure (sizeof (double) * h->n_k * d * d));
   HIPCHK (h->dU.ensure (sizeof (double) * h->n_k * d * d));
   HIPCHK (h->dlnnorm.ensure (sizeof (double) * h->n_k));
-  HIPCHK (nck::launch_welford_cov (h->dX.as<double> (), d, (int) h->k, h->dknn.as<int> (), (int) h->k, (int) h->n_k,
-                                   h->dcov.as<double> (), s));
+  // local covariances: the sample estimate for SAMPLE and FIXED
+  // (ncm_stats_dist_vkde.c:460-473), the robust ones otherwise
+  if (h->cov_type == NCKDE_COV_SAMPLE || h->cov_type == NCKDE_COV_FIXED)
+    HIPCHK (nck::launch_welford_cov (h->dX.as<double> (), d, (int) h->k, h->dknn.as<int> (), (int) h->k,
+                                     (int) h->n_k, h->dcov.as<double> (), s));
+  else
+    TRY (robust_cov (h, h->cov_type, (int) h->k, h->dknn.as<int> (), (int) h->n_k, h->dcov.as<double> ()));
 
   int status = 0;
 
@@ -1764,6 +1792,7 @@ trim_work (nckde *h)
   h->dlu.release ();
   h->dscratch.release ();
   h->drw.release ();
+  h->drobust.release ();
 
   return NCKDE_OK;
 }

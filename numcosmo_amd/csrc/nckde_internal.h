@@ -127,6 +127,11 @@ hipError_t mixed_build (const double *X, int n, int d, int D, int DP, const doub
                         hipStream_t s);
 hipError_t mixed_refresh_b (const double *b, double bref, int n, float *coef, hipStream_t s);
 
+// ---- robust covariances (k_robust.hip) -------------------------------------
+size_t robust_work_doubles (int type, int d, int nrows, int nmats);
+hipError_t launch_robust_cov (int type, const double *X, int d, int nrows, const int *rows, int rows_stride,
+                              int nmats, double *cov, double *work, hipStream_t s);
+
 // ---- interpolation matrix & linear algebra (k_linalg.hip) ----------------
 struct IMArgs
 {

@@ -57,6 +57,13 @@ ok_set_error (const char *msg)
   snprintf (ok_errbuf, sizeof (ok_errbuf), "%s", msg);
 }
 
+static void
+ok_set_error_few (int n)
+{
+  snprintf (ok_errbuf, sizeof (ok_errbuf),
+            "ncm_stats_vec_compute_cov_robust_diag: too few points to estimate the covariance [%d].", n);
+}
+
 /* ======================================================================= */
 /* Kernel family                                                           */
 /* ======================================================================= */
@@ -1309,6 +1316,208 @@ ok_set_cov_fixed (ok_sd *sd, const double *cov)
   return 0;
 }
 
+/*
+ * Qn scale estimator of Rousseeuw & Croux (J. Am. Stat. Assoc. 88 (1993)
+ * 1273-1283; Croux & Rousseeuw, Comput. Stat. 1 (1992) 411-428), the
+ * estimator behind gsl_stats_Qn_from_sorted_data (GSL 2.8) as called in
+ * ncm_stats_vec.c:1849-1855:
+ *   Qn = c_n * (2.21914 * d_(k)),  d_(k) the k-th smallest of x_j - x_i
+ *   (i < j) over the sorted data, k = h (h - 1) / 2, h = n / 2 + 1,
+ *   c_n = .399 .994 .512 .844 .611 .857 .669 .872 for n = 2..9, else
+ *   n / (n + 1.4) (n odd) or n / (n + 3.8) (n even).
+ * Brute force: every pairwise difference, then the k-th order statistic.
+ */
+static int
+ok_dcmp (const void *a, const void *b)
+{
+  const double x = *(const double *) a, y = *(const double *) b;
+
+  return (x > y) - (x < y);
+}
+
+double
+ok_qn (int n, const double *x)
+{
+  static const double cn_small[10] = {0.0, 0.0, 0.399, 0.994, 0.512, 0.844, 0.611, 0.857, 0.669, 0.872};
+  double *v  = malloc (sizeof (double) * n);
+  const long np = (long) n * (n - 1) / 2;
+  double *dd = malloc (sizeof (double) * (np > 0 ? np : 1));
+  const long h = n / 2 + 1;
+  const long k = h * (h - 1) / 2;
+  double cn, kth;
+  long i, j, q = 0;
+
+  memcpy (v, x, sizeof (double) * n);
+  qsort (v, n, sizeof (double), ok_dcmp);
+
+  for (i = 0; i < n; i++)
+    for (j = i + 1; j < n; j++)
+      dd[q++] = v[j] - v[i];
+
+  qsort (dd, np, sizeof (double), ok_dcmp);
+  kth = dd[k - 1];
+
+  if (n <= 9)
+    cn = cn_small[n];
+  else
+    cn = (n % 2) ? (double) n / (n + 1.4) : (double) n / (n + 3.8);
+
+  free (v);
+  free (dd);
+
+  return cn * (2.21914 * kth);
+}
+
+/* ncm_stats_vec.c:1834-1875: diagonal covariance of squared Qn scales */
+int
+ok_cov_robust_diag (int d, int nrows, const double *X, const int *rows, double *cov)
+{
+  double *col = malloc (sizeof (double) * nrows);
+  int i, a;
+
+  if (nrows < 4)
+  {
+    ok_set_error_few (nrows);
+    free (col);
+    return -1;
+  }
+
+  memset (cov, 0, sizeof (double) * d * d);
+
+  for (i = 0; i < d; i++)
+  {
+    double s;
+
+    for (a = 0; a < nrows; a++)
+      col[a] = X[(size_t) (rows ? rows[a] : a) * d + i];
+
+    s              = ok_qn (nrows, col);
+    cov[i * d + i] = s * s;
+  }
+
+  free (col);
+
+  return 0;
+}
+
+/*
+ * ncm_stats_vec.c:1889-2064: Orthogonalized Gnanadesikan-Kettenring.
+ *   y = x / sigma_x (times the reciprocal, ncm_matrix_mul_col);
+ *   U_ij = (Qn(y_i + y_j)^2 - Qn(y_i - y_j)^2) / 4, U_ii = 1;
+ *   U = E^T diag E (rows of E the eigenvectors; ok_jacobi_eigen for dsyevr);
+ *   z = E y^T, sigma_z = Qn of the rows of z;
+ *   V_ij = sigma_z_i E_ij sigma_x_j;  cov = V^T V (both triangles written).
+ * cov is invariant to the sign and order of the eigenvectors.
+ */
+int
+ok_cov_robust_ogk (int d, int nrows, const double *X, const int *rows, double *cov)
+{
+  double *y    = malloc (sizeof (double) * (size_t) nrows * d);
+  double *col  = malloc (sizeof (double) * nrows);
+  double *sx   = malloc (sizeof (double) * d);
+  double *sz   = malloc (sizeof (double) * d);
+  double *U    = calloc ((size_t) d * d, sizeof (double));
+  double *E    = malloc (sizeof (double) * d * d);
+  double *eva  = malloc (sizeof (double) * d);
+  double *V    = malloc (sizeof (double) * d * d);
+  int i, j, a, l;
+
+  if (nrows < 4)
+  {
+    ok_set_error_few (nrows);
+    free (y); free (col); free (sx); free (sz); free (U); free (E); free (eva); free (V);
+    return -1;
+  }
+
+  for (i = 0; i < d; i++)
+  {
+    double inv;
+
+    for (a = 0; a < nrows; a++)
+      col[a] = X[(size_t) (rows ? rows[a] : a) * d + i];
+
+    sx[i] = ok_qn (nrows, col);
+    inv   = 1.0 / sx[i];
+
+    for (a = 0; a < nrows; a++)
+      y[(size_t) a * d + i] = col[a] * inv;
+  }
+
+  for (i = 0; i < d; i++)
+    for (j = 0; j < d; j++)
+      U[i * d + j] = (i == j) ? 1.0 : 0.0;
+
+  for (i = 0; i < d; i++)
+    for (j = i + 1; j < d; j++)
+    {
+      double sp, sm;
+
+      for (a = 0; a < nrows; a++)
+        col[a] = y[(size_t) a * d + i] + y[(size_t) a * d + j];
+
+      sp = ok_qn (nrows, col);
+
+      for (a = 0; a < nrows; a++)
+        col[a] = y[(size_t) a * d + i] - y[(size_t) a * d + j];
+
+      sm           = ok_qn (nrows, col);
+      U[i * d + j] = 0.25 * (sp * sp - sm * sm);
+    }
+
+  ok_jacobi_eigen (d, U, eva, E);
+
+  for (i = 0; i < d; i++)
+  {
+    for (a = 0; a < nrows; a++)
+    {
+      double t = 0.0;
+
+      for (l = 0; l < d; l++)
+        t = t + E[i * d + l] * y[(size_t) a * d + l];
+
+      col[a] = t;
+    }
+
+    sz[i] = ok_qn (nrows, col);
+  }
+
+  for (i = 0; i < d; i++)
+    for (j = 0; j < d; j++)
+      V[i * d + j] = sz[i] * E[i * d + j] * sx[j];
+
+  for (j = 0; j < d; j++)
+    for (l = j; l < d; l++)
+    {
+      double t = 0.0;
+
+      for (i = 0; i < d; i++)
+        t = t + V[i * d + j] * V[i * d + l];
+
+      cov[j * d + l] = t;
+      cov[l * d + j] = t;
+    }
+
+  free (y); free (col); free (sx); free (sz); free (U); free (E); free (eva); free (V);
+
+  return 0;
+}
+
+/* covariance of the rows (all, or the listed ones) for cov_type: the
+ * NcmStatsVec one-pass estimate, or one of the robust estimators */
+static int
+ok_cov_of_type (int cov_type, int d, int nrows, const double *X, const int *rows, double *cov)
+{
+  if (cov_type == OK_COV_ROBUST_DIAG)
+    return ok_cov_robust_diag (d, nrows, X, rows, cov);
+
+  if (cov_type == OK_COV_ROBUST)
+    return ok_cov_robust_ogk (d, nrows, X, rows, cov);
+
+  ok_cov_welford (d, nrows, X, rows, cov);
+
+  return 0;
+}
+
 /* ncm_stats_dist.c:1096-1101 */
 void
 ok_reset (ok_sd *sd)
@@ -1353,10 +1562,11 @@ ok_kde_prepare_kernel (ok_sd *sd)
   sd->cov        = malloc (sizeof (double) * d * d);
   sd->cov_decomp = malloc (sizeof (double) * d * d);
 
-  ok_cov_welford (d, sd->n_kernels, sd->X, NULL, sd->cov);
-
-  if (sd->cov_type == OK_COV_SAMPLE)
+  if (sd->cov_type != OK_COV_FIXED)
   {
+    if (ok_cov_of_type (sd->cov_type, d, sd->n_kernels, sd->X, NULL, sd->cov) != 0)
+      return -1;
+
     ok_cholesky_decomp_fallback (d, sd->cov, sd->cov_decomp, sd->nearPD_maxiter);
   }
   else
@@ -1410,18 +1620,40 @@ ok_vkde_build_cov_array (ok_sd *sd)
   sd->lnnorms = malloc (sizeof (double) * sd->n_kernels);
   sd->knn     = malloc (sizeof (int) * (size_t) sd->n_kernels * sd->k);
 
-  #pragma omp parallel for schedule(dynamic, 1) if (sd->use_threads)
-  for (i = 0; i < sd->n_kernels; i++)
   {
-    double *cov = malloc (sizeof (double) * d * d);
-    int *nb     = sd->knn + (size_t) i * sd->k;
-    double *U_i = sd->U_arr + (size_t) i * d * d;
+    int failed = 0;
+    /* local covariances: the sample estimate for SAMPLE and FIXED
+     * (ncm_stats_dist_vkde.c:460-473), the robust ones otherwise */
+    const int ltype = (sd->cov_type == OK_COV_FIXED) ? OK_COV_SAMPLE : sd->cov_type;
 
-    ok_knn (sd->n_obs, d, sd->invU, i, sd->k, nb, NULL);
-    ok_cov_welford (d, sd->k, sd->X, nb, cov);
-    ok_cholesky_decomp_fallback (d, cov, U_i, sd->nearPD_maxiter);
-    sd->lnnorms[i] = ok_kernel_lnnorm (sd->kernel, sd->nu, d, U_i);
-    free (cov);
+    #pragma omp parallel for schedule(dynamic, 1) if (sd->use_threads)
+    for (i = 0; i < sd->n_kernels; i++)
+    {
+      double *cov = malloc (sizeof (double) * d * d);
+      int *nb     = sd->knn + (size_t) i * sd->k;
+      double *U_i = sd->U_arr + (size_t) i * d * d;
+
+      ok_knn (sd->n_obs, d, sd->invU, i, sd->k, nb, NULL);
+
+      if (ok_cov_of_type (ltype, d, sd->k, sd->X, nb, cov) != 0)
+      {
+        #pragma omp atomic write
+        failed = 1;
+      }
+      else
+      {
+        ok_cholesky_decomp_fallback (d, cov, U_i, sd->nearPD_maxiter);
+        sd->lnnorms[i] = ok_kernel_lnnorm (sd->kernel, sd->nu, d, U_i);
+      }
+
+      free (cov);
+    }
+
+    if (failed)
+    {
+      ok_set_error_few (sd->k);
+      return -1;
+    }
   }
 
   return 0;

@@ -23,7 +23,7 @@ extern "C" {
 
 enum { OK_KDE = 0, OK_VKDE = 1 };
 enum { OK_GAUSS = 0, OK_ST = 1 };
-enum { OK_COV_SAMPLE = 0, OK_COV_FIXED = 1 };
+enum { OK_COV_SAMPLE = 0, OK_COV_FIXED = 1, OK_COV_ROBUST_DIAG = 2, OK_COV_ROBUST = 3 };
 
 typedef struct ok_sd ok_sd;
 
@@ -38,6 +38,10 @@ void   ok_kernel_sum1 (int kernel, double nu, int d, int n, const double *chi2,
 
 /* ---- dense algebra (ncm_matrix.c, ncm_lapack.c) ----------------------- */
 int    ok_cholesky_upper (int d, double *A, int lda);
+/* ---- robust scale and covariance (ncm_stats_vec.c:1834-2064) ----------- */
+double ok_qn (int n, const double *x);
+int    ok_cov_robust_diag (int d, int nrows, const double *X, const int *rows, double *cov);
+int    ok_cov_robust_ogk (int d, int nrows, const double *X, const int *rows, double *cov);
 double ok_cholesky_lndet (int d, const double *U, int lda);
 int    ok_nearPD (int d, double *A, int maxiter);
 void   ok_cholesky_decomp_fallback (int d, const double *cov, double *U, int maxiter);

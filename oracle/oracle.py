@@ -21,7 +21,7 @@ _lib = None
 
 KDE, VKDE = 0, 1
 GAUSS, ST = 0, 1
-COV_SAMPLE, COV_FIXED = 0, 1
+COV_SAMPLE, COV_FIXED, COV_ROBUST_DIAG, COV_ROBUST = 0, 1, 2, 3
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -93,6 +93,9 @@ def lib():
             "ok_get_invUsample": (None, [C.c_void_p, _dp]),
             "ok_get_knn": (None, [C.c_void_p, _ip]),
             "ok_last_error": (C.c_char_p, []),
+            "ok_qn": (C.c_double, [C.c_int, _dp]),
+            "ok_cov_robust_diag": (C.c_int, [C.c_int, C.c_int, _dp, C.c_void_p, _dp]),
+            "ok_cov_robust_ogk": (C.c_int, [C.c_int, C.c_int, _dp, C.c_void_p, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -180,6 +183,25 @@ def cov_welford(X, rows=None):
     else:
         r = np.ascontiguousarray(rows, dtype=np.int32)
         lib().ok_cov_welford(d, len(r), X, r.ctypes.data, cov)
+    return cov
+
+
+def qn(x):
+    """Qn scale estimator (ncm_stats_vec.c:1849-1855, gsl_stats_Qn_from_sorted_data)."""
+    x = _f64(x)
+    return lib().ok_qn(x.shape[0], x)
+
+
+def cov_robust(X, kind, rows=None):
+    """ROBUST_DIAG (kind 2) or OGK (kind 3) covariance of the rows of X."""
+    X = _f64(X)
+    d = X.shape[1]
+    cov = np.empty((d, d))
+    r = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    n = X.shape[0] if r is None else len(r)
+    f = lib().ok_cov_robust_diag if kind == COV_ROBUST_DIAG else lib().ok_cov_robust_ogk
+    if f(d, n, X, None if r is None else r.ctypes.data, cov) != 0:
+        raise OracleError(lib().ok_last_error().decode())
     return cov
 
 
