@@ -180,6 +180,18 @@ int nckde_get_invUsample (nckde *h, double *Y);
  * (squared distance, index) order */
 int nckde_get_knn (nckde *h, size_t *k, int32_t *idx);
 
+/* ---- serialisation ------------------------------------------------------
+ * The reference serialises an object's configuration with NcmSerialize
+ * (ncm_serialize.h; exercised by test_ncm_stats_dist.c:955-957: a
+ * serialised-then-restored object fed the same sample gives the same fit).
+ * nckde_serialize writes the configuration and the sample, and with
+ * with_state != 0 also the prepared state (factors, weights, neighbour
+ * lists), so that a served session can be checkpointed and resumed without
+ * a new prepare.  buf == NULL: only *len (the size needed) is set.  The
+ * format is native-endian binary, versioned by its 8-byte magic. */
+int nckde_serialize (nckde *h, int with_state, void *buf, size_t cap, size_t *len);
+int nckde_deserialize (nckde **out, const void *buf, size_t len);
+
 /* ---- measurement --------------------------------------------------------
  * No counterpart in the reference.  With profiling on, every eval records HIP
  * events around its main evaluation kernel (k_eval_f64, or the first pass of

@@ -59,6 +59,8 @@ SIGNATURES = {
     "nckde_get_sample": (C.c_int, [_h, _dp]),
     "nckde_get_invUsample": (C.c_int, [_h, _dp]),
     "nckde_get_knn": (C.c_int, [_h, C.POINTER(_sz), C.c_void_p]),
+    "nckde_serialize": (C.c_int, [_h, C.c_int, C.c_void_p, _sz, C.POINTER(_sz)]),
+    "nckde_deserialize": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, _sz]),
     "nckde_set_profiling": (C.c_int, [_h, C.c_int]),
     "nckde_get_kernel_ms": (C.c_int, [_h, C.POINTER(C.c_float)]),
     "nckde_apes_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint, C.c_uint, C.c_int, C.c_int, C.c_uint64]),

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -1103,6 +1104,67 @@ ensure_wcum (nckde *h)
 // ========================================================================
 // C-ABI
 // ========================================================================
+// serialisation helpers (byte writer / bounds-checked reader)
+namespace
+{
+
+struct Writer
+{
+  unsigned char *buf;
+  size_t cap, len = 0;
+
+  void
+  put (const void *src, size_t n)
+  {
+    if (buf != nullptr && len + n <= cap)
+      std::memcpy (buf + len, src, n);
+
+    len += n;
+  }
+
+  template <class T>
+  void
+  val (T v)
+  {
+    put (&v, sizeof (T));
+  }
+};
+
+struct Reader
+{
+  const unsigned char *buf;
+  size_t len, pos = 0;
+  bool ok = true;
+
+  void
+  get (void *dst, size_t n)
+  {
+    if (!ok || pos + n > len)
+    {
+      ok = false;
+      return;
+    }
+
+    std::memcpy (dst, buf + pos, n);
+    pos += n;
+  }
+
+  template <class T>
+  T
+  val ()
+  {
+    T v {};
+
+    get (&v, sizeof (T));
+
+    return v;
+  }
+};
+
+const char SER_MAGIC[8] = {'N', 'C', 'K', 'D', 'E', '0', '1', '\0'};
+
+} // namespace
+
 extern "C" {
 
 const char *
@@ -1673,6 +1735,257 @@ nckde_get_knn (nckde *h, size_t *k, int32_t *idx)
 
   if (idx)
     HIPCHK (hipMemcpy (idx, h->dknn.p, sizeof (int32_t) * h->n_k * h->k, hipMemcpyDeviceToHost));
+
+  return NCKDE_OK;
+}
+
+// ---- serialisation --------------------------------------------------------
+//
+// Layout (native byte order): "NCKDE01" + NUL; int32 kind, kernel, d, cv,
+// use_rot_href, cov_type, nearPD_maxiter, has_cov_fixed, prepared; double
+// nu, over_smooth, shrink, local_frac; [d*d cov_fixed]; uint64 n_samples and
+// the sample rows; if prepared: uint64 n_obs, n_k, k; double href,
+// kernel_lnnorm, rnorm; the whitened sample (n_obs x d), the global
+// covariance and its factor (d x d each), the weights (n_k); VKDE only: the
+// local covariances and factors (n_k x d x d each), their lnnorms (n_k) and
+// the neighbour lists (n_k x k int32).  Evaluation factors are rebuilt on
+// load.
+
+
+int
+nckde_serialize (nckde *h, int with_state, void *buf, size_t cap, size_t *len)
+{
+  TRY (check_handle (h));
+
+  if (len == nullptr)
+  {
+    set_error ("nckde_serialize: NULL length pointer");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+  const int d          = h->d;
+  const bool state     = with_state && h->prepared;
+  const size_t dd      = (size_t) d * d;
+  Writer w {(unsigned char *) buf, cap};
+
+  if (state)
+    HIPCHK (hipStreamSynchronize (h->stream));
+
+  w.put (SER_MAGIC, sizeof (SER_MAGIC));
+
+  for (int v : {h->kind, h->kernel, d, h->cv, h->use_rot_href, h->cov_type, (int) h->nearPD_maxiter,
+                (int) h->has_cov_fixed, (int) state})
+    w.val<int32_t> (v);
+
+  for (double v : {h->nu, h->over_smooth, h->shrink, h->local_frac})
+    w.val<double> (v);
+
+  if (h->has_cov_fixed)
+    w.put (h->cov_fixed.data (), sizeof (double) * dd);
+
+  w.val<uint64_t> (h->n_samples);
+  w.put (h->X.data (), sizeof (double) * h->n_samples * d);
+
+  if (state)
+  {
+    w.val<uint64_t> (h->n_obs);
+    w.val<uint64_t> (h->n_k);
+    w.val<uint64_t> (h->k);
+    w.val<double> (h->href);
+    w.val<double> (h->kernel_lnnorm);
+    w.val<double> (h->rnorm);
+
+    auto dev = [&] (const DBuf &b, size_t bytes) -> int {
+      std::vector<unsigned char> tmp (bytes);
+
+      if (bytes > 0)
+        HIPCHK (hipMemcpy (tmp.data (), b.p, bytes, hipMemcpyDeviceToHost));
+
+      w.put (tmp.data (), bytes);
+
+      return NCKDE_OK;
+    };
+
+    TRY (dev (h->dY, sizeof (double) * h->n_obs * d));
+    TRY (dev (h->dcovg, sizeof (double) * dd));
+    TRY (dev (h->dUg, sizeof (double) * dd));
+    TRY (dev (h->dw, sizeof (double) * h->n_k));
+
+    if (h->kind == NCKDE_VKDE)
+    {
+      TRY (dev (h->dcov, sizeof (double) * h->n_k * dd));
+      TRY (dev (h->dU, sizeof (double) * h->n_k * dd));
+      TRY (dev (h->dlnnorm, sizeof (double) * h->n_k));
+      TRY (dev (h->dknn, sizeof (int32_t) * h->n_k * h->k));
+    }
+  }
+
+  *len = w.len;
+
+  if (buf != nullptr && w.len > cap)
+  {
+    set_error ("nckde_serialize: buffer of %zu bytes, %zu needed", cap, w.len);
+    return NCKDE_EINVAL;
+  }
+
+  return NCKDE_OK;
+}
+
+int
+nckde_deserialize (nckde **out, const void *buf, size_t len)
+{
+  if (out == nullptr || buf == nullptr)
+  {
+    set_error ("nckde_deserialize: NULL argument");
+    return NCKDE_EINVAL;
+  }
+
+  *out = nullptr;
+
+  Reader r {(const unsigned char *) buf, len};
+  char magic[8];
+
+  r.get (magic, sizeof (magic));
+
+  if (!r.ok || std::memcmp (magic, SER_MAGIC, sizeof (magic)) != 0)
+  {
+    set_error ("nckde_deserialize: not an nckde serialisation");
+    return NCKDE_EINVAL;
+  }
+
+  const int kind = r.val<int32_t> (), kernel = r.val<int32_t> (), d = r.val<int32_t> (), cv = r.val<int32_t> ();
+  const int rot = r.val<int32_t> (), cov_type = r.val<int32_t> (), maxit = r.val<int32_t> ();
+  const int has_fixed = r.val<int32_t> (), state = r.val<int32_t> ();
+  const double nu = r.val<double> (), os = r.val<double> (), shrink = r.val<double> (), lf = r.val<double> ();
+
+  if (!r.ok || d <= 0 || d > 4096)
+  {
+    set_error ("nckde_deserialize: truncated or invalid header");
+    return NCKDE_EINVAL;
+  }
+
+  nckde *h = nullptr;
+
+  TRY (nckde_create (&h, kind, kernel, nu, (unsigned) d, cv));
+
+  std::unique_ptr<nckde, int (*) (nckde *)> guard (h, nckde_destroy);
+  const size_t dd = (size_t) d * d;
+
+  h->use_rot_href   = rot;
+  h->over_smooth    = os;
+  h->shrink         = shrink;
+  h->local_frac     = lf;
+  h->nearPD_maxiter = (unsigned) maxit;
+
+  if (has_fixed)
+  {
+    std::vector<double> cf (dd);
+
+    r.get (cf.data (), sizeof (double) * dd);
+
+    if (!r.ok)
+    {
+      set_error ("nckde_deserialize: truncated fixed covariance");
+      return NCKDE_EINVAL;
+    }
+
+    TRY (nckde_set_cov_fixed (h, cf.data ()));
+  }
+
+  TRY (nckde_set_cov_type (h, cov_type));
+
+  const uint64_t ns = r.val<uint64_t> ();
+
+  if (!r.ok || ns > (len / sizeof (double)) / (size_t) d)
+  {
+    set_error ("nckde_deserialize: truncated sample");
+    return NCKDE_EINVAL;
+  }
+
+  h->X.resize (ns * d);
+  r.get (h->X.data (), sizeof (double) * ns * d);
+  h->n_samples = ns;
+
+  if (state)
+  {
+    const uint64_t n_obs = r.val<uint64_t> (), n_k = r.val<uint64_t> (), k = r.val<uint64_t> ();
+
+    h->href          = r.val<double> ();
+    h->kernel_lnnorm = r.val<double> ();
+    h->rnorm         = r.val<double> ();
+
+    if (!r.ok || n_obs > ns || n_k > n_obs || (kind == NCKDE_VKDE && (k == 0 || k > n_obs)))
+    {
+      set_error ("nckde_deserialize: invalid prepared state");
+      return NCKDE_EINVAL;
+    }
+
+    h->n_obs = n_obs;
+    h->n_k   = n_k;
+    h->k     = k;
+    h->D     = nck::padded_dim (d);
+
+    auto dev = [&] (DBuf &b, size_t bytes) -> int {
+      std::vector<unsigned char> tmp (bytes);
+
+      r.get (tmp.data (), bytes);
+
+      if (!r.ok)
+      {
+        set_error ("nckde_deserialize: truncated prepared state");
+        return NCKDE_EINVAL;
+      }
+
+      HIPCHK (b.ensure (bytes));
+
+      if (bytes > 0)
+        HIPCHK (hipMemcpy (b.p, tmp.data (), bytes, hipMemcpyHostToDevice));
+
+      return NCKDE_OK;
+    };
+
+    HIPCHK (h->dX.ensure (sizeof (double) * n_obs * d));
+    HIPCHK (hipMemcpy (h->dX.p, h->X.data (), sizeof (double) * n_obs * d, hipMemcpyHostToDevice));
+    TRY (dev (h->dY, sizeof (double) * n_obs * d));
+    TRY (dev (h->dcovg, sizeof (double) * dd));
+    TRY (dev (h->dUg, sizeof (double) * dd));
+    TRY (dev (h->dw, sizeof (double) * n_k));
+
+    if (kind == NCKDE_VKDE)
+    {
+      TRY (dev (h->dcov, sizeof (double) * n_k * dd));
+      TRY (dev (h->dU, sizeof (double) * n_k * dd));
+      TRY (dev (h->dlnnorm, sizeof (double) * n_k));
+      TRY (dev (h->dknn, sizeof (int32_t) * n_k * k));
+    }
+
+    HIPCHK (h->dTh.ensure (sizeof (double) * n_k * h->D));
+
+    if (kind == NCKDE_VKDE)
+    {
+      HIPCHK (h->dWp.ensure (sizeof (double) * n_k * (h->D * (h->D + 1) / 2)));
+      HIPCHK (nck::launch_vkde_factors (h->dU.as<double> (), h->dX.as<double> (), (int) n_k, d, h->D,
+                                        h->dWp.as<double> (), h->dTh.as<double> (), h->stream));
+    }
+    else
+    {
+      HIPCHK (nck::launch_kde_factors (h->dY.as<double> (), (int) n_k, d, h->D, h->dTh.as<double> (), h->stream));
+    }
+
+    TRY (refresh_bvec (h));
+    HIPCHK (h->dwcum.ensure (sizeof (double) * (n_k + 1)));
+    HIPCHK (hipStreamSynchronize (h->stream));
+    h->prepared = true;
+  }
+
+  if (!r.ok)
+  {
+    set_error ("nckde_deserialize: truncated input");
+    return NCKDE_EINVAL;
+  }
+
+  *out = guard.release ();
 
   return NCKDE_OK;
 }

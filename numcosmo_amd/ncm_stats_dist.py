@@ -30,7 +30,7 @@ from ._lib import NckdeError, check, lib
 
 __all__ = [
     "StatsDistCV", "StatsDistKDECovType", "Precision", "RNG", "NckdeError",
-    "StatsDistKernelGauss", "StatsDistKernelST", "StatsDistKDE", "StatsDistVKDE",
+    "StatsDistKernelGauss", "StatsDistKernelST", "StatsDistKDE", "StatsDistVKDE", "deserialize",
 ]
 
 
@@ -321,6 +321,43 @@ class _StatsDist:
 class StatsDistKDE(_StatsDist):
     """NcmStatsDistKDE (ncm_stats_dist_kde.h:74-86)."""
     _kind = 0
+
+
+def _serialize(self, with_state: bool = True) -> bytes:
+    """Configuration and sample, plus the prepared state when with_state
+    (nckde_serialize); restore with ``deserialize``."""
+    n = C.c_size_t()
+    check(lib().nckde_serialize(self._h, int(bool(with_state)), None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value)
+    check(lib().nckde_serialize(self._h, int(bool(with_state)), buf, n.value, C.byref(n)))
+    return buf.raw[: n.value]
+
+
+_StatsDist.serialize = _serialize
+
+
+def deserialize(blob: bytes):
+    """Object restored from ``serialize`` output (a StatsDistKDE or
+    StatsDistVKDE, prepared if the blob carries the state)."""
+    import struct
+
+    kind, kernel, d, cv, rot, _ct, _mi, _hf, _st = struct.unpack_from("=9i", blob, 8)
+    nu, over_smooth, shrink, local_frac = struct.unpack_from("=4d", blob, 8 + 36)
+    h = C.c_void_p()
+    buf = C.create_string_buffer(blob, len(blob))
+    check(lib().nckde_deserialize(C.byref(h), buf, len(blob)))
+    cls = StatsDistVKDE if kind == 1 else StatsDistKDE
+    sd = cls.__new__(cls)
+    sd._h = h
+    sd.kernel = StatsDistKernelST(d, nu) if kernel == 1 else StatsDistKernelGauss(d)
+    sd.d = d
+    sd._over_smooth = over_smooth
+    sd._shrink = shrink
+    sd.precision = Precision.FP64
+    if kind == 1:
+        sd._local_frac = local_frac
+        sd._use_rot_href = bool(rot)
+    return sd
 
 
 class StatsDistVKDE(StatsDistKDE):
