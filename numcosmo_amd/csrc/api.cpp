@@ -117,7 +117,7 @@ struct nckde
   double ysc = 1.0, bref = 0.0;
   // work buffers
   DBuf dXe, dout, dpartM, dpartS, du, dz, dchi2, didx;
-  DBuf dAT, dM, dS, dbv, dbs, dx, dxt, dr, drt, dg, dP, dpart, dscal, dm2lnp, dlu, dtv, drw, drobust;
+  DBuf dAT, dM, dS, dbv, dbs, dx, dxt, dr, drt, dg, dP, dpart, dscal, dm2lnp, dlu, dtv, drw, drobust, dpd;
 };
 
 #define HIPCHK(call)                                                                 \
@@ -554,7 +554,8 @@ nnls_unconstrained (nckde *h, int n, const std::vector<int> &P, double *dxo)
   HIPCHK (nck::launch_gather_sym (h->dM.as<double> (), n, h->dP.as<int> (), np, h->dS.as<double> (),
                                   h->dbv.as<double> (), h->dbs.as<double> (), s));
   HIPCHK (hipMemsetAsync (h->dscal.p, 0, sizeof (int) * 2, s));
-  HIPCHK (nck::launch_potrf (h->dS.as<double> (), np, h->dscal.as<int> (), s));
+  HIPCHK (h->dpd.ensure (sizeof (double) * nck::potrf_diag_doubles (np)));
+  HIPCHK (nck::launch_potrf (h->dS.as<double> (), np, h->dscal.as<int> (), h->dpd.as<double> (), s));
   HIPCHK (hipMemcpyAsync (&info, h->dscal.p, sizeof (int), hipMemcpyDeviceToHost, s));
   HIPCHK (hipStreamSynchronize (s));
 
@@ -2106,6 +2107,7 @@ trim_work (nckde *h)
   h->dscratch.release ();
   h->drw.release ();
   h->drobust.release ();
+  h->dpd.release ();
 
   return NCKDE_OK;
 }

@@ -572,23 +572,19 @@ launch_gather_sym (const double *M, int n, const int *P, int np, double *S, cons
 // --------------------------------------------------------------------------
 constexpr int CH_NB = 64;
 
-__global__ void __launch_bounds__ (256)
-k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ info)
+// Factor of the 64 x 64 diagonal block, right-looking with the block in
+// registers: thread (tr, tc) of a 16 x 16 grid owns entries
+// (tr + 16 a, tc + 16 b), a, b < 4.  Per column j the owner of (j, j)
+// publishes the pivot, the owners of row j publish the scaled row, and every
+// thread updates its own entries from two LDS vectors (two barriers per
+// column).  Returns the failing pivot + 1 (LAPACK info), else 0; every
+// thread of the block returns the same value.
+__device__ inline int
+factor_diag_block (const double *__restrict__ S, int np, int kb, int nb, double (&x)[4][4], double *row,
+                   double &piv)
 {
-  // right-looking with the block in registers: thread (tr, tc) of a 16 x 16
-  // grid owns entries (tr + 16 a, tc + 16 b), a, b < 4.  Per column j the
-  // owner of (j, j) publishes the pivot, the owners of row j publish the
-  // scaled row, and every thread updates its own entries from two LDS
-  // vectors (two barriers per column, 8 LDS reads per thread).
-  __shared__ double row[CH_NB];
-  __shared__ double piv;
   const int tid = threadIdx.x;
   const int tr  = tid >> 4, tc = tid & 15;
-
-  if (*info != 0)
-    return;
-
-  double x[4][4];
 
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -600,13 +596,10 @@ k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ 
       x[a][b] = (r < nb && c < nb) ? S[(size_t) (kb + r) * np + kb + c] : 0.0;
     }
 
-  int fail = 0;
-
   for (int j = 0; j < nb; ++j)
   {
     const int ja = j >> 4, jr = j & 15;
 
-    // publish the pivot (entry (j, j) lives in thread (jr, jr), slot (ja, ja))
     if (tr == jr && tc == jr)
     {
 #pragma unroll
@@ -620,14 +613,10 @@ k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ 
     const double djj = piv;
 
     if (!(djj > 0.0))
-    {
-      fail = kb + j + 1;
-      break;
-    }
+      return kb + j + 1;
 
     const double ujj = sqrt (djj);
 
-    // row j: owners scale their entries (c >= j) and publish them
     if (tr == jr)
     {
 #pragma unroll
@@ -670,9 +659,47 @@ k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ 
         x[a][b] = fma (-ur[a], uc[b], x[a][b]);
   }
 
+  return 0;
+}
+
+// One panel step: every block factors the diagonal block (redundantly, from
+// the unfactored values, which stay in S until k_copy_diag), then solves
+// U_kk^T x = S[kb:kb+nb][c] for its 64 columns c >= kb+nb in LDS (wave 0, one
+// lane per column, four partial sums per row) and writes them back.  Block
+// 0 stores the factor in diagbuf.  Staging loads of the panel columns are
+// issued before the factorisation.
+__global__ void __launch_bounds__ (256)
+k_potrf_panel (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ info, double *__restrict__ diagbuf)
+{
+  __shared__ double U[CH_NB][CH_NB + 1];
+  __shared__ double Xs[CH_NB][64 + 1];
+  __shared__ double row[CH_NB];
+  __shared__ double piv;
+  const int tid  = threadIdx.x;
+  const int tr   = tid >> 4, tc = tid & 15;
+  const int lane = tid & 63, w = tid >> 6;
+  const int c0   = kb + nb + blockIdx.x * 64;
+  const int c    = c0 + lane;
+
+  if (*info != 0)
+    return;
+
+  double vx[16];
+
+#pragma unroll
+  for (int q = 0; q < 16; ++q)
+  {
+    const int r = w * 16 + q;
+
+    vx[q] = (r < nb && c < np) ? S[(size_t) (kb + r) * np + c] : 0.0;
+  }
+
+  double x[4][4];
+  const int fail = factor_diag_block (S, np, kb, nb, x, row, piv);
+
   if (fail)
   {
-    if (tid == 0)
+    if (blockIdx.x == 0 && tid == 0)
       *info = fail;
 
     return;
@@ -683,54 +710,24 @@ k_potrf_diag (double *__restrict__ S, int np, int kb, int nb, int *__restrict__ 
 #pragma unroll
     for (int b = 0; b < 4; ++b)
     {
-      const int r = tr + 16 * a, c = tc + 16 * b;
+      const int r = tr + 16 * a, cc = tc + 16 * b;
 
-      if (r < nb && c < nb && c >= r)
-        S[(size_t) (kb + r) * np + kb + c] = x[a][b];
-    }
-}
+      if (r < nb && cc < nb)
+      {
+        U[r][cc] = x[a][b];
 
-// row panel: for each column c >= kb+nb solve U_kk^T x = S[kb:kb+nb][c].
-// A block takes 64 columns: all four waves stage the panel block and U_kk
-// in LDS, then wave 0 runs the forward substitution, one lane per column
-// (four partial sums per row to shorten the dependency chain), and all
-// waves write the result back.
-__global__ void __launch_bounds__ (256)
-k_trsm_row (double *__restrict__ S, int np, int kb, int nb, const int *__restrict__ info)
-{
-  __shared__ double U[CH_NB][CH_NB + 1];
-  __shared__ double Xs[CH_NB][64 + 1];
-  const int tid  = threadIdx.x;
-  const int lane = tid & 63, w = tid >> 6;
-  const int c0   = kb + nb + blockIdx.x * 64;
-  const int c    = c0 + lane;
-
-  if (*info != 0)
-    return;
-
-  {
-    double vu[16], vx[16];
-
-#pragma unroll
-    for (int q = 0; q < 16; ++q)
-    {
-      const int r = w * 16 + q;
-
-      vu[q] = (r < nb && lane < nb) ? S[(size_t) (kb + r) * np + kb + lane] : (r == lane ? 1.0 : 0.0);
-      vx[q] = (r < nb && c < np) ? S[(size_t) (kb + r) * np + c] : 0.0;
+        if (blockIdx.x == 0 && cc >= r)
+          diagbuf[r * CH_NB + cc] = x[a][b];
+      }
     }
 
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
-    {
-      U[w * 16 + q][lane]  = vu[q];
-      Xs[w * 16 + q][lane] = vx[q];
-    }
-  }
+  for (int q = 0; q < 16; ++q)
+    Xs[w * 16 + q][lane] = vx[q];
 
   __syncthreads ();
 
-  if (w == 0)
+  if (w == 0 && c0 < np)
   {
     for (int r = 0; r < nb; ++r)
     {
@@ -765,6 +762,23 @@ k_trsm_row (double *__restrict__ S, int np, int kb, int nb, const int *__restric
     }
 }
 
+// the factored diagonal blocks (upper triangles) into S
+__global__ void __launch_bounds__ (256)
+k_copy_diag (double *__restrict__ S, int np, const double *__restrict__ diagbuf)
+{
+  const int kb = blockIdx.x * CH_NB;
+  const int nb = (np - kb < CH_NB) ? np - kb : CH_NB;
+  const double *D = diagbuf + (size_t) blockIdx.x * CH_NB * CH_NB;
+
+  for (int e = threadIdx.x; e < CH_NB * CH_NB; e += 256)
+  {
+    const int r = e >> 6, c = e & 63;
+
+    if (r < nb && c < nb && c >= r)
+      S[(size_t) (kb + r) * np + kb + c] = D[e];
+  }
+}
+
 // lower triangle <- transpose of the upper factor (for the backward solve)
 __global__ void
 k_mirror_lower (double *__restrict__ S, int np)
@@ -777,26 +791,31 @@ k_mirror_lower (double *__restrict__ S, int np)
 }
 
 hipError_t
-launch_potrf (double *S, int np, int *info, hipStream_t s)
+launch_potrf (double *S, int np, int *info, double *diagbuf, hipStream_t s)
 {
   for (int kb = 0; kb < np; kb += CH_NB)
   {
-    const int nb = (np - kb < CH_NB) ? np - kb : CH_NB;
+    const int nb   = (np - kb < CH_NB) ? np - kb : CH_NB;
+    const int rest = np - kb - nb;
+    const int grid = (rest > 0) ? (rest + 63) / 64 : 1;
 
-    hipLaunchKernelGGL (k_potrf_diag, dim3 (1), dim3 (256), 0, s, S, np, kb, nb, info);
+    hipLaunchKernelGGL (k_potrf_panel, dim3 (grid), dim3 (256), 0, s, S, np, kb, nb, info,
+                        diagbuf + (size_t) (kb / CH_NB) * CH_NB * CH_NB);
 
-    if (kb + nb < np)
-    {
-      const int rest = np - kb - nb;
-
-      hipLaunchKernelGGL (k_trsm_row, dim3 ((rest + 63) / 64), dim3 (256), 0, s, S, np, kb, nb, info);
+    if (rest > 0)
       launch_syrk_mfma<1> (S, np, nb, np, kb + nb, kb, S, s);
-    }
   }
 
+  hipLaunchKernelGGL (k_copy_diag, dim3 ((np + CH_NB - 1) / CH_NB), dim3 (256), 0, s, S, np, diagbuf);
   hipLaunchKernelGGL (k_mirror_lower, dim3 ((np + 255) / 256, np), dim3 (256), 0, s, S, np);
 
   return hipGetLastError ();
+}
+
+size_t
+potrf_diag_doubles (int np)
+{
+  return (size_t) ((np + CH_NB - 1) / CH_NB) * CH_NB * CH_NB;
 }
 
 // U^T y = b then U x = y with the factor of launch_potrf (upper triangle =

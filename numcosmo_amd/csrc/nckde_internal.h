@@ -151,7 +151,8 @@ hipError_t launch_residual (const double *AT, int nk, int nobs, const double *x,
                             double *r, double *partial, double *rnorm2, double *work, hipStream_t s);
 hipError_t launch_gather_sym (const double *M, int n, const int *P, int np, double *S,
                               const double *b, double *bs, hipStream_t s);
-hipError_t launch_potrf (double *S, int np, int *info, hipStream_t s);
+size_t potrf_diag_doubles (int np);
+hipError_t launch_potrf (double *S, int np, int *info, double *diagbuf, hipStream_t s);
 hipError_t launch_trsv_pair (const double *S, int np, double *b, double *tmp, hipStream_t s);
 hipError_t launch_lu_solve (double *S, int np, double *b, int *info, hipStream_t s);
 hipError_t launch_scatter (const double *sol, const int *P, int np, int n, double *x, hipStream_t s);
