@@ -1111,14 +1111,28 @@ __global__ void
 k_shrink (double *__restrict__ w, int n, double shrink, int *__restrict__ status)
 {
   __shared__ double total;
+  __shared__ double wl[4096];
+  double t = 0.0;
+
+  // serial sum in index order (thread 0) over chunks staged in LDS
+  for (int i0 = 0; i0 < n; i0 += 4096)
+  {
+    const int cnt = (n - i0 < 4096) ? n - i0 : 4096;
+
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x)
+      wl[i] = w[i0 + i];
+
+    __syncthreads ();
+
+    if (threadIdx.x == 0)
+      for (int i = 0; i < cnt; ++i)
+        t = __dadd_rn (t, wl[i]);
+
+    __syncthreads ();
+  }
 
   if (threadIdx.x == 0)
   {
-    double t = 0.0;
-
-    for (int i = 0; i < n; ++i)
-      t = __dadd_rn (t, w[i]);
-
     total = t;
 
     if (!(t > 0.0))

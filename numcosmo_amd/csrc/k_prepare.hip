@@ -34,57 +34,82 @@ pair_of (int p, int d, int &a, int &b)
 // rows in their insertion order; the per-entry recurrence needs only the
 // running means of columns a and b, which the thread tracks itself.
 // --------------------------------------------------------------------------
-__global__ void
+constexpr int WF_LDS = 4096;   // doubles of LDS per block: 4096 / d rows staged per step
+
+__global__ void __launch_bounds__ (256)
 k_welford_cov (const double *__restrict__ X, int d, int nrows, const int *__restrict__ rows,
                int rows_stride, int nmats, double *__restrict__ cov_out)
 {
+  extern __shared__ double xs[];   // (WF_LDS / d) rows x d
   const int npairs = d * (d + 1) / 2;
   const long mat   = blockIdx.y + (long) blockIdx.z * gridDim.y;
   const int p      = blockIdx.x * blockDim.x + threadIdx.x;
 
-  if (mat >= nmats || p >= npairs)
+  if (mat >= nmats)
     return;
 
-  int a, b;
+  const bool active = p < npairs;
+  int a = 0, b = 0;
 
-  pair_of (p, d, a, b);
+  if (active)
+    pair_of (p, d, a, b);
 
   const int *rw  = rows ? rows + mat * (size_t) rows_stride : nullptr;
   const double w = 1.0;
   double mean_a = 0.0, mean_b = 0.0, acc = 0.0, weight = 0.0, weight2 = 0.0;
 
-  for (int t = 0; t < nrows; ++t)
+  const int chunk = WF_LDS / d;
+
+  for (int t0 = 0; t0 < nrows; t0 += chunk)
   {
-    const size_t r         = rw ? (size_t) rw[t] : (size_t) t;
-    const double xa        = X[r * d + a];
-    const double xb        = X[r * d + b];
-    const double curweight = weight + w;
-    const double delta_a   = xa - mean_a;
-    const double R_a       = delta_a * w / curweight;
+    const int cnt = (nrows - t0 < chunk) ? nrows - t0 : chunk;
 
-    if (a == b)
+    for (int e = threadIdx.x; e < cnt * d; e += blockDim.x)
     {
-      const double dvar = weight * delta_a * R_a;
+      const int t    = e / d, col = e - t * d;
+      const size_t r = rw ? (size_t) rw[t0 + t] : (size_t) (t0 + t);
 
-      acc    = acc + dvar;
-      mean_a = mean_a + R_a;
-    }
-    else
-    {
-      const double mean_a_new = mean_a + R_a;
-      const double dC         = w * (xa - mean_a_new) * (xb - mean_b);
-      const double delta_b    = xb - mean_b;
-      const double R_b        = delta_b * w / curweight;
-
-      acc    = acc + dC;
-      mean_a = mean_a_new;
-      mean_b = mean_b + R_b;
+      xs[e] = X[r * d + col];
     }
 
-    weight  = curweight;
-    weight2 = weight2 + w * w;
+    __syncthreads ();
+
+    if (active)
+      for (int t = 0; t < cnt; ++t)
+      {
+        const double xa        = xs[t * d + a];
+        const double xb        = xs[t * d + b];
+        const double curweight = weight + w;
+        const double delta_a   = xa - mean_a;
+        const double R_a       = delta_a * w / curweight;
+
+        if (a == b)
+        {
+          const double dvar = weight * delta_a * R_a;
+
+          acc    = acc + dvar;
+          mean_a = mean_a + R_a;
+        }
+        else
+        {
+          const double mean_a_new = mean_a + R_a;
+          const double dC         = w * (xa - mean_a_new) * (xb - mean_b);
+          const double delta_b    = xb - mean_b;
+          const double R_b        = delta_b * w / curweight;
+
+          acc    = acc + dC;
+          mean_a = mean_a_new;
+          mean_b = mean_b + R_b;
+        }
+
+        weight  = curweight;
+        weight2 = weight2 + w * w;
+      }
+
+    __syncthreads ();
   }
 
+  if (active)
   {
     const double bias_wt = 1.0 / (weight - weight2 / weight);
     double *C            = cov_out + mat * (size_t) d * d;
@@ -100,12 +125,13 @@ launch_welford_cov (const double *X, int d, int nrows, const int *rows, int rows
                     double *cov, hipStream_t s)
 {
   const int npairs = d * (d + 1) / 2;
-  const int bs     = 64;
+  const int bs     = 256;
   const int gy     = nmats < 32768 ? nmats : 32768;
   const int gz     = (nmats + gy - 1) / gy;
   dim3 grid ((npairs + bs - 1) / bs, gy, gz);
 
-  hipLaunchKernelGGL (k_welford_cov, grid, dim3 (bs), 0, s, X, d, nrows, rows, rows_stride, nmats, cov);
+  hipLaunchKernelGGL (k_welford_cov, grid, dim3 (bs), sizeof (double) * (WF_LDS / d) * d, s, X, d, nrows, rows,
+                      rows_stride, nmats, cov);
 
   return hipGetLastError ();
 }

@@ -17,22 +17,41 @@ namespace nck
 __global__ void
 k_wcum (const double *__restrict__ w, int n, double *__restrict__ wcum)
 {
+  // the serial prefix sum of the reference, one thread, over chunks staged
+  // in LDS by the whole block (the loads are off the dependency chain)
+  __shared__ double wl[4096];
   __shared__ double s_scale;
+  double cum = 0.0;
 
   if (threadIdx.x == 0)
+    wcum[0] = 0.0;
+
+  for (int i0 = 0; i0 < n; i0 += 4096)
   {
-    double cum = 0.0;
+    const int cnt = (n - i0 < 4096) ? n - i0 : 4096;
 
-    wcum[0] = cum;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x)
+      wl[i] = w[i0 + i];
 
-    for (int i = 0; i < n; i++)
-    {
-      cum        += w[i];
-      wcum[i + 1] = cum;
-    }
+    __syncthreads ();
 
-    s_scale = 1.0 / cum;
+    if (threadIdx.x == 0)
+      for (int i = 0; i < cnt; i++)
+      {
+        cum   += wl[i];
+        wl[i] = cum;
+      }
+
+    __syncthreads ();
+
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x)
+      wcum[i0 + i + 1] = wl[i];
+
+    __syncthreads ();
   }
+
+  if (threadIdx.x == 0)
+    s_scale = 1.0 / cum;
 
   __syncthreads ();
 
