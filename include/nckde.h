@@ -207,9 +207,11 @@ int nckde_get_kernel_ms (nckde *h, float *ms);
  * (ncm_fit_esmcmc.c:2136-2288).  Each half-ensemble is moved with a VKDE of
  * the other half: one prepare_interp, one batched sample and one batched eval
  * of both transition densities per half.  The likelihood is a host callback
- * over a batch of m points (the reference calls NcmFit per walker).  Draws
- * come from a seeded host generator in the reference's per-walker order; the
- * stream differs from GSL's mt19937.
+ * over a batch of m points (the reference calls NcmFit per walker).  The
+ * jump uniforms, random-walk decisions and steps come from a seeded host
+ * generator in the reference's per-walker order; the variates of the KDE
+ * draws come from a counter-based device generator (Philox4x32-10) keyed by
+ * the same seed.  Neither stream is GSL's mt19937.
  */
 typedef struct nckde_apes nckde_apes;
 

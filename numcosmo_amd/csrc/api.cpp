@@ -1740,6 +1740,54 @@ nckde_get_knn (nckde *h, size_t *k, int32_t *idx)
   return NCKDE_OK;
 }
 
+} // extern "C"
+
+namespace nck
+{
+
+// Mixture draws whose variates come from the counter-based device generator
+// instead of the caller (used by the APES walker; the transform is the one
+// of nckde_sample).
+int
+sample_counter_rng (nckde *h, size_t m, uint64_t key, uint64_t ctr, double *out)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+  const int d = h->d;
+
+  if (m == 0)
+    return NCKDE_OK;
+
+  const int nu_int = (h->kernel == NCKDE_KERNEL_ST) ? (int) h->nu : 0;
+
+  if (h->kernel == NCKDE_KERNEL_ST && (double) nu_int != h->nu)
+  {
+    set_error ("nckde: device draws need an integer Student-t nu");
+    return NCKDE_EUNSUPPORTED;
+  }
+
+  TRY (ensure_wcum (h));
+  HIPCHK (h->du.ensure (sizeof (double) * m));
+  HIPCHK (h->dz.ensure (sizeof (double) * m * d));
+  HIPCHK (h->dchi2.ensure (sizeof (double) * m));
+  HIPCHK (h->dout.ensure (sizeof (double) * m * d));
+  HIPCHK (h->didx.ensure (sizeof (uint32_t) * m));
+  HIPCHK (nck::launch_draws ((int) m, d, nu_int, key, ctr, h->du.as<double> (), h->dz.as<double> (),
+                             nu_int > 0 ? h->dchi2.as<double> () : nullptr, h->stream));
+  HIPCHK (nck::launch_sample (h->dwcum.as<double> (), (int) h->n_k, h->du.as<double> (), h->dz.as<double> (),
+                              h->dchi2.as<double> (), (int) m, d, h->kind, h->kernel, h->nu, h->href,
+                              (h->kind == NCKDE_VKDE) ? h->dU.as<double> () : h->dUg.as<double> (),
+                              h->dX.as<double> (), h->dout.as<double> (), h->didx.as<uint32_t> (), h->stream));
+  HIPCHK (hipMemcpyAsync (out, h->dout.p, sizeof (double) * m * d, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK (hipStreamSynchronize (h->stream));
+
+  return NCKDE_OK;
+}
+
+} // namespace nck
+
+extern "C" {
+
 // ---- serialisation --------------------------------------------------------
 //
 // Layout (native byte order): "NCKDE01" + NUL; int32 kind, kernel, d, cv,

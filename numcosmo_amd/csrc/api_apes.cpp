@@ -62,6 +62,7 @@ struct nckde_apes
   std::vector<double> lb, ub;
   std::vector<double> theta, m2lnL;  // n x d, n
   std::mt19937_64 rng;
+  uint64_t seed = 0, draw_ctr = 0;   // counter-based device draws of the KDE proposals
   nckde *sd[2] = {nullptr, nullptr};
   uint64_t accepted = 0, proposed = 0, offboard = 0;
   double t_prepare = 0, t_sample = 0, t_eval = 0, t_like = 0;
@@ -151,13 +152,22 @@ apes_half (nckde_apes *a, unsigned k0, unsigned k1, unsigned other0, int which, 
     }
   }
 
+  // per coordinate: 0.5 ln 2 pi + ln sigma_i, and whether it is unbounded
+  std::vector<double> rw_ln0 (d, 0.0);
+  std::vector<char> rw_open (d, 0);
+
+  for (unsigned i = 0; i < d; ++i)
+  {
+    rw_ln0[i]  = 0.5 * LN2PI + std::log (rw_std[i]);
+    rw_open[i] = std::isinf (a->lb[i]) && a->lb[i] < 0 && std::isinf (a->ub[i]) && a->ub[i] > 0;
+  }
+
   a->t_prepare += now_s () - t0;
   t0 = now_s ();
 
   // ---- proposals ------------------------------------------------------
   std::uniform_real_distribution<double> U01 (0.0, 1.0);
   std::normal_distribution<double> N01 (0.0, 1.0);
-  std::chi_squared_distribution<double> CHI (kernel == NCKDE_KERNEL_ST ? nu : 1.0);
   std::vector<double> prop ((size_t) nh * d);
   std::vector<char> todo (nh, 1);
   unsigned left = nh;
@@ -171,7 +181,6 @@ apes_half (nckde_apes *a, unsigned k0, unsigned k1, unsigned other0, int which, 
     }
 
     std::vector<unsigned> kde_w;
-    std::vector<double> u, z, c2;
 
     for (unsigned j = 0; j < nh; ++j)
     {
@@ -201,22 +210,16 @@ apes_half (nckde_apes *a, unsigned k0, unsigned k1, unsigned other0, int which, 
       else
       {
         kde_w.push_back (j);
-        u.push_back (U01 (a->rng));
-
-        for (unsigned i = 0; i < d; ++i)
-          z.push_back (N01 (a->rng));
-
-        if (kernel == NCKDE_KERNEL_ST)
-          c2.push_back (CHI (a->rng));
       }
     }
 
     if (!kde_w.empty ())
     {
+      // the uniform, normals and chi-squared variates of the KDE draws come
+      // from the device's counter-based stream (one counter per batch)
       std::vector<double> out (kde_w.size () * d);
 
-      TRYA (nckde_sample (sd, u.data (), z.data (), c2.empty () ? nullptr : c2.data (), kde_w.size (), out.data (),
-                          nullptr));
+      TRYA (nck::sample_counter_rng (sd, kde_w.size (), a->seed, a->draw_ctr++, out.data ()));
 
       for (size_t q = 0; q < kde_w.size (); ++q)
         std::memcpy (prop.data () + (size_t) kde_w[q] * d, out.data () + q * d, sizeof (double) * d);
@@ -263,10 +266,11 @@ apes_half (nckde_apes *a, unsigned k0, unsigned k1, unsigned other0, int which, 
       for (unsigned i = 0; i < d; ++i)
       {
         const double sd_i = rw_std[i];
-        const double ln_s = 0.5 * LN2PI + std::log (sd_i) +
-                            log_normal_interval ((a->lb[i] - th[i]) / sd_i, (a->ub[i] - th[i]) / sd_i);
-        const double ln_c = 0.5 * LN2PI + std::log (sd_i) +
-                            log_normal_interval ((a->lb[i] - ts[i]) / sd_i, (a->ub[i] - ts[i]) / sd_i);
+        // the truncation integral is exactly 1 (log 0) for an unbounded coordinate
+        const double ln_s = rw_ln0[i] + (rw_open[i] ? 0.0 : log_normal_interval ((a->lb[i] - th[i]) / sd_i,
+                                                                                (a->ub[i] - th[i]) / sd_i));
+        const double ln_c = rw_ln0[i] + (rw_open[i] ? 0.0 : log_normal_interval ((a->lb[i] - ts[i]) / sd_i,
+                                                                                (a->ub[i] - ts[i]) / sd_i));
         const double dz   = (ts[i] - th[i]) / sd_i;
 
         rw_star += dz * dz + 2.0 * ln_s;
@@ -363,6 +367,7 @@ nckde_apes_create (nckde_apes **out, unsigned nwalkers, unsigned dim, int method
   a->lb.assign (dim, -INFINITY);
   a->ub.assign (dim, INFINITY);
   a->rng.seed (seed);
+  a->seed = seed;
 
   if (apes_kernel (a, kernel, nu) != NCKDE_OK)
   {

@@ -149,6 +149,127 @@ k_sample (const double *__restrict__ wcum, int n, const double *__restrict__ u, 
     idx[p] = (uint32_t) i;
 }
 
+// --------------------------------------------------------------------------
+// Counter-based draws for a batch of m samples (Philox4x32-10 of Salmon,
+// Moraes, Dror & Shaw, SC'11 "Parallel random numbers: as easy as 1, 2, 3").
+// Sample j uses counters (j, q, c0, c1) for q = 0, 1, ...: one 53-bit uniform
+// u, d standard normals z (Box-Muller on pairs of uniforms) and, for the
+// Student-t kernel with integer nu, chi2 = sum of nu squared normals.
+// --------------------------------------------------------------------------
+struct u32x4
+{
+  uint32_t x, y, z, w;
+};
+
+__device__ inline u32x4
+philox4x32_10 (u32x4 c, uint32_t k0, uint32_t k1)
+{
+  for (int r = 0; r < 10; ++r)
+  {
+    const uint64_t p0 = (uint64_t) 0xD2511F53u * c.x;
+    const uint64_t p1 = (uint64_t) 0xCD9E8D57u * c.z;
+    const u32x4 n     = {(uint32_t) (p1 >> 32) ^ c.y ^ k0, (uint32_t) p1, (uint32_t) (p0 >> 32) ^ c.w ^ k1,
+                         (uint32_t) p0};
+
+    c = n;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+
+  return c;
+}
+
+__device__ inline double
+u53 (uint32_t a, uint32_t b)
+{
+  return ((double) (a >> 5) * 67108864.0 + (double) (b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+struct DrawStream
+{
+  uint32_t j, q, c0, c1, k0, k1;
+  double spare;
+  bool has_spare;
+
+  __device__ u32x4
+  next ()
+  {
+    return philox4x32_10 ({j, q++, c0, c1}, k0, k1);
+  }
+
+  __device__ double
+  uniform ()
+  {
+    const u32x4 r = next ();
+
+    return u53 (r.x, r.y);
+  }
+
+  __device__ double
+  normal ()
+  {
+    if (has_spare)
+    {
+      has_spare = false;
+      return spare;
+    }
+
+    const u32x4 r  = next ();
+    const double a = 1.0 - u53 (r.x, r.y);   // (0, 1]
+    const double b = u53 (r.z, r.w);
+    const double s = sqrt (-2.0 * log (a));
+    double sn, cs;
+
+    sincospi (2.0 * b, &sn, &cs);
+    spare     = s * sn;
+    has_spare = true;
+
+    return s * cs;
+  }
+};
+
+__global__ void
+k_draws (int m, int d, int nu_int, uint64_t key, uint64_t ctr, double *__restrict__ u, double *__restrict__ z,
+         double *__restrict__ chi2)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+
+  if (j >= m)
+    return;
+
+  DrawStream st {(uint32_t) j, 0u, (uint32_t) ctr, (uint32_t) (ctr >> 32), (uint32_t) key, (uint32_t) (key >> 32),
+                 0.0, false};
+
+  u[j] = st.uniform ();
+
+  for (int i = 0; i < d; ++i)
+    z[(size_t) j * d + i] = st.normal ();
+
+  if (chi2)
+  {
+    double c = 0.0;
+
+    for (int k = 0; k < nu_int; ++k)
+    {
+      const double g = st.normal ();
+
+      c += g * g;
+    }
+
+    chi2[j] = c;
+  }
+}
+
+hipError_t
+launch_draws (int m, int d, int nu_int, uint64_t key, uint64_t ctr, double *u, double *z, double *chi2,
+              hipStream_t s)
+{
+  if (m > 0)
+    hipLaunchKernelGGL (k_draws, dim3 ((m + 127) / 128), dim3 (128), 0, s, m, d, nu_int, key, ctr, u, z, chi2);
+
+  return hipGetLastError ();
+}
+
 hipError_t
 launch_sample (const double *wcum, int n, const double *u, const double *z, const double *chi2, int m, int d,
                int kind, int kernel, double nu, double href, const double *U, const double *X, double *out,

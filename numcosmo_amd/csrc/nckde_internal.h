@@ -161,6 +161,11 @@ hipError_t launch_shrink (double *w, int n, double shrink, int *status, hipStrea
 // ---- sampling (k_sample.hip) ----------------------------------------------
 hipError_t launch_wcum (const double *w, int n, double *wcum, hipStream_t s);
 hipError_t launch_choose (const double *wcum, int n, const double *u, int m, uint32_t *idx, hipStream_t s);
+hipError_t launch_draws (int m, int d, int nu_int, uint64_t key, uint64_t ctr, double *u, double *z, double *chi2,
+                        hipStream_t s);
+// m mixture draws with device-generated variates (Philox stream (key, ctr));
+// defined in api.cpp; nu must be an integer for the Student-t kernel
+int sample_counter_rng (nckde *h, size_t m, uint64_t key, uint64_t ctr, double *out);
 hipError_t launch_sample (const double *wcum, int n, const double *u, const double *z, const double *chi2,
                           int m, int d, int kind, int kernel, double nu, double href, const double *U,
                           const double *X, double *out, uint32_t *idx, hipStream_t s);

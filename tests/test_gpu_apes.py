@@ -78,3 +78,9 @@ def test_apes_mixed_precision_stationary():
     tgt, s = _run(3, 1, 60, precision=1, seed=5)
     chain = s.chain[10:]
     _check_moments(tgt, chain[..., 1:].reshape(-1, 3), chain[..., 0].ravel(), 3)
+
+
+def test_apes_is_reproducible_for_a_seed():
+    _, s1 = _run(2, 1, 8, seed=21)
+    _, s2 = _run(2, 1, 8, seed=21)
+    np.testing.assert_array_equal(s1.chain, s2.chain)
