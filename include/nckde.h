@@ -39,7 +39,7 @@ typedef struct nckde nckde;
 enum { NCKDE_KDE = 0, NCKDE_VKDE = 1 };
 /* NcmStatsDistKernelGauss / NcmStatsDistKernelST */
 enum { NCKDE_KERNEL_GAUSS = 0, NCKDE_KERNEL_ST = 1 };
-/* NcmStatsDistCV (ncm_stats_dist.h:77-85); only CV_NONE is implemented */
+/* NcmStatsDistCV (ncm_stats_dist.h:77-85) */
 enum { NCKDE_CV_NONE = 0, NCKDE_CV_SPLIT = 1, NCKDE_CV_SPLIT_NOFIT = 2, NCKDE_CV_LOO = 3 };
 /* NcmStatsDistKDECovType (ncm_stats_dist_kde.h:64-72): sample, fixed, robust diagonal (Qn scales) and
  * robust OGK covariances (ncm_stats_vec.c:1834-2064) */
@@ -91,6 +91,20 @@ int nckde_set_cov_fixed (nckde *h, const double *cov);
 int nckde_set_nearPD_maxiter (nckde *h, unsigned maxiter);
 /* ncm_stats_dist_set_use_threads (ncm_stats_dist.h:115): accepted, no effect */
 int nckde_set_use_threads (nckde *h, int use_threads);
+/* ncm_stats_dist_get_over_smooth (ncm_stats_dist.h:101); the CV_SPLIT,
+ * CV_SPLIT_NOFIT and CV_LOO fits change it */
+int nckde_get_over_smooth (nckde *h, double *over_smooth);
+/* ncm_stats_dist_set_split_frac / get (ncm_stats_dist.h:103-104); the
+ * reference asserts 0.01 <= split_frac <= 1, this returns NCKDE_EINVAL */
+int nckde_set_split_frac (nckde *h, double split_frac);
+int nckde_get_split_frac (nckde *h, double *split_frac);
+/* ncm_stats_dist_set_print_fit / get (ncm_stats_dist.h:109-110): one stdout
+ * line per objective evaluation of the cross-validation fits */
+int nckde_set_print_fit (nckde *h, int print_fit);
+int nckde_get_print_fit (nckde *h, int *print_fit);
+/* ncm_stats_dist_set_cv_type / get (ncm_stats_dist.h:112-113) */
+int nckde_set_cv_type (nckde *h, int cv_type);
+int nckde_get_cv_type (nckde *h, int *cv_type);
 
 /* ---- sample ------------------------------------------------------------ */
 

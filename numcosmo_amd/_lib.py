@@ -34,6 +34,13 @@ SIGNATURES = {
     "nckde_set_cov_fixed": (C.c_int, [_h, _dp]),
     "nckde_set_nearPD_maxiter": (C.c_int, [_h, C.c_uint]),
     "nckde_set_use_threads": (C.c_int, [_h, C.c_int]),
+    "nckde_get_over_smooth": (C.c_int, [_h, C.POINTER(C.c_double)]),
+    "nckde_set_split_frac": (C.c_int, [_h, C.c_double]),
+    "nckde_get_split_frac": (C.c_int, [_h, C.POINTER(C.c_double)]),
+    "nckde_set_print_fit": (C.c_int, [_h, C.c_int]),
+    "nckde_get_print_fit": (C.c_int, [_h, C.POINTER(C.c_int)]),
+    "nckde_set_cv_type": (C.c_int, [_h, C.c_int]),
+    "nckde_get_cv_type": (C.c_int, [_h, C.POINTER(C.c_int)]),
     "nckde_add_obs": (C.c_int, [_h, _dp]),
     "nckde_add_obs_batch": (C.c_int, [_h, _dp, _sz, _sz]),
     "nckde_reset": (C.c_int, [_h]),

@@ -84,6 +84,9 @@ struct nckde
   double over_smooth      = 1.0;
   double shrink           = 0.01;
   double local_frac       = 0.05;
+  double split_frac       = 0.5;     // "split-frac" property default (ncm_stats_dist.c:430-434)
+  bool print_fit          = false;
+  uint64_t cv_draws       = 0;       // normals used so far by the split fit's restarts
   int use_rot_href        = 0;
   int cov_type            = NCKDE_COV_SAMPLE;
   unsigned nearPD_maxiter = 200;
@@ -118,6 +121,8 @@ struct nckde
   // work buffers
   DBuf dXe, dout, dpartM, dpartS, du, dz, dchi2, didx;
   DBuf dAT, dM, dS, dbv, dbs, dx, dxt, dr, drt, dg, dP, dpart, dscal, dm2lnp, dlu, dtv, drw, drobust, dpd;
+  // cross-validation work buffers
+  DBuf dcv1, dcv2, dorder;
 };
 
 #define HIPCHK(call)                                                                 \
@@ -398,7 +403,10 @@ refresh_bvec (nckde *h)
   return NCKDE_OK;
 }
 
-// _ncm_stats_dist_prepare (ncm_stats_dist.c:703-789), CV_NONE
+static int cv_prepare_minimize (nckde *h);
+static int cv_split_fit (nckde *h, const double *m2lnp, double min_m2lnp);
+
+// _ncm_stats_dist_prepare (ncm_stats_dist.c:703-789)
 static int
 do_prepare (nckde *h)
 {
@@ -406,7 +414,11 @@ do_prepare (nckde *h)
 
   h->prepared  = false;
   h->n_obs     = h->n_samples;
-  h->n_k       = h->n_samples;
+  // the split cross-validations keep the first ceil(n split_frac) points as
+  // kernels and the rest as held-out observations
+  h->n_k       = (h->cv == NCKDE_CV_SPLIT || h->cv == NCKDE_CV_SPLIT_NOFIT)
+                   ? (size_t) std::ceil (h->n_samples * h->split_frac)
+                   : h->n_samples;
 
   if (h->n_obs <= (size_t) d)
   {
@@ -470,6 +482,11 @@ do_prepare (nckde *h)
   h->rnorm       = 0.0;
   h->prepared    = true;
   h->mixed_ready = false;
+
+  // bandwidth by minimisation for CV_SPLIT_NOFIT and CV_LOO
+  // (ncm_stats_dist.c:769-788)
+  if (h->cv == NCKDE_CV_SPLIT_NOFIT || h->cv == NCKDE_CV_LOO)
+    TRY (cv_prepare_minimize (h));
 
   return NCKDE_OK;
 }
@@ -762,7 +779,7 @@ nnls_solve (nckde *h)
   return NCKDE_OK;
 }
 
-// _ncm_stats_dist_prepare_interp (ncm_stats_dist.c:878-1094), CV_NONE
+// _ncm_stats_dist_prepare_interp (ncm_stats_dist.c:878-1094)
 static int
 do_prepare_interp (nckde *h, const double *m2lnp, size_t len)
 {
@@ -786,7 +803,9 @@ do_prepare_interp (nckde *h, const double *m2lnp, size_t len)
 
   if (mx - mn > limit)
   {
-    std::vector<size_t> srt (h->n_k);
+    // every observation is sorted (gsl_sort_index over the whole m2lnp
+    // vector, ncm_stats_dist.c:911-914); the cut looks at the first n_k
+    std::vector<size_t> srt (h->n_obs);
 
     std::iota (srt.begin (), srt.end (), 0);
     std::stable_sort (srt.begin (), srt.end (), [&] (size_t a, size_t b) { return m2lnp[a] < m2lnp[b]; });
@@ -805,8 +824,13 @@ do_prepare_interp (nckde *h, const double *m2lnp, size_t len)
       // 90% of the weight inside the window, 10% outside; no shrink
       std::vector<double> w (h->n_k, 0.1 / (h->n_k - n_cut));
 
+      // with split cross-validation a point inside the window may be a
+      // held-out observation; the reference writes past its weight vector
+      // there (ncm_stats_dist.c:933-940), this build gives weights to
+      // kernels only
       for (size_t i = 0; i < n_cut; ++i)
-        w[srt[i]] = 0.9 / n_cut;
+        if (srt[i] < h->n_k)
+          w[srt[i]] = 0.9 / n_cut;
 
       HIPCHK (hipMemcpyAsync (h->dw.p, w.data (), sizeof (double) * h->n_k, hipMemcpyHostToDevice, h->stream));
       TRY (refresh_bvec (h));
@@ -839,8 +863,15 @@ do_prepare_interp (nckde *h, const double *m2lnp, size_t len)
 
   HIPCHK (h->dm2lnp.ensure (sizeof (double) * h->n_obs));
   HIPCHK (hipMemcpyAsync (h->dm2lnp.p, m2lnp, sizeof (double) * h->n_obs, hipMemcpyHostToDevice, h->stream));
-  TRY (build_IM (h, h->dm2lnp.as<double> (), mn));
-  TRY (nnls_solve (h));
+  if (h->cv == NCKDE_CV_SPLIT)
+  {
+    TRY (cv_split_fit (h, m2lnp, mn));
+  }
+  else
+  {
+    TRY (build_IM (h, h->dm2lnp.as<double> (), mn));
+    TRY (nnls_solve (h));
+  }
 
   HIPCHK (h->dstatus.ensure (sizeof (int) * 4));
   HIPCHK (hipMemsetAsync (h->dstatus.p, 0, sizeof (int) * 4, h->stream));
@@ -1102,6 +1133,340 @@ ensure_wcum (nckde *h)
   return NCKDE_OK;
 }
 
+// ---- cross-validated bandwidth (ncm_stats_dist.c:484-701, 1018-1073) ------
+//
+// The objectives run on the device (IM, NNLS, evaluation, Monte-Carlo pairs);
+// the host keeps the scalar minimisers and the serial reductions whose order
+// follows the reference's loops.
+
+// over_smooth = exp(lnos) and href = factor * href(over_smooth), as every
+// objective of the reference sets them before it evaluates
+static void
+cv_set_lnos (nckde *h, double lnos, double factor)
+{
+  h->over_smooth = std::exp (lnos);
+  h->href        = factor * get_href (h);
+  h->mixed_ready = false;
+}
+
+// n(n-1) in the reference's unsigned arithmetic (guint products)
+static double
+cv_pairs (size_t n)
+{
+  return (double) (unsigned) ((unsigned) n * ((unsigned) n - 1u));
+}
+
+// IM row sums of every observation, without and with the diagonal term
+static int
+cv_im_sums (nckde *h, std::vector<double> &off, std::vector<double> &dg)
+{
+  const size_t no = h->n_obs;
+
+  TRY (build_IM (h, nullptr, 0.0));
+  HIPCHK (h->dcv2.ensure (sizeof (double) * 2 * no));
+  HIPCHK (nck::launch_im_obs_sums (h->dAT.as<double> (), (int) h->n_k, (int) no, h->dcv2.as<double> (),
+                                   h->dcv2.as<double> () + no, h->stream));
+  off.resize (no);
+  dg.resize (no);
+  HIPCHK (hipMemcpyAsync (off.data (), h->dcv2.p, sizeof (double) * no, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK (hipMemcpyAsync (dg.data (), h->dcv2.as<double> () + no, sizeof (double) * no, hipMemcpyDeviceToHost,
+                          h->stream));
+  HIPCHK (hipStreamSynchronize (h->stream));
+
+  return NCKDE_OK;
+}
+
+// m2lnp of rows [r0, r0 + m) of the sample, FP64
+static int
+cv_eval_rows (nckde *h, size_t r0, size_t m, std::vector<double> &v)
+{
+  v.resize (m);
+
+  if (m == 0)
+    return NCKDE_OK;
+
+  HIPCHK (h->dcv1.ensure (sizeof (double) * m));
+  TRY (eval_device (h, h->dX.as<double> () + r0 * h->d, m, h->dcv1.as<double> (), NCKDE_PREC_FP64, 0, h->stream));
+  HIPCHK (hipMemcpyAsync (v.data (), h->dcv1.p, sizeof (double) * m, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK (hipStreamSynchronize (h->stream));
+
+  return NCKDE_OK;
+}
+
+// _ncm_stats_dist_m2lnp (ncm_stats_dist.c:484-510): sum of m2lnp over the
+// held-out observations, kernels at uniform weights
+static int
+cv_obj_m2lnp (nckde *h, double lnos, double &val)
+{
+  std::vector<double> v;
+
+  cv_set_lnos (h, lnos, 1.0);
+  TRY (cv_eval_rows (h, h->n_k, h->n_obs - h->n_k, v));
+
+  double m2lnp = 0.0;
+
+  for (double x : v)
+    m2lnp += x;
+
+  if (h->print_fit)
+  {
+    printf ("# over-smooth: % 22.15g, m2lnp = % 22.15g\n", h->over_smooth, m2lnp);
+    fflush (stdout);
+  }
+
+  val = m2lnp;
+
+  return NCKDE_OK;
+}
+
+// _ncm_stats_dist_amise_kde_gauss (ncm_stats_dist.c:512-559): the integral
+// of the squared density is the IM sum at sqrt(2) href, the leave-one-out
+// term the off-diagonal IM sum at href
+static int
+cv_obj_amise_kde_gauss (nckde *h, double lnos, double &val)
+{
+  const size_t n = h->n_k;
+  const double n2 = (double) n * (double) n;
+  std::vector<double> off, dg;
+  double amise = 0.0;
+
+  cv_set_lnos (h, lnos, std::sqrt (2.0));
+  TRY (cv_im_sums (h, off, dg));
+
+  for (size_t i = 0; i < n; ++i)
+    amise += (off[i] + dg[i]) / n2;
+
+  cv_set_lnos (h, lnos, 1.0);
+  TRY (cv_im_sums (h, off, dg));
+
+  for (size_t i = 0; i < n; ++i)
+    amise -= 2.0 * off[i] / cv_pairs (n);
+
+  if (h->print_fit)
+  {
+    printf ("# over-smooth: % 22.15g, amise = % 22.15g\n", h->over_smooth, amise);
+    fflush (stdout);
+  }
+
+  val = amise;
+
+  return NCKDE_OK;
+}
+
+// antithetic pairs per device batch of the Monte-Carlo integral below; pair t
+// uses draws 2t and 2t+1 (mod the batch) of Philox stream (key 0, batch)
+static constexpr int CV_MC_BATCH = 4096;
+
+// _ncm_stats_dist_amise (ncm_stats_dist.c:563-660): leave-one-out term from
+// the IM row sums; the integral of the squared density by Monte Carlo over
+// antithetic kernel pairs (kernels ordered by their leave-one-out density),
+// 100 warm-up pairs, then until the relative standard error of the mean
+// density falls below 1e-2.  Every call restarts the generator from the same
+// seed, as the reference does.
+static int
+cv_obj_amise (nckde *h, double lnos, double &val)
+{
+  const size_t n = h->n_k;
+  const int d    = h->d;
+  std::vector<double> off, dg;
+  double amise = 0.0;
+
+  cv_set_lnos (h, lnos, 1.0);
+  TRY (cv_im_sums (h, off, dg));
+
+  std::vector<double> lp (n);
+
+  for (size_t i = 0; i < n; ++i)
+  {
+    amise -= 2.0 * off[i] / cv_pairs (n);
+    lp[i] = (off[i] + dg[i]) / n;
+  }
+
+  std::vector<uint32_t> order (n);
+
+  std::iota (order.begin (), order.end (), 0u);
+  std::stable_sort (order.begin (), order.end (), [&] (uint32_t a, uint32_t b) { return lp[a] < lp[b]; });
+
+  const int nu_int = (h->kernel == NCKDE_KERNEL_ST) ? (int) h->nu : 0;
+
+  if (h->kernel == NCKDE_KERNEL_ST && (double) nu_int != h->nu)
+  {
+    set_error ("nckde: the leave-one-out Monte-Carlo integral needs an integer Student-t nu");
+    return NCKDE_EUNSUPPORTED;
+  }
+
+  const int m2 = 2 * CV_MC_BATCH;
+
+  HIPCHK (h->dorder.ensure (sizeof (uint32_t) * n));
+  HIPCHK (hipMemcpyAsync (h->dorder.p, order.data (), sizeof (uint32_t) * n, hipMemcpyHostToDevice, h->stream));
+  TRY (ensure_wcum (h));
+  HIPCHK (h->du.ensure (sizeof (double) * m2));
+  HIPCHK (h->dz.ensure (sizeof (double) * m2 * d));
+  HIPCHK (h->dchi2.ensure (sizeof (double) * m2));
+  HIPCHK (h->didx.ensure (sizeof (uint32_t) * m2));
+  HIPCHK (h->dcv2.ensure (sizeof (double) * (size_t) m2 * d));
+  HIPCHK (h->dcv1.ensure (sizeof (double) * m2));
+
+  const uint64_t max_iter = 100000000;
+  std::vector<double> p (m2);
+  nck::PairStats st;
+  uint64_t count = 0;
+  double mean    = 0.0;
+  bool done      = false;
+
+  for (uint64_t b = 0; !done; ++b)
+  {
+    HIPCHK (nck::launch_draws (m2, d, nu_int, 0, b, h->du.as<double> (), h->dz.as<double> (),
+                               nu_int > 0 ? h->dchi2.as<double> () : nullptr, h->stream));
+    HIPCHK (nck::launch_pair_index (h->dwcum.as<double> (), (int) n, h->du.as<double> (), CV_MC_BATCH,
+                                    h->dorder.as<uint32_t> (), h->didx.as<uint32_t> (), h->stream));
+    HIPCHK (nck::launch_sample_at (h->didx.as<uint32_t> (), h->dz.as<double> (), h->dchi2.as<double> (), m2, d,
+                                   h->kind, h->kernel, h->nu, h->href,
+                                   (h->kind == NCKDE_VKDE) ? h->dU.as<double> () : h->dUg.as<double> (),
+                                   h->dX.as<double> (), h->dcv2.as<double> (), h->stream));
+    TRY (eval_device (h, h->dcv2.as<double> (), m2, h->dcv1.as<double> (), NCKDE_PREC_FP64, 1, h->stream));
+    HIPCHK (hipMemcpyAsync (p.data (), h->dcv1.p, sizeof (double) * m2, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK (hipStreamSynchronize (h->stream));
+
+    for (int t = 0; t < CV_MC_BATCH && !done; ++t)
+    {
+      nck::pair_stats_update (st, p[2 * t], p[2 * t + 1]);
+      count++;
+
+      if (count <= 100)
+        continue;
+
+      const uint64_t i = count - 101;
+
+      mean = 0.5 * (st.mean[0] + st.mean[1]);
+
+      const double var = 0.25 * (st.var[0] * st.bias + st.var[1] * st.bias + 2.0 * (st.cov01 * st.bias));
+      const double msd = std::sqrt (var / (i + 101.0)) / mean;
+
+      if (msd < 1.0e-2 || i + 1 >= max_iter)
+        done = true;
+    }
+  }
+
+  amise += mean;
+
+  if (h->print_fit)
+  {
+    printf ("# over-smooth: % 22.15g, amise = % 22.15g\n", h->over_smooth, amise);
+    fflush (stdout);
+  }
+
+  val = amise;
+
+  return NCKDE_OK;
+}
+
+// _ncm_stats_dist_minimize_obj (ncm_stats_dist.c:664-701): Nelder-Mead over
+// ln(over_smooth) from the current value with step 0.1, until the simplex
+// size is below 1e-3 or 1000 iterations.  over_smooth and href keep the
+// values of the last evaluation, as in the reference.
+static int
+cv_prepare_minimize (nckde *h)
+{
+  int (*obj) (nckde *, double, double &) = cv_obj_m2lnp;
+
+  if (h->cv == NCKDE_CV_LOO)
+    obj = (h->kind == NCKDE_KDE && h->kernel == NCKDE_KERNEL_GAUSS) ? cv_obj_amise_kde_gauss : cv_obj_amise;
+
+  std::vector<double> x {std::log (h->over_smooth)}, step {0.1};
+  nck::NMInfo info {};
+  nck::ObjFn f = [&] (const double *v, double &val) { return obj (h, v[0], val); };
+
+  TRY (nck::nm_minimize (f, x, step, 1.0e-3, 1000, &info));
+
+  if (h->print_fit)
+  {
+    printf ("# iter: %d, over-smooth: % 22.15g, m2lnp = % 22.15g, gsl status (%d)\n", info.iter, h->over_smooth,
+            info.fval, info.status == 0 ? 0 : (info.status == 1 ? -2 : 5));
+    fflush (stdout);
+  }
+
+  return NCKDE_OK;
+}
+
+// IM scaled by 1/f at over_smooth = exp(lnos) and its NNLS fit (weights in
+// dw, unnormalised): _ncm_stats_dist_compute_IM_full + NCM_NNLS_SOLVE
+static int
+cv_fit_at (nckde *h, double lnos, double min_m2lnp, double &rnorm)
+{
+  cv_set_lnos (h, lnos, 1.0);
+  TRY (build_IM (h, h->dm2lnp.as<double> (), min_m2lnp));
+  TRY (nnls_solve (h));
+  rnorm = h->rnorm;
+
+  return NCKDE_OK;
+}
+
+// CV_SPLIT branch of _ncm_stats_dist_prepare_interp (ncm_stats_dist.c:1018-1073):
+// ten Gaussian restarts of ln(over_smooth) (sigma 0.5) keep the smallest NNLS
+// residual; then levmar fits ln(over_smooth) to the residuals
+// expm1(-(m2lnp_interp_i - (m2lnp_i - min))/2) over every observation; the
+// final NNLS fit stays in the weights.  The restarts draw from a per-handle
+// counter-based normal stream (key 0), where the reference uses its own
+// mt19937 stream seeded with 0.
+static int
+cv_split_fit (nckde *h, const double *m2lnp, double min_m2lnp)
+{
+  const size_t no = h->n_obs;
+  double ln_os    = std::log (h->over_smooth);
+  double rnorm0, rnorm_try;
+
+  TRY (cv_fit_at (h, ln_os, min_m2lnp, rnorm0));
+
+  for (int i = 0; i < 10; ++i)
+  {
+    const double ln_os_try = ln_os + 0.5 * nck::host_normal (0, h->cv_draws++);
+
+    TRY (cv_fit_at (h, ln_os_try, min_m2lnp, rnorm_try));
+
+    if (rnorm_try < rnorm0)
+    {
+      ln_os  = ln_os_try;
+      rnorm0 = rnorm_try;
+    }
+  }
+
+  std::vector<double> v;
+  nck::ResFn f = [&] (double lp, double *hx) -> int {
+    double rn;
+
+    TRY (cv_fit_at (h, lp, min_m2lnp, rn));
+    TRY (refresh_bvec (h));
+    TRY (cv_eval_rows (h, 0, no, v));
+
+    double fn2 = 0.0;
+
+    for (size_t i = 0; i < no; ++i)
+    {
+      hx[i] = std::expm1 (-0.5 * (v[i] - (m2lnp[i] - min_m2lnp)));
+      fn2 += hx[i] * hx[i];
+    }
+
+    if (h->print_fit)
+    {
+      printf ("# over-smooth: % 22.15g, rnorm = % 22.15g, fnorm = % 22.15g\n", h->over_smooth, rn, std::sqrt (fn2));
+      fflush (stdout);
+    }
+
+    return NCKDE_OK;
+  };
+  const double opts[5] = {1.0e-3, 1.0e-7, 1.0e-7, 1.0e-10, 1.0e-6};
+  nck::LMInfo info {};
+
+  TRY (nck::lm_dif1 (f, ln_os, (int) no, 10000, opts, &info));
+
+  double rn;
+
+  TRY (cv_fit_at (h, ln_os, min_m2lnp, rn));
+
+  return NCKDE_OK;
+}
+
 // ========================================================================
 // C-ABI
 // ========================================================================
@@ -1162,7 +1527,7 @@ struct Reader
   }
 };
 
-const char SER_MAGIC[8] = {'N', 'C', 'K', 'D', 'E', '0', '1', '\0'};
+const char SER_MAGIC[8] = {'N', 'C', 'K', 'D', 'E', '0', '2', '\0'};
 
 } // namespace
 
@@ -1198,10 +1563,10 @@ nckde_create (nckde **out, int kind, int kernel, double nu, unsigned dim, int cv
     return NCKDE_EINVAL;
   }
 
-  if (cv_type != NCKDE_CV_NONE)
+  if (cv_type < NCKDE_CV_NONE || cv_type > NCKDE_CV_LOO)
   {
-    set_error ("nckde_create: only NCM_STATS_DIST_CV_NONE is implemented");
-    return NCKDE_EUNSUPPORTED;
+    set_error ("nckde_create: invalid cross-validation type %d", cv_type);
+    return NCKDE_EINVAL;
   }
 
   int ndev = 0;
@@ -1281,6 +1646,111 @@ nckde_set_shrink (nckde *h, double v)
   }
 
   h->shrink = v;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_over_smooth (nckde *h, double *v)
+{
+  TRY (check_handle (h));
+
+  if (v == nullptr)
+  {
+    set_error ("nckde_get_over_smooth: NULL output pointer");
+    return NCKDE_EINVAL;
+  }
+
+  *v = h->over_smooth;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_split_frac (nckde *h, double v)
+{
+  TRY (check_handle (h));
+
+  if (!(v >= 0.01 && v <= 1.0))
+  {
+    set_error ("nckde_set_split_frac: split_frac must be in [0.01, 1]");
+    return NCKDE_EINVAL;
+  }
+
+  h->split_frac = v;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_split_frac (nckde *h, double *v)
+{
+  TRY (check_handle (h));
+
+  if (v == nullptr)
+  {
+    set_error ("nckde_get_split_frac: NULL output pointer");
+    return NCKDE_EINVAL;
+  }
+
+  *v = h->split_frac;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_print_fit (nckde *h, int v)
+{
+  TRY (check_handle (h));
+  h->print_fit = v != 0;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_print_fit (nckde *h, int *v)
+{
+  TRY (check_handle (h));
+
+  if (v == nullptr)
+  {
+    set_error ("nckde_get_print_fit: NULL output pointer");
+    return NCKDE_EINVAL;
+  }
+
+  *v = h->print_fit ? 1 : 0;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_cv_type (nckde *h, int v)
+{
+  TRY (check_handle (h));
+
+  if (v < NCKDE_CV_NONE || v > NCKDE_CV_LOO)
+  {
+    set_error ("nckde_set_cv_type: invalid cross-validation type %d", v);
+    return NCKDE_EINVAL;
+  }
+
+  h->cv = v;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_cv_type (nckde *h, int *v)
+{
+  TRY (check_handle (h));
+
+  if (v == nullptr)
+  {
+    set_error ("nckde_get_cv_type: NULL output pointer");
+    return NCKDE_EINVAL;
+  }
+
+  *v = h->cv;
 
   return NCKDE_OK;
 }
@@ -1790,9 +2260,10 @@ extern "C" {
 
 // ---- serialisation --------------------------------------------------------
 //
-// Layout (native byte order): "NCKDE01" + NUL; int32 kind, kernel, d, cv,
-// use_rot_href, cov_type, nearPD_maxiter, has_cov_fixed, prepared; double
-// nu, over_smooth, shrink, local_frac; [d*d cov_fixed]; uint64 n_samples and
+// Layout (native byte order): "NCKDE02" + NUL; int32 kind, kernel, d, cv,
+// use_rot_href, cov_type, nearPD_maxiter, has_cov_fixed, prepared,
+// print_fit; double nu, over_smooth, shrink, local_frac, split_frac; uint64
+// cv_draws; [d*d cov_fixed]; uint64 n_samples and
 // the sample rows; if prepared: uint64 n_obs, n_k, k; double href,
 // kernel_lnnorm, rnorm; the whitened sample (n_obs x d), the global
 // covariance and its factor (d x d each), the weights (n_k); VKDE only: the
@@ -1824,11 +2295,13 @@ nckde_serialize (nckde *h, int with_state, void *buf, size_t cap, size_t *len)
   w.put (SER_MAGIC, sizeof (SER_MAGIC));
 
   for (int v : {h->kind, h->kernel, d, h->cv, h->use_rot_href, h->cov_type, (int) h->nearPD_maxiter,
-                (int) h->has_cov_fixed, (int) state})
+                (int) h->has_cov_fixed, (int) state, (int) h->print_fit})
     w.val<int32_t> (v);
 
-  for (double v : {h->nu, h->over_smooth, h->shrink, h->local_frac})
+  for (double v : {h->nu, h->over_smooth, h->shrink, h->local_frac, h->split_frac})
     w.val<double> (v);
+
+  w.val<uint64_t> (h->cv_draws);
 
   if (h->has_cov_fixed)
     w.put (h->cov_fixed.data (), sizeof (double) * dd);
@@ -1905,8 +2378,10 @@ nckde_deserialize (nckde **out, const void *buf, size_t len)
 
   const int kind = r.val<int32_t> (), kernel = r.val<int32_t> (), d = r.val<int32_t> (), cv = r.val<int32_t> ();
   const int rot = r.val<int32_t> (), cov_type = r.val<int32_t> (), maxit = r.val<int32_t> ();
-  const int has_fixed = r.val<int32_t> (), state = r.val<int32_t> ();
+  const int has_fixed = r.val<int32_t> (), state = r.val<int32_t> (), pfit = r.val<int32_t> ();
   const double nu = r.val<double> (), os = r.val<double> (), shrink = r.val<double> (), lf = r.val<double> ();
+  const double sf = r.val<double> ();
+  const uint64_t cv_draws = r.val<uint64_t> ();
 
   if (!r.ok || d <= 0 || d > 4096)
   {
@@ -1925,6 +2400,9 @@ nckde_deserialize (nckde **out, const void *buf, size_t len)
   h->over_smooth    = os;
   h->shrink         = shrink;
   h->local_frac     = lf;
+  h->split_frac     = sf;
+  h->print_fit      = pfit != 0;
+  h->cv_draws       = cv_draws;
   h->nearPD_maxiter = (unsigned) maxit;
 
   if (has_fixed)

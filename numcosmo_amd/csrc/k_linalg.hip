@@ -410,6 +410,50 @@ launch_gram_nt (const double *AT, int n, int K, double *M, hipStream_t s)
   return hipGetLastError ();
 }
 
+// Row sums of IM for the leave-one-out criteria (ncm_stats_dist.c:512-600):
+// thread j walks the kernels i = 0..nk-1 of observation j in ascending order,
+// the order of the reference's row loops, skipping i == j; the loads go out
+// in batches of 16 so that they overlap.
+__global__ void
+k_im_obs_sums (const double *__restrict__ AT, int nk, int nobs, double *__restrict__ off, double *__restrict__ diag)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+
+  if (j >= nobs)
+    return;
+
+  double s = 0.0;
+  int i    = 0;
+
+  for (; i + 16 <= nk; i += 16)
+  {
+    double v[16];
+
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      v[q] = AT[(size_t) (i + q) * nobs + j];
+
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      s += (i + q != j) ? v[q] : 0.0;
+  }
+
+  for (; i < nk; ++i)
+    s += (i != j) ? AT[(size_t) i * nobs + j] : 0.0;
+
+  off[j]  = s;
+  diag[j] = (j < nk) ? AT[(size_t) j * nobs + j] : 0.0;
+}
+
+hipError_t
+launch_im_obs_sums (const double *AT, int nk, int nobs, double *off, double *diag, hipStream_t s)
+{
+  if (nobs > 0)
+    hipLaunchKernelGGL (k_im_obs_sums, dim3 ((nobs + 127) / 128), dim3 (128), 0, s, AT, nk, nobs, off, diag);
+
+  return hipGetLastError ();
+}
+
 // out[a] = sum_j AT[a][j] v[j] (v == NULL: ones).  One wave per row.
 __global__ void
 k_row_dot (const double *__restrict__ AT, int n, int K, const double *__restrict__ v, double *__restrict__ out)

@@ -108,14 +108,14 @@ __global__ void
 k_sample (const double *__restrict__ wcum, int n, const double *__restrict__ u, const double *__restrict__ z,
           const double *__restrict__ chi2, int m, int d, int kind, int kernel, double nu, double href,
           const double *__restrict__ Uall, const double *__restrict__ X, double *__restrict__ out,
-          uint32_t *__restrict__ idx)
+          uint32_t *__restrict__ idx, const uint32_t *__restrict__ idx_in)
 {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
 
   if (p >= m)
     return;
 
-  const int i      = choose (wcum, n, u[p]);
+  const int i      = idx_in ? (int) idx_in[p] : choose (wcum, n, u[p]);
   const double *U  = (kind == NCKDE_VKDE) ? Uall + (size_t) i * d * d : Uall;
   const double *mu = X + (size_t) i * d;
   const double *zp = z + (size_t) p * d;
@@ -276,7 +276,48 @@ launch_sample (const double *wcum, int n, const double *u, const double *z, cons
                uint32_t *idx, hipStream_t s)
 {
   hipLaunchKernelGGL (k_sample, dim3 ((m + 127) / 128), dim3 (128), 0, s, wcum, n, u, z, chi2, m, d, kind, kernel,
-                      nu, href, U, X, out, idx);
+                      nu, href, U, X, out, idx, (const uint32_t *) nullptr);
+
+  return hipGetLastError ();
+}
+
+hipError_t
+launch_sample_at (const uint32_t *idx_in, const double *z, const double *chi2, int m, int d, int kind, int kernel,
+                  double nu, double href, const double *U, const double *X, double *out, hipStream_t s)
+{
+  if (m > 0)
+    hipLaunchKernelGGL (k_sample, dim3 ((m + 127) / 128), dim3 (128), 0, s, (const double *) nullptr, 0,
+                        (const double *) nullptr, z, chi2, m, d, kind, kernel, nu, href, U, X, out,
+                        (uint32_t *) nullptr, idx_in);
+
+  return hipGetLastError ();
+}
+
+// Antithetic kernel pairs of ncm_stats_dist_sample2 (ncm_stats_dist.c:1629-1651):
+// the first draw of pair t chooses i from its uniform; the pair samples the
+// kernels order[i] and order[n-1-i].
+__global__ void
+k_pair_index (const double *__restrict__ wcum, int n, const double *__restrict__ u, int npairs,
+              const uint32_t *__restrict__ order, uint32_t *__restrict__ idx)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+
+  if (t >= npairs)
+    return;
+
+  const int i = choose (wcum, n, u[2 * t]);
+
+  idx[2 * t]     = order[i];
+  idx[2 * t + 1] = order[n - 1 - i];
+}
+
+hipError_t
+launch_pair_index (const double *wcum, int n, const double *u, int npairs, const uint32_t *order, uint32_t *idx,
+                   hipStream_t s)
+{
+  if (npairs > 0)
+    hipLaunchKernelGGL (k_pair_index, dim3 ((npairs + 255) / 256), dim3 (256), 0, s, wcum, n, u, npairs, order,
+                        idx);
 
   return hipGetLastError ();
 }

@@ -16,6 +16,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
+#include <vector>
+
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -169,5 +172,59 @@ int sample_counter_rng (nckde *h, size_t m, uint64_t key, uint64_t ctr, double *
 hipError_t launch_sample (const double *wcum, int n, const double *u, const double *z, const double *chi2,
                           int m, int d, int kind, int kernel, double nu, double href, const double *U,
                           const double *X, double *out, uint32_t *idx, hipStream_t s);
+// as launch_sample, with the kernel of draw p given by idx_in[p]
+hipError_t launch_sample_at (const uint32_t *idx_in, const double *z, const double *chi2, int m, int d, int kind,
+                             int kernel, double nu, double href, const double *U, const double *X, double *out,
+                             hipStream_t s);
+// antithetic pairs: draw 2t picks i = choose(u[2t]); idx[2t] = order[i],
+// idx[2t+1] = order[n-1-i]
+hipError_t launch_pair_index (const double *wcum, int n, const double *u, int npairs, const uint32_t *order,
+                              uint32_t *idx, hipStream_t s);
+// per observation j of IM^T (n_k x n_obs): off[j] = sum over kernels i != j
+// in ascending i, diag[j] = IM^T[j][j] (0 for j >= n_k)
+hipError_t launch_im_obs_sums (const double *AT, int nk, int nobs, double *off, double *diag, hipStream_t s);
+
+// ---- cross-validation minimisers (cv_min.cpp) -----------------------------
+#define TRY_OK(call)                                                                 \
+  do {                                                                               \
+    int r_ok_ = (call);                                                              \
+    if (r_ok_ != NCKDE_OK)                                                           \
+      return r_ok_;                                                                  \
+  } while (0)
+
+using ObjFn = std::function<int (const double *x, double &val)>;
+using ResFn = std::function<int (double p, double *hx)>;
+
+struct NMInfo
+{
+  int iter, status;    // status: 0 size below tolerance, 1 iteration cap, -1 failed contraction
+  double fval, size;
+};
+// Nelder-Mead (GSL nmsimplex2) from x with initial steps step, until the
+// simplex size falls below epsabs or maxiter iterations
+int nm_minimize (const ObjFn &f, std::vector<double> &x, const std::vector<double> &step, double epsabs,
+                 int maxiter, NMInfo *info);
+
+struct LMInfo
+{
+  double init_eL2, eL2;
+  int iter, stop, nfev, njac;
+};
+// levmar's forward-difference Levenberg-Marquardt for one parameter p and
+// n residuals hx (zero measurement vector); opts = {mu, eps1, eps2, eps3, delta}
+int lm_dif1 (const ResFn &func, double &p, int n, int itmax, const double opts[5], LMInfo *info);
+
+struct PairStats
+{
+  double W = 0.0, W2 = 0.0, bias = 0.0;
+  double mean[2] = {0.0, 0.0}, var[2] = {0.0, 0.0}, cov01 = 0.0;
+};
+void pair_stats_update (PairStats &st, double x0, double x1);
+
+// third counter word of the host normal stream (separates it from the
+// sampling streams that share a key)
+constexpr uint32_t HOST_NORMAL_TAG = 0x43565350u;
+// standard normal number ctr of the stream with this key (Box-Muller, cosine branch)
+double host_normal (uint64_t key, uint64_t ctr);
 
 } // namespace nck

@@ -142,7 +142,34 @@ class _StatsDist:
         self._over_smooth = float(v)
 
     def get_over_smooth(self):
-        return self._over_smooth
+        v = C.c_double()
+        check(lib().nckde_get_over_smooth(self._h, C.byref(v)))
+        self._over_smooth = v.value
+        return v.value
+
+    def set_split_frac(self, v):
+        check(lib().nckde_set_split_frac(self._h, float(v)))
+
+    def get_split_frac(self):
+        v = C.c_double()
+        check(lib().nckde_get_split_frac(self._h, C.byref(v)))
+        return v.value
+
+    def set_print_fit(self, v):
+        check(lib().nckde_set_print_fit(self._h, int(bool(v))))
+
+    def get_print_fit(self):
+        v = C.c_int()
+        check(lib().nckde_get_print_fit(self._h, C.byref(v)))
+        return bool(v.value)
+
+    def set_cv_type(self, cv_type):
+        check(lib().nckde_set_cv_type(self._h, int(cv_type)))
+
+    def get_cv_type(self):
+        v = C.c_int()
+        check(lib().nckde_get_cv_type(self._h, C.byref(v)))
+        return StatsDistCV(v.value)
 
     def set_shrink(self, v):
         check(lib().nckde_set_shrink(self._h, float(v)))
@@ -341,8 +368,9 @@ def deserialize(blob: bytes):
     StatsDistVKDE, prepared if the blob carries the state)."""
     import struct
 
-    kind, kernel, d, cv, rot, _ct, _mi, _hf, _st = struct.unpack_from("=9i", blob, 8)
-    nu, over_smooth, shrink, local_frac = struct.unpack_from("=4d", blob, 8 + 36)
+    # header of nckde_serialize: magic[8], int32 x 10, double x 5 (layout in csrc/api.cpp)
+    kind, kernel, d, cv, rot, _ct, _mi, _hf, _st, _pf = struct.unpack_from("=10i", blob, 8)
+    nu, over_smooth, shrink, local_frac, _sf = struct.unpack_from("=5d", blob, 8 + 40)
     h = C.c_void_p()
     buf = C.create_string_buffer(blob, len(blob))
     check(lib().nckde_deserialize(C.byref(h), buf, len(blob)))

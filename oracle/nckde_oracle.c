@@ -1192,8 +1192,9 @@ struct ok_sd
 {
   int kind, kernel, d;
   double nu;
-  double over_smooth, shrink, local_frac;
-  int use_rot_href, cov_type, nearPD_maxiter, use_threads;
+  double over_smooth, shrink, local_frac, split_frac;
+  int use_rot_href, cov_type, nearPD_maxiter, use_threads, cv_type;
+  uint64_t cv_draws;
   double *cov_fixed, *cov_fixed_decomp;
 
   double *X;            /* sample array (add_obs copies), row-major */
@@ -1222,6 +1223,8 @@ ok_new (int kind, int kernel, double nu, int d)
   sd->over_smooth    = 1.0;   /* ncm_stats_dist.c:389-396 */
   sd->shrink         = 0.01;  /* ncm_stats_dist.c:421-427 */
   sd->local_frac     = 0.05;  /* ncm_stats_dist_vkde.c:276-282 */
+  sd->split_frac     = 0.5;   /* ncm_stats_dist.c:430-434 */
+  sd->cv_type        = OK_CV_NONE;
   sd->use_rot_href   = 0;
   sd->cov_type       = OK_COV_SAMPLE;
   sd->nearPD_maxiter = 200;   /* ncm_stats_dist_kde.c:290-296 */
@@ -1659,7 +1662,10 @@ ok_vkde_build_cov_array (ok_sd *sd)
   return 0;
 }
 
-/* ncm_stats_dist.c:703-789 (CV_NONE) */
+static int ok_cv_minimize (ok_sd *sd);
+static int ok_cv_split_fit (ok_sd *sd, const double *m2lnp, double *IM, const double *f, const double *f1);
+
+/* ncm_stats_dist.c:703-789 */
 int
 ok_prepare (ok_sd *sd)
 {
@@ -1668,7 +1674,9 @@ ok_prepare (ok_sd *sd)
   ok_clear_prepared (sd);
 
   sd->n_obs     = sd->n_samples;
-  sd->n_kernels = sd->n_samples;
+  sd->n_kernels = (sd->cv_type == OK_CV_SPLIT || sd->cv_type == OK_CV_SPLIT_NOFIT)
+                    ? (int) ceil (sd->n_samples * sd->split_frac)
+                    : sd->n_samples;
 
   if (sd->n_obs <= sd->d)
   {
@@ -1702,6 +1710,10 @@ ok_prepare (ok_sd *sd)
   sd->wcum_ready = 0;
   sd->prepared   = 1;
   sd->rnorm      = 0.0;
+
+  /* ncm_stats_dist.c:769-788 */
+  if (sd->cv_type == OK_CV_SPLIT_NOFIT || sd->cv_type == OK_CV_LOO)
+    return ok_cv_minimize (sd);
 
   return 0;
 }
@@ -1743,6 +1755,29 @@ ok_compute_IM (ok_sd *sd, double *IM)
 
         IM[(size_t) i * n_k + j] = chi2;
         IM[(size_t) j * n_k + i] = chi2;
+      }
+    }
+
+    /* held-out rows of the split cross-validations (kde.c:526-546) */
+    for (i = n_k; i < n_obs; i++)
+    {
+      const double *yi = sd->invU + (size_t) i * d;
+      int j;
+
+      for (j = 0; j < n_k; j++)
+      {
+        const double *yj = sd->invU + (size_t) j * d;
+        double chi2      = 0.0;
+        int l;
+
+        for (l = 0; l < d; l++)
+        {
+          const double t = yi[l] - yj[l];
+
+          chi2 = chi2 + t * t;
+        }
+
+        IM[(size_t) i * n_k + j] = chi2 / h2;
       }
     }
 
@@ -1827,7 +1862,7 @@ ok_sort_cmp_val (const void *a, const void *b)
   return ok_pair_cmp (pa, pb);
 }
 
-/* ncm_stats_dist.c:878-1094 (CV_NONE) */
+/* ncm_stats_dist.c:878-1094 */
 int
 ok_prepare_interp (ok_sd *sd, const double *m2lnp, int len)
 {
@@ -1854,16 +1889,17 @@ ok_prepare_interp (ok_sd *sd, const double *m2lnp, int len)
 
   if (sd->max_m2lnp - sd->min_m2lnp > limit)
   {
-    ok_pair *srt = malloc (sizeof (ok_pair) * sd->n_kernels);
+    /* every observation is sorted (ncm_stats_dist.c:911-914) */
+    ok_pair *srt = malloc (sizeof (ok_pair) * sd->n_obs);
     int n_cut    = 0;
 
-    for (i = 0; i < sd->n_kernels; i++)
+    for (i = 0; i < sd->n_obs; i++)
     {
       srt[i].d2 = m2lnp[i];
       srt[i].j  = i;
     }
 
-    qsort (srt, sd->n_kernels, sizeof (ok_pair), ok_sort_cmp_val);
+    qsort (srt, sd->n_obs, sizeof (ok_pair), ok_sort_cmp_val);
 
     for (i = 0; i < sd->n_kernels; i++)
     {
@@ -1879,8 +1915,11 @@ ok_prepare_interp (ok_sd *sd, const double *m2lnp, int len)
       for (i = 0; i < sd->n_kernels; i++)
         sd->weights[i] = 0.1 / (sd->n_kernels - n_cut);
 
+      /* held-out observations inside the window get no weight: the
+       * reference indexes past its weight vector there (ncm_stats_dist.c:933-940) */
       for (i = 0; i < n_cut; i++)
-        sd->weights[srt[i].j] = 0.9 / n_cut;
+        if (srt[i].j < sd->n_kernels)
+          sd->weights[srt[i].j] = 0.9 / n_cut;
 
       sd->wcum_ready = 0;
       free (srt);
@@ -1929,19 +1968,32 @@ ok_prepare_interp (ok_sd *sd, const double *m2lnp, int len)
       f1[i] = 1.0;
     }
 
-    ok_compute_IM (sd, IM);
-
-    /* ncm_stats_dist.c:791-804: row i scaled by 1/f_i */
-    for (i = 0; i < n_obs; i++)
+    if (sd->cv_type == OK_CV_SPLIT)
     {
-      const double s = 1.0 / f[i];
-      int j;
-
-      for (j = 0; j < n_k; j++)
-        IM[(size_t) i * n_k + j] = IM[(size_t) i * n_k + j] * s;
+      if (ok_cv_split_fit (sd, m2lnp, IM, f, f1) != 0)
+      {
+        free (IM);
+        free (f);
+        free (f1);
+        return -1;
+      }
     }
+    else
+    {
+      ok_compute_IM (sd, IM);
 
-    sd->rnorm = ok_nnls (n_obs, n_k, IM, f1, sd->weights);
+      /* ncm_stats_dist.c:791-804: row i scaled by 1/f_i */
+      for (i = 0; i < n_obs; i++)
+      {
+        const double s = 1.0 / f[i];
+        int j;
+
+        for (j = 0; j < n_k; j++)
+          IM[(size_t) i * n_k + j] = IM[(size_t) i * n_k + j] * s;
+      }
+
+      sd->rnorm = ok_nnls (n_obs, n_k, IM, f1, sd->weights);
+    }
 
     free (IM);
     free (f);
@@ -2172,8 +2224,18 @@ ok_kernel_choose (ok_sd *sd, double p)
 int
 ok_sample (ok_sd *sd, double u, const double *z, double chi2, double *x)
 {
-  const int d = sd->d;
   const int i = ok_kernel_choose (sd, u);
+
+  ok_sample_at (sd, i, z, chi2, x);
+
+  return i;
+}
+
+/* the kernel sample of ok_sample for a given kernel i */
+void
+ok_sample_at (ok_sd *sd, int i, const double *z, double chi2, double *x)
+{
+  const int d = sd->d;
   const double *U  = (sd->kind == OK_KDE) ? sd->cov_decomp : sd->U_arr + (size_t) i * d * d;
   const double *mu = sd->X + (size_t) i * d;
   int l, m;
@@ -2202,8 +2264,6 @@ ok_sample (ok_sd *sd, double u, const double *z, double chi2, double *x)
 
   for (l = 0; l < d; l++)
     x[l] = x[l] + mu[l];
-
-  return i;
 }
 
 void
@@ -2279,4 +2339,777 @@ ok_get_knn (ok_sd *sd, int *idx)
 {
   if (sd->knn)
     memcpy (idx, sd->knn, sizeof (int) * (size_t) sd->n_kernels * sd->k);
+}
+
+/* ======================================================================== */
+/* Cross-validated bandwidth (ncm_stats_dist.c:484-701, 769-788, 1018-1073)  */
+/* ======================================================================== */
+
+void ok_set_cv_type (ok_sd *sd, int cv) { sd->cv_type = cv; }
+void ok_set_split_frac (ok_sd *sd, double v) { sd->split_frac = v; }
+double ok_get_over_smooth (ok_sd *sd) { return sd->over_smooth; }
+
+/* GSL 2.8 nmsimplex2 (multimin/simplex2.c), the minimiser of
+ * ncm_stats_dist.c:175,664-701, in one dimension: P = 2 corners, corner moves
+ * x' = a c + b x_hi with a = (1-t) P/(P-1), b = (P t - 1)/(P-1), the squared
+ * size S2 = sum (x_i - c)^2 / P updated incrementally by each move and
+ * recomputed after a contraction towards the best corner. */
+static double
+ok_nm_size (const double *x1, double c, double *S2)
+{
+  double ss = 0.0;
+  int i;
+
+  for (i = 0; i < 2; i++)
+  {
+    const double t = x1[i] - c;
+
+    ss += t * t;
+  }
+
+  *S2 = ss / 2;
+
+  return sqrt (ss / 2);
+}
+
+static void
+ok_nm_update (double *x1, double *y1, double *c, double *S2, int i, double x, double val)
+{
+  const int P        = 2;
+  const double delta = x - x1[i];
+  const double xmc   = x1[i] - *c;
+
+  *S2 += (2.0 / P) * (xmc * delta) + ((P - 1.0) / P) * ((delta * delta) / P);
+  *c   = *c - x1[i] / P;
+  *c   = *c + x / P;
+  x1[i] = x;
+  y1[i] = val;
+}
+
+int
+ok_nm_minimize1 (ok_fn1 f, void *ctx, double *x, double step, double epsabs, int maxiter, int *iter_out,
+                 double *fval_out)
+{
+  const int P = 2;
+  double x1[2], y1[2], c, S2, size, val, val2, xc, xc2, fval;
+  int iter = 0, status = 1;
+
+  x1[0] = *x;
+  y1[0] = f (ctx, x1[0]);
+  x1[1] = *x + step;
+  y1[1] = f (ctx, x1[1]);
+
+  if (!isfinite (y1[0]) || !isfinite (y1[1]))
+    return -1;
+
+  c    = (0.0 + x1[0] + x1[1]) * (1.0 / P);
+  size = ok_nm_size (x1, c, &S2);
+  fval = y1[0];
+
+  do
+  {
+    int hi = 0, lo = 0, s_hi = 1, failed = 0;
+    double dhi = y1[0], dlo = y1[0];
+
+    iter++;
+
+    if (y1[1] < dlo)
+    {
+      dlo = y1[1];
+      lo  = 1;
+    }
+    else if (y1[1] > dhi)
+    {
+      s_hi = hi;
+      dhi  = y1[1];
+      hi   = 1;
+    }
+
+    /* reflection (t = -1) */
+    xc  = (1.0 - -1.0) * P / (P - 1.0) * c + (P * -1.0 - 1.0) / (P - 1.0) * x1[hi];
+    val = f (ctx, xc);
+
+    if (isfinite (val) && val < y1[lo])
+    {
+      /* expansion (t = -2) */
+      xc2  = (1.0 - -2.0) * P / (P - 1.0) * c + (P * -2.0 - 1.0) / (P - 1.0) * x1[hi];
+      val2 = f (ctx, xc2);
+
+      if (isfinite (val2) && val2 < y1[lo])
+        ok_nm_update (x1, y1, &c, &S2, hi, xc2, val2);
+      else
+        ok_nm_update (x1, y1, &c, &S2, hi, xc, val);
+    }
+    else if (!isfinite (val) || val > y1[s_hi])
+    {
+      if (isfinite (val) && val <= y1[hi])
+        ok_nm_update (x1, y1, &c, &S2, hi, xc, val);
+
+      /* one-dimensional contraction (t = 1/2) */
+      xc2  = (1.0 - 0.5) * P / (P - 1.0) * c + (P * 0.5 - 1.0) / (P - 1.0) * x1[hi];
+      val2 = f (ctx, xc2);
+
+      if (isfinite (val2) && val2 <= y1[hi])
+      {
+        ok_nm_update (x1, y1, &c, &S2, hi, xc2, val2);
+      }
+      else
+      {
+        const int o = 1 - lo;
+
+        x1[o] = 0.5 * (x1[o] + x1[lo]);
+        y1[o] = f (ctx, x1[o]);
+
+        if (!isfinite (y1[o]))
+          failed = 1;
+
+        c = (0.0 + x1[0] + x1[1]) * (1.0 / P);
+        ok_nm_size (x1, c, &S2);
+      }
+    }
+    else
+    {
+      ok_nm_update (x1, y1, &c, &S2, hi, xc, val);
+    }
+
+    if (failed)
+    {
+      status = -1;
+      break;
+    }
+
+    lo     = (y1[1] < y1[0]) ? 1 : 0;
+    *x     = x1[lo];
+    fval   = y1[lo];
+    size   = (S2 > 0.0) ? sqrt (S2) : ok_nm_size (x1, c, &S2);
+    status = (size < epsabs) ? 0 : 1;
+  } while (status == 1 && iter < maxiter);
+
+  if (iter_out)
+    *iter_out = iter;
+
+  if (fval_out)
+    *fval_out = fval;
+
+  return status;
+}
+
+/* levmar 2.6 dlevmar_dif (external/levmar/lm_core.c:439-850, forward
+ * differences of misc_core.c:137-171) for one parameter and a zero
+ * measurement vector, as called by ncm_stats_dist.c:1063-1065.  Returns the
+ * stop reason (info[6] of levmar). */
+int
+ok_lm_dif1 (ok_res_fn func, void *ctx, double *p_io, int n, int itmax, const double *opts, int *iter_out)
+{
+  const double tau = opts[0], eps1 = opts[1], eps2 = opts[2], eps2_sq = opts[2] * opts[2], eps3 = opts[3];
+  const double delta = opts[4];
+  const int K        = 10;
+  double *e    = malloc (sizeof (double) * n), *hx = malloc (sizeof (double) * n), *jac = malloc (sizeof (double) * n);
+  double *wrk  = malloc (sizeof (double) * n), *wrk2 = malloc (sizeof (double) * n);
+  double p     = *p_io, mu = 0.0, jacTe_inf = 0.0, p_L2 = 0.0, Dp_L2 = DBL_MAX, jacTjac = 0.0, jacTe = 0.0;
+  double diag  = 0.0, p_eL2 = 0.0;
+  int nu = 20, updjac = 0, updp = 1, newjac = 0, stop = 0, k, i, l;
+
+  if (func (ctx, p, hx) != 0)
+    stop = 8;
+
+  for (i = 0; i < n; i++)
+  {
+    e[i] = -hx[i];
+    p_eL2 += e[i] * e[i];
+  }
+
+  if (!isfinite (p_eL2))
+    stop = 7;
+
+  for (k = 0; k < itmax && !stop; ++k)
+  {
+    if (p_eL2 <= eps3)
+    {
+      stop = 6;
+      break;
+    }
+
+    if ((updp && nu > 16) || updjac == K)
+    {
+      double dstep = fabs (1e-4 * p), inv;
+
+      if (dstep < delta)
+        dstep = delta;
+
+      if (func (ctx, p + dstep, wrk) != 0)
+      {
+        stop = 8;
+        break;
+      }
+
+      inv = 1.0 / dstep;
+
+      for (i = 0; i < n; i++)
+        jac[i] = (wrk[i] - hx[i]) * inv;
+
+      nu     = 2;
+      updjac = 0;
+      updp   = 0;
+      newjac = 1;
+    }
+
+    if (newjac)
+    {
+      newjac  = 0;
+      jacTjac = 0.0;
+      jacTe   = 0.0;
+
+      for (l = n; l-- > 0;)
+      {
+        jacTjac += jac[l] * jac[l];
+        jacTe   += jac[l] * e[l];
+      }
+
+      jacTe_inf = fabs (jacTe);
+      diag      = jacTjac;
+      p_L2      = p * p;
+    }
+
+    if (jacTe_inf <= eps1)
+    {
+      Dp_L2 = 0.0;
+      stop  = 1;
+      break;
+    }
+
+    if (k == 0)
+      mu = tau * ((diag > DBL_MIN) ? diag : DBL_MIN);
+
+    jacTjac += mu;
+
+    if (jacTjac != 0.0)
+    {
+      const double Dp  = jacTe / jacTjac;
+      const double pDp = p + Dp;
+      double pDp_eL2   = 0.0, dF, dL;
+
+      Dp_L2 = Dp * Dp;
+
+      if (Dp_L2 <= eps2_sq * p_L2)
+      {
+        stop = 2;
+        break;
+      }
+
+      if (Dp_L2 >= (p_L2 + eps2) / (DBL_EPSILON * DBL_EPSILON))
+      {
+        stop = 4;
+        break;
+      }
+
+      if (func (ctx, pDp, wrk) != 0)
+      {
+        stop = 8;
+        break;
+      }
+
+      for (i = 0; i < n; i++)
+      {
+        wrk2[i] = -wrk[i];
+        pDp_eL2 += wrk2[i] * wrk2[i];
+      }
+
+      if (!isfinite (pDp_eL2))
+      {
+        stop = 7;
+        break;
+      }
+
+      dF = p_eL2 - pDp_eL2;
+
+      if (updp || dF > 0.0)
+      {
+        for (i = 0; i < n; i++)
+        {
+          const double t = (wrk[i] - hx[i] - jac[i] * Dp) / Dp_L2;
+
+          jac[i] += t * Dp;
+        }
+
+        ++updjac;
+        newjac = 1;
+      }
+
+      dL = Dp * (mu * Dp + jacTe);
+
+      if (dL > 0.0 && dF > 0.0)
+      {
+        double t = 2.0 * dF / dL - 1.0;
+
+        t  = 1.0 - t * t * t;
+        mu = mu * ((t >= 1.0 / 3.0) ? t : 1.0 / 3.0);
+        nu = 2;
+        p  = pDp;
+
+        for (i = 0; i < n; i++)
+        {
+          e[i]  = wrk2[i];
+          hx[i] = wrk[i];
+        }
+
+        p_eL2 = pDp_eL2;
+        updp  = 1;
+        continue;
+      }
+    }
+
+    mu *= nu;
+
+    if ((nu << 1) <= nu)
+    {
+      stop = 5;
+      break;
+    }
+
+    nu      = nu << 1;
+    jacTjac = diag;
+  }
+
+  if (k >= itmax && !stop)
+    stop = 3;
+
+  *p_io = p;
+
+  if (iter_out)
+    *iter_out = k;
+
+  free (e);
+  free (hx);
+  free (jac);
+  free (wrk);
+  free (wrk2);
+
+  return stop;
+}
+
+/* Weighted Welford / West update of NcmStatsVec for two variables and unit
+ * weight (ncm_stats_vec.c:510-565): the covariance term pairs the updated
+ * mean of the first variable with the old mean of the second. */
+void
+ok_pair_stats_update (double *st, double x0, double x1)
+{
+  /* st: W, W2, bias, mean0, mean1, var0, var1, cov01 */
+  const double cw = st[0] + 1.0;
+  const double x[2] = {x0, x1};
+  int i;
+
+  for (i = 0; i < 2; i++)
+  {
+    const double delta = x[i] - st[3 + i];
+    const double R     = delta * 1.0 / cw;
+    const double dvar  = st[0] * delta * R;
+
+    st[3 + i] += R;
+    st[5 + i] += dvar;
+
+    if (i == 0)
+      st[7] += 1.0 * (x[0] - st[3]) * (x[1] - st[4]);
+  }
+
+  st[0] = cw;
+  st[1] += 1.0;
+  st[2] = 1.0 / (st[0] - st[1] / st[0]);
+}
+
+/* The build's counter-based variates (Philox4x32-10 of Salmon, Moraes, Dror
+ * and Shaw, SC'11): draw j of stream (key, ctr) uses counters (j, q, ctr_lo,
+ * ctr_hi), q = 0, 1, ...: a 53-bit uniform, d Box-Muller normals (cosine
+ * first, sine kept for the next), then nu_int squared normals summed. */
+static void
+ok_philox (uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+  int r;
+
+  for (r = 0; r < 10; r++)
+  {
+    const uint64_t p0 = (uint64_t) 0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t) 0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t) (p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n1 = (uint32_t) p1;
+    const uint32_t n2 = (uint32_t) (p0 >> 32) ^ c[3] ^ k1;
+    const uint32_t n3 = (uint32_t) p0;
+
+    c[0] = n0;
+    c[1] = n1;
+    c[2] = n2;
+    c[3] = n3;
+    k0  += 0x9E3779B9u;
+    k1  += 0xBB67AE85u;
+  }
+}
+
+static double
+ok_u53 (uint32_t a, uint32_t b)
+{
+  return ((double) (a >> 5) * 67108864.0 + (double) (b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+typedef struct
+{
+  uint32_t j, q, c0, c1, k0, k1;
+  double spare;
+  int has_spare;
+} ok_stream;
+
+static void
+ok_stream_next (ok_stream *st, uint32_t r[4])
+{
+  r[0] = st->j;
+  r[1] = st->q++;
+  r[2] = st->c0;
+  r[3] = st->c1;
+  ok_philox (r, st->k0, st->k1);
+}
+
+static double
+ok_stream_normal (ok_stream *st)
+{
+  uint32_t r[4];
+  double a, b, s;
+
+  if (st->has_spare)
+  {
+    st->has_spare = 0;
+    return st->spare;
+  }
+
+  ok_stream_next (st, r);
+  a = 1.0 - ok_u53 (r[0], r[1]);
+  b = ok_u53 (r[2], r[3]);
+  s = sqrt (-2.0 * log (a));
+
+  st->spare     = s * sin (2.0 * M_PI * b);
+  st->has_spare = 1;
+
+  return s * cos (2.0 * M_PI * b);
+}
+
+void
+ok_draw (uint64_t key, uint64_t ctr, uint32_t j, int d, int nu_int, double *u, double *z, double *chi2)
+{
+  ok_stream st = {j, 0u, (uint32_t) ctr, (uint32_t) (ctr >> 32), (uint32_t) key, (uint32_t) (key >> 32), 0.0, 0};
+  uint32_t r[4];
+  int i;
+
+  ok_stream_next (&st, r);
+  *u = ok_u53 (r[0], r[1]);
+
+  for (i = 0; i < d; i++)
+    z[i] = ok_stream_normal (&st);
+
+  *chi2 = 0.0;
+
+  for (i = 0; i < nu_int; i++)
+  {
+    const double g = ok_stream_normal (&st);
+
+    *chi2 += g * g;
+  }
+}
+
+/* normal number ctr of the build's host stream (key, tag 0x43565350) */
+double
+ok_host_normal (uint64_t key, uint64_t ctr)
+{
+  uint32_t c[4] = {(uint32_t) ctr, 0u, (uint32_t) (ctr >> 32), 0x43565350u};
+  double a, b;
+
+  ok_philox (c, (uint32_t) key, (uint32_t) (key >> 32));
+  a = 1.0 - ok_u53 (c[0], c[1]);
+  b = ok_u53 (c[2], c[3]);
+
+  return sqrt (-2.0 * log (a)) * cos (2.0 * M_PI * b);
+}
+
+static void
+ok_cv_set_lnos (ok_sd *sd, double lnos, double factor)
+{
+  sd->over_smooth = exp (lnos);
+  sd->href        = factor * ok_get_href_internal (sd);
+}
+
+/* _ncm_stats_dist_m2lnp (ncm_stats_dist.c:484-510) */
+static double
+ok_cv_obj_m2lnp (void *ctx, double lnos)
+{
+  ok_sd *sd    = ctx;
+  double m2lnp = 0.0;
+  int i;
+
+  ok_cv_set_lnos (sd, lnos, 1.0);
+
+  for (i = sd->n_kernels; i < sd->n_obs; i++)
+    m2lnp += ok_eval_m2lnp (sd, sd->X + (size_t) i * sd->d);
+
+  return m2lnp;
+}
+
+/* _ncm_stats_dist_amise_kde_gauss (ncm_stats_dist.c:512-559) */
+static double
+ok_cv_obj_amise_kde_gauss (void *ctx, double lnos)
+{
+  ok_sd *sd           = ctx;
+  const int n         = sd->n_kernels;
+  const double npairs = (double) (unsigned) ((unsigned) n * ((unsigned) n - 1u));
+  double *IM          = malloc (sizeof (double) * (size_t) n * n);
+  double amise        = 0.0;
+  int i, j;
+
+  ok_cv_set_lnos (sd, lnos, sqrt (2.0));
+  ok_compute_IM (sd, IM);
+
+  for (i = 0; i < n; i++)
+    for (j = 0; j < n; j++)
+      amise += IM[(size_t) i * n + j] / ((double) n * (double) n);
+
+  ok_cv_set_lnos (sd, lnos, 1.0);
+  ok_compute_IM (sd, IM);
+
+  for (i = 0; i < n; i++)
+  {
+    for (j = 0; j < i; j++)
+      amise -= 2.0 * IM[(size_t) i * n + j] / npairs;
+
+    for (j = i + 1; j < n; j++)
+      amise -= 2.0 * IM[(size_t) i * n + j] / npairs;
+  }
+
+  free (IM);
+
+  return amise;
+}
+
+/* pairs of one device batch of the Monte-Carlo integral; pair t uses draws
+ * 2 (t mod B) and 2 (t mod B) + 1 of stream (key 0, t / B) */
+#define OK_CV_MC_BATCH 4096
+
+static int
+ok_cmp_lp (const void *a, const void *b)
+{
+  return ok_pair_cmp (a, b);
+}
+
+/* _ncm_stats_dist_amise (ncm_stats_dist.c:563-660) with the build's
+ * counter-based variates in place of the reference's mt19937 stream */
+static double
+ok_cv_obj_amise (void *ctx, double lnos)
+{
+  ok_sd *sd           = ctx;
+  const int n         = sd->n_kernels;
+  const int d         = sd->d;
+  const double npairs = (double) (unsigned) ((unsigned) n * ((unsigned) n - 1u));
+  const int nu_int    = (sd->kernel == OK_ST) ? (int) sd->nu : 0;
+  double *IM          = malloc (sizeof (double) * (size_t) n * n);
+  ok_pair *srt        = malloc (sizeof (ok_pair) * n);
+  double *z1 = malloc (sizeof (double) * d), *z2 = malloc (sizeof (double) * d);
+  double *x1 = malloc (sizeof (double) * d), *x2 = malloc (sizeof (double) * d);
+  double st[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double amise = 0.0, mean = 0.0;
+  uint64_t t;
+  int i, j;
+
+  ok_cv_set_lnos (sd, lnos, 1.0);
+  ok_compute_IM (sd, IM);
+
+  for (i = 0; i < n; i++)
+  {
+    double row_sum = 0.0;
+
+    for (j = 0; j < i; j++)
+      row_sum += IM[(size_t) i * n + j];
+
+    for (j = i + 1; j < n; j++)
+      row_sum += IM[(size_t) i * n + j];
+
+    amise -= 2.0 * row_sum / npairs;
+
+    srt[i].d2 = (row_sum + IM[(size_t) i * n + i]) / n;
+    srt[i].j  = i;
+  }
+
+  qsort (srt, n, sizeof (ok_pair), ok_cmp_lp);
+
+  for (t = 0;; t++)
+  {
+    const uint64_t b  = t / OK_CV_MC_BATCH;
+    const uint32_t jj = (uint32_t) (t % OK_CV_MC_BATCH);
+    double u1, u2, c1, c2, p1, p2;
+    int ci;
+
+    ok_draw (0, b, 2 * jj, d, nu_int, &u1, z1, &c1);
+    ok_draw (0, b, 2 * jj + 1, d, nu_int, &u2, z2, &c2);
+    ci = ok_kernel_choose (sd, u1);
+    ok_sample_at (sd, srt[ci].j, z1, c1, x1);
+    ok_sample_at (sd, srt[n - 1 - ci].j, z2, c2, x2);
+    p1 = ok_eval (sd, x1);
+    p2 = ok_eval (sd, x2);
+    ok_pair_stats_update (st, p1, p2);
+
+    if (t + 1 <= 100)
+      continue;
+
+    {
+      const uint64_t it = t + 1 - 101;
+      double var, msd;
+
+      mean = 0.5 * (st[3] + st[4]);
+      var  = 0.25 * (st[5] * st[2] + st[6] * st[2] + 2.0 * (st[7] * st[2]));
+      msd  = sqrt (var / (it + 101.0)) / mean;
+
+      if (msd < 1.0e-2 || it + 1 >= 100000000)
+        break;
+    }
+  }
+
+  amise += mean;
+
+  free (IM);
+  free (srt);
+  free (z1);
+  free (z2);
+  free (x1);
+  free (x2);
+
+  return amise;
+}
+
+/* the objective of the CV type at ln(over_smooth) (sets over_smooth and
+ * href as the reference's objectives do); for the pinning tests */
+double
+ok_cv_objective (ok_sd *sd, double lnos)
+{
+  if (sd->cv_type == OK_CV_SPLIT_NOFIT)
+    return ok_cv_obj_m2lnp (sd, lnos);
+
+  if (sd->kind == OK_KDE && sd->kernel == OK_GAUSS)
+    return ok_cv_obj_amise_kde_gauss (sd, lnos);
+
+  return ok_cv_obj_amise (sd, lnos);
+}
+
+/* _ncm_stats_dist_minimize_obj (ncm_stats_dist.c:664-701) */
+static int
+ok_cv_minimize (ok_sd *sd)
+{
+  double x = log (sd->over_smooth), fval;
+  int iter;
+  ok_fn1 obj;
+
+  if (sd->cv_type == OK_CV_LOO && sd->kernel == OK_ST && (double) (int) sd->nu != sd->nu)
+  {
+    ok_set_error ("the leave-one-out Monte-Carlo integral needs an integer Student-t nu");
+    return -1;
+  }
+
+  if (sd->cv_type == OK_CV_SPLIT_NOFIT)
+    obj = ok_cv_obj_m2lnp;
+  else if (sd->kind == OK_KDE && sd->kernel == OK_GAUSS)
+    obj = ok_cv_obj_amise_kde_gauss;
+  else
+    obj = ok_cv_obj_amise;
+
+  if (ok_nm_minimize1 (obj, sd, &x, 0.1, 1.0e-3, 1000, &iter, &fval) == -1 && iter == 0)
+  {
+    ok_set_error ("nm_minimize: non-finite function value at the start");
+    return -1;
+  }
+
+  return 0;
+}
+
+typedef struct
+{
+  ok_sd *sd;
+  const double *m2lnp, *f, *f1;
+  double *IM;
+  double rnorm;
+} ok_split_ctx;
+
+/* IM scaled by 1/f and its NNLS fit at ln(over_smooth) = lnos */
+static void
+ok_cv_fit_at (ok_split_ctx *c, double lnos)
+{
+  ok_sd *sd = c->sd;
+  const int n_obs = sd->n_obs, n_k = sd->n_kernels;
+  int i, j;
+
+  ok_cv_set_lnos (sd, lnos, 1.0);
+  ok_compute_IM (sd, c->IM);
+
+  for (i = 0; i < n_obs; i++)
+  {
+    const double s = 1.0 / c->f[i];
+
+    for (j = 0; j < n_k; j++)
+      c->IM[(size_t) i * n_k + j] = c->IM[(size_t) i * n_k + j] * s;
+  }
+
+  c->rnorm = ok_nnls (n_obs, n_k, c->IM, c->f1, sd->weights);
+}
+
+/* _ncm_stats_dist_prepare_interp_fit_nnls_f (ncm_stats_dist.c:826-855) */
+static int
+ok_cv_split_res (void *ctx, double p, double *hx)
+{
+  ok_split_ctx *c = ctx;
+  ok_sd *sd       = c->sd;
+  int i;
+
+  ok_cv_fit_at (c, p);
+
+  for (i = 0; i < sd->n_obs; i++)
+  {
+    const double m2lnpt = c->m2lnp[i] - sd->min_m2lnp;
+    const double m2lnpi = ok_eval_m2lnp (sd, sd->X + (size_t) i * sd->d);
+
+    hx[i] = expm1 (-0.5 * (m2lnpi - m2lnpt));
+  }
+
+  return 0;
+}
+
+/* CV_SPLIT branch of _ncm_stats_dist_prepare_interp (ncm_stats_dist.c:1018-1073);
+ * the ten restarts draw from the build's host normal stream (key 0) */
+static int
+ok_cv_split_fit (ok_sd *sd, const double *m2lnp, double *IM, const double *f, const double *f1)
+{
+  ok_split_ctx c = {sd, m2lnp, f, f1, IM, 0.0};
+  const double opts[5] = {1.0e-3, 1.0e-7, 1.0e-7, 1.0e-10, 1.0e-6};
+  double ln_os = log (sd->over_smooth), rnorm0;
+  int i, it;
+
+  ok_cv_fit_at (&c, ln_os);
+  rnorm0 = c.rnorm;
+
+  for (i = 0; i < 10; i++)
+  {
+    const double ln_os_try = ln_os + 0.5 * ok_host_normal (0, sd->cv_draws++);
+
+    ok_cv_fit_at (&c, ln_os_try);
+
+    if (c.rnorm < rnorm0)
+    {
+      ln_os  = ln_os_try;
+      rnorm0 = c.rnorm;
+    }
+  }
+
+  ok_lm_dif1 (ok_cv_split_res, &c, &ln_os, sd->n_obs, 10000, opts, &it);
+  ok_cv_fit_at (&c, ln_os);
+  sd->rnorm = c.rnorm;
+
+  return 0;
+}
+
+/* one Philox4x32-10 block (the known-answer tests of Random123 pin it) */
+void
+ok_philox4x32_10 (uint32_t *c, const uint32_t *k)
+{
+  ok_philox (c, k[0], k[1]);
 }

@@ -25,6 +25,8 @@ enum { OK_KDE = 0, OK_VKDE = 1 };
 enum { OK_GAUSS = 0, OK_ST = 1 };
 enum { OK_COV_SAMPLE = 0, OK_COV_FIXED = 1, OK_COV_ROBUST_DIAG = 2, OK_COV_ROBUST = 3 };
 
+enum { OK_CV_NONE = 0, OK_CV_SPLIT = 1, OK_CV_SPLIT_NOFIT = 2, OK_CV_LOO = 3 };
+
 typedef struct ok_sd ok_sd;
 
 /* ---- kernel family (ncm_stats_dist_kernel_{gauss,st}.c) ---------------- */
@@ -99,6 +101,22 @@ void   ok_get_sample (ok_sd *sd, double *X);
 void   ok_get_invUsample (ok_sd *sd, double *Y);
 void   ok_get_knn (ok_sd *sd, int *idx);
 const char *ok_last_error (void);
+
+/* ---- cross-validated bandwidth (ncm_stats_dist.c:484-701, 1018-1073) ---- */
+typedef double (*ok_fn1) (void *ctx, double x);
+typedef int (*ok_res_fn) (void *ctx, double p, double *hx);
+void   ok_set_cv_type (ok_sd *sd, int cv);
+void   ok_set_split_frac (ok_sd *sd, double v);
+double ok_get_over_smooth (ok_sd *sd);
+double ok_cv_objective (ok_sd *sd, double lnos);
+int    ok_nm_minimize1 (ok_fn1 f, void *ctx, double *x, double step, double epsabs, int maxiter, int *iter,
+                        double *fval);
+int    ok_lm_dif1 (ok_res_fn func, void *ctx, double *p, int n, int itmax, const double *opts, int *iter);
+void   ok_pair_stats_update (double *st, double x0, double x1);
+void   ok_draw (uint64_t key, uint64_t ctr, uint32_t j, int d, int nu_int, double *u, double *z, double *chi2);
+double ok_host_normal (uint64_t key, uint64_t ctr);
+void   ok_philox4x32_10 (uint32_t *c, const uint32_t *k);
+void   ok_sample_at (ok_sd *sd, int i, const double *z, double chi2, double *x);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,10 @@ _lib = None
 KDE, VKDE = 0, 1
 GAUSS, ST = 0, 1
 COV_SAMPLE, COV_FIXED, COV_ROBUST_DIAG, COV_ROBUST = 0, 1, 2, 3
+CV_NONE, CV_SPLIT, CV_SPLIT_NOFIT, CV_LOO = 0, 1, 2, 3
+
+FN1 = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double)
+RESFN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_double))
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -96,6 +100,21 @@ def lib():
             "ok_qn": (C.c_double, [C.c_int, _dp]),
             "ok_cov_robust_diag": (C.c_int, [C.c_int, C.c_int, _dp, C.c_void_p, _dp]),
             "ok_cov_robust_ogk": (C.c_int, [C.c_int, C.c_int, _dp, C.c_void_p, _dp]),
+            "ok_set_cv_type": (None, [C.c_void_p, C.c_int]),
+            "ok_set_split_frac": (None, [C.c_void_p, C.c_double]),
+            "ok_get_over_smooth": (C.c_double, [C.c_void_p]),
+            "ok_cv_objective": (C.c_double, [C.c_void_p, C.c_double]),
+            "ok_nm_minimize1": (C.c_int, [FN1, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double,
+                                          C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+            "ok_lm_dif1": (C.c_int, [RESFN, C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_int, _dp,
+                                     C.POINTER(C.c_int)]),
+            "ok_pair_stats_update": (None, [_dp, C.c_double, C.c_double]),
+            "ok_draw": (None, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.POINTER(C.c_double), _dp,
+                               C.POINTER(C.c_double)]),
+            "ok_host_normal": (C.c_double, [C.c_uint64, C.c_uint64]),
+            "ok_philox4x32_10": (None, [np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
+                                        np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]),
+            "ok_sample_at": (None, [C.c_void_p, C.c_int, _dp, C.c_double, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -220,13 +239,70 @@ def nnls(A, f):
     return x, rnorm
 
 
+# ------------------------------------------------------ cross-validation --
+def nm_minimize1(f, x0, step=0.1, epsabs=1e-3, maxiter=1000):
+    """GSL nmsimplex2 in one dimension (the minimiser of ncm_stats_dist.c:664-701).
+
+    Returns (x, fval, iterations, status); status 0 = size below epsabs."""
+    cb = FN1(lambda ctx, x: float(f(x)))
+    x = C.c_double(x0)
+    it, fv = C.c_int(), C.c_double()
+    st = lib().ok_nm_minimize1(cb, None, C.byref(x), step, epsabs, maxiter, C.byref(it), C.byref(fv))
+    return x.value, fv.value, it.value, st
+
+
+def lm_dif1(res, p0, n, itmax=10000, opts=(1e-3, 1e-7, 1e-7, 1e-10, 1e-6)):
+    """levmar dlevmar_dif for one parameter (ncm_stats_dist.c:1063-1065).
+
+    res(p) -> n residuals; returns (p, stop reason, iterations)."""
+    def cb(ctx, p, hx):
+        r = np.asarray(res(p), dtype=np.float64)
+        for i in range(n):
+            hx[i] = r[i]
+        return 0
+
+    fn = RESFN(cb)
+    p = C.c_double(p0)
+    it = C.c_int()
+    stop = lib().ok_lm_dif1(fn, None, C.byref(p), n, itmax, _f64(opts), C.byref(it))
+    return p.value, stop, it.value
+
+
+def pair_stats(x0, x1):
+    """NcmStatsVec COV statistics of two series: (mean0, mean1, var0, var1, cov01)."""
+    st = np.zeros(8)
+    for a, b in zip(x0, x1):
+        lib().ok_pair_stats_update(st, float(a), float(b))
+    return st[3], st[4], st[5] * st[2], st[6] * st[2], st[7] * st[2]
+
+
+def draw(key, ctr, j, d, nu_int=0):
+    """Draw j of the build's counter-based stream (key, ctr): (u, z[d], chi2)."""
+    u, c = C.c_double(), C.c_double()
+    z = np.empty(d)
+    lib().ok_draw(key, ctr, j, d, nu_int, C.byref(u), z, C.byref(c))
+    return u.value, z, c.value
+
+
+def philox4x32_10(ctr, key):
+    """One Philox4x32-10 block: 4 counter words, 2 key words -> 4 output words."""
+    c = np.array(ctr, dtype=np.uint32)
+    lib().ok_philox4x32_10(c, np.array(key, dtype=np.uint32))
+    return [int(v) for v in c]
+
+
+def host_normal(key, ctr):
+    return lib().ok_host_normal(key, ctr)
+
+
 # ------------------------------------------------------------ stats dist --
 class StatsDist:
-    """Oracle mirror of NcmStatsDistKDE / NcmStatsDistVKDE (CV_NONE)."""
+    """Oracle mirror of NcmStatsDistKDE / NcmStatsDistVKDE."""
 
-    def __init__(self, kind, kernel, d, nu=3.0):
+    def __init__(self, kind, kernel, d, nu=3.0, cv_type=CV_NONE):
         self._h = lib().ok_new(kind, kernel, float(nu), int(d))
         self.kind, self.kernel, self.d, self.nu = kind, kernel, int(d), float(nu)
+        lib().ok_set_cv_type(self._h, int(cv_type))
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -246,6 +322,18 @@ class StatsDist:
     def set_use_threads(self, v): lib().ok_set_use_threads(self._h, int(v))
     def set_cov_type(self, v): self._check(lib().ok_set_cov_type(self._h, int(v)))
     def set_cov_fixed(self, cov): self._check(lib().ok_set_cov_fixed(self._h, _f64(cov)))
+    def set_cv_type(self, v): lib().ok_set_cv_type(self._h, int(v))
+    def set_split_frac(self, v): lib().ok_set_split_frac(self._h, float(v))
+    def get_over_smooth(self): return lib().ok_get_over_smooth(self._h)
+
+    def cv_objective(self, lnos):
+        """Objective of the CV type at ln(over_smooth) on the prepared object."""
+        return lib().ok_cv_objective(self._h, float(lnos))
+
+    def sample_at(self, i, z, chi2=1.0):
+        x = np.empty(self.d)
+        lib().ok_sample_at(self._h, int(i), _f64(z), float(chi2), x)
+        return x
 
     def reset(self): lib().ok_reset(self._h)
 
