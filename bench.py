@@ -329,6 +329,48 @@ def c5_case(m, m_fp64):
     return res
 
 
+def cv_case(oracle_mod):
+    """Cross-validated bandwidths on VKDE Student-t nu=3, d=8, banana, n=2000
+    (local_frac 0.05): CV_LOO (Nelder-Mead on the Monte-Carlo AMISE),
+    CV_SPLIT_NOFIT (Nelder-Mead on the held-out m2lnp) and CV_SPLIT (restarts
+    and levmar on NNLS fits), each as one prepare_interp on the device.  The
+    oracle times one LOO objective and one SPLIT_NOFIT objective at
+    over_smooth 1 on one host core (the reference evaluates them serially)."""
+    import numcosmo_amd as na
+    from numcosmo_amd import synth
+
+    d, n = 8, 2000
+    tgt, X, m2, _ = synth.workload("banana", n, 10, d, 2)
+    res = {"workload": f"VKDE Student-t nu=3, d={d}, banana, n={n}"}
+    for cv in ("LOO", "SPLIT_NOFIT", "SPLIT"):
+        sd = na.StatsDistVKDE.new(na.StatsDistKernelST.new(d, 3.0), getattr(na.StatsDistCV, cv))
+        sd.add_obs_array(X)
+        t0 = time.perf_counter()
+        sd.prepare_interp(m2)
+        wall = time.perf_counter() - t0
+        nobj, secs = sd.get_cv_stats()
+        res[cv] = {"prepare_interp_s": wall, "fit_s": secs, "objectives": nobj,
+                   "s_per_objective": secs / max(nobj, 1), "over_smooth": sd.get_over_smooth()}
+        log(f"[cv] {cv}: prepare_interp {wall:.2f} s, fit {secs:.2f} s over {nobj} objectives, "
+            f"over_smooth {sd.get_over_smooth():.6f}")
+    if oracle_mod is not None:
+        od = oracle_mod.StatsDist(oracle_mod.VKDE, oracle_mod.ST, d, 3.0)
+        od.add_obs_array(X)
+        od.prepare()
+        for cv, key in ((oracle_mod.CV_LOO, "LOO"), (oracle_mod.CV_SPLIT_NOFIT, "SPLIT_NOFIT")):
+            if cv == oracle_mod.CV_SPLIT_NOFIT:
+                od = oracle_mod.StatsDist(oracle_mod.VKDE, oracle_mod.ST, d, 3.0, cv)
+                od.add_obs_array(X)
+                od.prepare()
+            od.set_cv_type(cv)
+            t0 = time.perf_counter()
+            od.cv_objective(0.0)
+            res[key]["cpu_s_per_objective"] = time.perf_counter() - t0
+            log(f"[cv] oracle {key} objective {res[key]['cpu_s_per_objective']:.3f} s (1 core)")
+        res["cpu_cores"] = 1
+    return res
+
+
 def pmc_probe(steps):
     """Short run for hardware-counter passes: the workload's mixed and FP64
     evals, then a plain device copy of 256 MiB (the calibration kernel of the
@@ -365,6 +407,7 @@ def main(argv=None):
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc passes")
     ap.add_argument("--apes-steps", type=int, default=40, help="APES ensemble steps (0: skip)")
     ap.add_argument("--c5", action="store_true", help="also run the d=64, n=50000 VKDE (minutes)")
+    ap.add_argument("--no-cv", action="store_true", help="skip the cross-validated bandwidth fits")
     ap.add_argument("--only-c5", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=10.0)
@@ -404,6 +447,8 @@ def main(argv=None):
     if args.apes_steps > 0:
         import numcosmo_amd as na
         apes = apes_case(args.apes_steps, na.Precision.MIXED)
+
+    cv = None if args.no_cv else cv_case(O)
 
     latency = None
     if not args.no_serve:
@@ -450,6 +495,7 @@ def main(argv=None):
         "latency": latency,
         "apes": apes,
         "c5": c5,
+        "cv": cv,
     }
     print(json.dumps(line))
     return 0

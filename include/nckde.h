@@ -105,6 +105,9 @@ int nckde_get_print_fit (nckde *h, int *print_fit);
 /* ncm_stats_dist_set_cv_type / get (ncm_stats_dist.h:112-113) */
 int nckde_set_cv_type (nckde *h, int cv_type);
 int nckde_get_cv_type (nckde *h, int *cv_type);
+/* No reference counterpart (measurement hook): objective evaluations (SPLIT:
+ * IM + NNLS fits) and wall seconds of the last cross-validation fit */
+int nckde_get_cv_stats (nckde *h, unsigned *n_objective, double *seconds);
 
 /* ---- sample ------------------------------------------------------------ */
 

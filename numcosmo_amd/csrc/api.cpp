@@ -9,6 +9,7 @@
 #include "nckde_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -87,6 +88,8 @@ struct nckde
   double split_frac       = 0.5;     // "split-frac" property default (ncm_stats_dist.c:430-434)
   bool print_fit          = false;
   uint64_t cv_draws       = 0;       // normals used so far by the split fit's restarts
+  unsigned cv_nobj        = 0;       // objective evaluations / NNLS fits of the last CV fit
+  double cv_seconds       = 0.0;     // wall time of the last CV fit
   int use_rot_href        = 0;
   int cov_type            = NCKDE_COV_SAMPLE;
   unsigned nearPD_maxiter = 200;
@@ -1375,9 +1378,15 @@ cv_prepare_minimize (nckde *h)
 
   std::vector<double> x {std::log (h->over_smooth)}, step {0.1};
   nck::NMInfo info {};
-  nck::ObjFn f = [&] (const double *v, double &val) { return obj (h, v[0], val); };
+  nck::ObjFn f = [&] (const double *v, double &val) {
+    h->cv_nobj++;
+    return obj (h, v[0], val);
+  };
+  const auto t0 = std::chrono::steady_clock::now ();
 
+  h->cv_nobj = 0;
   TRY (nck::nm_minimize (f, x, step, 1.0e-3, 1000, &info));
+  h->cv_seconds = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
 
   if (h->print_fit)
   {
@@ -1394,6 +1403,7 @@ cv_prepare_minimize (nckde *h)
 static int
 cv_fit_at (nckde *h, double lnos, double min_m2lnp, double &rnorm)
 {
+  h->cv_nobj++;
   cv_set_lnos (h, lnos, 1.0);
   TRY (build_IM (h, h->dm2lnp.as<double> (), min_m2lnp));
   TRY (nnls_solve (h));
@@ -1415,7 +1425,9 @@ cv_split_fit (nckde *h, const double *m2lnp, double min_m2lnp)
   const size_t no = h->n_obs;
   double ln_os    = std::log (h->over_smooth);
   double rnorm0, rnorm_try;
+  const auto t0 = std::chrono::steady_clock::now ();
 
+  h->cv_nobj = 0;
   TRY (cv_fit_at (h, ln_os, min_m2lnp, rnorm0));
 
   for (int i = 0; i < 10; ++i)
@@ -1463,6 +1475,7 @@ cv_split_fit (nckde *h, const double *m2lnp, double min_m2lnp)
   double rn;
 
   TRY (cv_fit_at (h, ln_os, min_m2lnp, rn));
+  h->cv_seconds = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
 
   return NCKDE_OK;
 }
@@ -1662,6 +1675,20 @@ nckde_get_over_smooth (nckde *h, double *v)
   }
 
   *v = h->over_smooth;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_cv_stats (nckde *h, unsigned *n_objective, double *seconds)
+{
+  TRY (check_handle (h));
+
+  if (n_objective)
+    *n_objective = h->cv_nobj;
+
+  if (seconds)
+    *seconds = h->cv_seconds;
 
   return NCKDE_OK;
 }

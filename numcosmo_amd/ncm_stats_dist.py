@@ -171,6 +171,13 @@ class _StatsDist:
         check(lib().nckde_get_cv_type(self._h, C.byref(v)))
         return StatsDistCV(v.value)
 
+    def get_cv_stats(self):
+        """(objective evaluations, seconds) of the last cross-validation fit;
+        no reference counterpart."""
+        n, t = C.c_uint(), C.c_double()
+        check(lib().nckde_get_cv_stats(self._h, C.byref(n), C.byref(t)))
+        return n.value, t.value
+
     def set_shrink(self, v):
         check(lib().nckde_set_shrink(self._h, float(v)))
         self._shrink = float(v)
