@@ -706,7 +706,9 @@ factor_diag_block (const double *__restrict__ S, int np, int kb, int nb, double 
   return 0;
 }
 
-// One panel step: every block factors the diagonal block (redundantly, from
+// Panel step for a panel narrower than 64 (the last one when np is not a
+// multiple of 64; full panels go through k_potrf_diag64 and
+// k_trsm_panel64): every block factors the diagonal block (redundantly, from
 // the unfactored values, which stay in S until k_copy_diag), then solves
 // U_kk^T x = S[kb:kb+nb][c] for its 64 columns c >= kb+nb in LDS (wave 0, one
 // lane per column, four partial sums per row) and writes them back.  Block
@@ -806,6 +808,162 @@ k_potrf_panel (double *__restrict__ S, int np, int kb, int nb, int *__restrict__
     }
 }
 
+// double broadcast from lane `src` (a compile-time constant after unrolling)
+__device__ inline double
+bcast_lane (double v, int src)
+{
+  const int lo = __builtin_amdgcn_readlane (__double2loint (v), src);
+  const int hi = __builtin_amdgcn_readlane (__double2hiint (v), src);
+
+  return __hiloint2double (hi, lo);
+}
+
+// Full panels (nb = 64) take two launches.
+// k_potrf_diag64 (one wave): the diagonal block, one column per lane in
+// registers.  Step j takes the pivot from lane j by readlane, scales row j
+// of U, publishes it through LDS (one write per lane, wide broadcast reads;
+// a single wave's LDS operations complete in order, so no barrier) and
+// applies the rank-one update.  Entries below the diagonal take part in the
+// update unmasked: they only ever combine with each other, are never read
+// as factor entries and are not stored.  The factor goes to Ud (upper
+// triangle, row-major 64 x 64) and the reciprocals of its diagonal to rd.
+__global__ void __launch_bounds__ (64)
+k_potrf_diag64 (const double *__restrict__ S, int np, int kb, int *__restrict__ info, double *__restrict__ Ud,
+                double *__restrict__ rd)
+{
+  __shared__ double R[CH_NB];
+  const int lane = threadIdx.x;
+
+  if (*info != 0)
+    return;
+
+  double a[CH_NB];
+
+#pragma unroll
+  for (int r = 0; r < CH_NB; ++r)
+    a[r] = (r <= lane) ? S[(size_t) (kb + r) * np + kb + lane] : 0.0;
+
+#pragma unroll
+  for (int j = 0; j < CH_NB; ++j)
+  {
+    const double djj = bcast_lane (a[j], j);
+
+    if (!(djj > 0.0))
+    {
+      if (lane == 0)
+        *info = kb + j + 1;
+
+      return;
+    }
+
+    const double ujj = sqrt (djj);
+
+    a[j] = (lane == j) ? ujj : a[j] / ujj;
+    R[lane] = a[j];
+    __builtin_amdgcn_fence (__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier ();
+    __builtin_amdgcn_fence (__ATOMIC_ACQUIRE, "wavefront");
+
+#pragma unroll
+    for (int i = j + 1; i < CH_NB; ++i)
+      a[i] = fma (-R[i], a[j], a[i]);
+
+    __builtin_amdgcn_wave_barrier ();
+  }
+
+#pragma unroll
+  for (int r = 0; r < CH_NB; ++r)
+  {
+    if (r <= lane)
+      Ud[r * CH_NB + lane] = a[r];
+
+    if (r == lane)
+      rd[lane] = 1.0 / a[r];
+  }
+}
+
+// U_kk^T x = S[kb:kb+64][c] for the columns c of the row panel by forward
+// substitution.  Four lanes share a column, 16 rows each: at step r the
+// lane holding row r scales it by 1/u_rr, a DPP quad broadcast hands x_r to
+// its three partners, and every lane applies x_r to its rows.  U_kk (32 KiB)
+// sits in LDS with zeros on and below the diagonal, so the updates need no
+// masks; the reciprocals of its diagonal sit beside it.  A block covers 64
+// columns.
+constexpr int CH_COLS2 = 64;
+
+template <int QB>
+__device__ inline double
+quad_bcast (double v)
+{
+  constexpr int ctrl = QB * 0x55;   // quad_perm [QB, QB, QB, QB]
+  const int lo       = __builtin_amdgcn_mov_dpp (__double2loint (v), ctrl, 0xF, 0xF, true);
+  const int hi       = __builtin_amdgcn_mov_dpp (__double2hiint (v), ctrl, 0xF, 0xF, true);
+
+  return __hiloint2double (hi, lo);
+}
+
+template <int QB>
+__device__ inline void
+trsm_quad_steps (double (&x)[16], const double *__restrict__ Us, const double *__restrict__ rs, int q)
+{
+#pragma unroll
+  for (int tr = 0; tr < 16; ++tr)
+  {
+    const int r     = QB * 16 + tr;
+    const double xr = quad_bcast<QB> (x[tr] * rs[r]);
+
+    if (q == QB)
+      x[tr] = xr;
+
+    const double *u = Us + r * CH_NB + q * 16;
+
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+      x[t] = fma (-u[t], xr, x[t]);
+  }
+}
+
+__global__ void __launch_bounds__ (256)
+k_trsm_panel64 (double *__restrict__ S, int np, int kb, const int *__restrict__ info, const double *__restrict__ Ud,
+                const double *__restrict__ rd)
+{
+  __shared__ double Us[CH_NB * CH_NB];
+  __shared__ double rs[CH_NB];
+  const int tid = threadIdx.x;
+  const int q   = tid & 3;
+  const int c   = kb + CH_NB + blockIdx.x * CH_COLS2 + (tid >> 2);
+
+  if (*info != 0)
+    return;
+
+  for (int e = tid; e < CH_NB * CH_NB; e += 256)
+    Us[e] = ((e % CH_NB) > (e / CH_NB)) ? Ud[e] : 0.0;
+
+  if (tid < CH_NB)
+    rs[tid] = rd[tid];
+
+  __syncthreads ();
+
+  // lanes past the last column work on a copy of it and store nothing
+  const bool act = c < np;
+  const int cc   = act ? c : np - 1;
+  double x[16];
+
+#pragma unroll
+  for (int t = 0; t < 16; ++t)
+    x[t] = S[(size_t) (kb + q * 16 + t) * np + cc];
+
+  trsm_quad_steps<0> (x, Us, rs, q);
+  trsm_quad_steps<1> (x, Us, rs, q);
+  trsm_quad_steps<2> (x, Us, rs, q);
+  trsm_quad_steps<3> (x, Us, rs, q);
+
+  if (act)
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+      S[(size_t) (kb + q * 16 + t) * np + c] = x[t];
+}
+
 // the factored diagonal blocks (upper triangles) into S
 __global__ void __launch_bounds__ (256)
 k_copy_diag (double *__restrict__ S, int np, const double *__restrict__ diagbuf)
@@ -834,17 +992,35 @@ k_mirror_lower (double *__restrict__ S, int np)
     S[(size_t) r * np + c] = S[(size_t) c * np + r];
 }
 
+// Right-looking blocked Cholesky S = U^T U in 64-row panels: per panel, the
+// diagonal factor (one wave), the row-panel solve, then the FP64-MFMA
+// trailing update of the upper triangle.  Factored diagonal blocks wait in
+// diagbuf until the end; *info is the 1-based failing pivot, 0 on success.
 hipError_t
 launch_potrf (double *S, int np, int *info, double *diagbuf, hipStream_t s)
 {
+  const int npan = (np + CH_NB - 1) / CH_NB;
+  double *rdall  = diagbuf + (size_t) npan * CH_NB * CH_NB;
+
   for (int kb = 0; kb < np; kb += CH_NB)
   {
     const int nb   = (np - kb < CH_NB) ? np - kb : CH_NB;
     const int rest = np - kb - nb;
-    const int grid = (rest > 0) ? (rest + 63) / 64 : 1;
+    double *db     = diagbuf + (size_t) (kb / CH_NB) * CH_NB * CH_NB;
 
-    hipLaunchKernelGGL (k_potrf_panel, dim3 (grid), dim3 (256), 0, s, S, np, kb, nb, info,
-                        diagbuf + (size_t) (kb / CH_NB) * CH_NB * CH_NB);
+    if (nb == CH_NB)
+    {
+      hipLaunchKernelGGL (k_potrf_diag64, dim3 (1), dim3 (64), 0, s, S, np, kb, info, db, rdall + kb);
+
+      if (rest > 0)
+        hipLaunchKernelGGL (k_trsm_panel64, dim3 ((rest + CH_COLS2 - 1) / CH_COLS2), dim3 (256), 0, s, S, np, kb,
+                            info, db, rdall + kb);
+    }
+    else
+    {
+      hipLaunchKernelGGL (k_potrf_panel, dim3 ((rest > 0) ? (rest + 63) / 64 : 1), dim3 (256), 0, s, S, np, kb, nb,
+                          info, db);
+    }
 
     if (rest > 0)
       launch_syrk_mfma<1> (S, np, nb, np, kb + nb, kb, S, s);
@@ -859,7 +1035,8 @@ launch_potrf (double *S, int np, int *info, double *diagbuf, hipStream_t s)
 size_t
 potrf_diag_doubles (int np)
 {
-  return (size_t) ((np + CH_NB - 1) / CH_NB) * CH_NB * CH_NB;
+  // the factored diagonal blocks, then the reciprocals of their diagonals
+  return (size_t) ((np + CH_NB - 1) / CH_NB) * CH_NB * CH_NB + (size_t) np + CH_NB;
 }
 
 // U^T y = b then U x = y with the factor of launch_potrf (upper triangle =
