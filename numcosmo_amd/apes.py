@@ -105,6 +105,19 @@ class APES:
         check(st)
         self.chain = chain
 
+    def write_catalog(self, path: str, param_names=None, param_symbols=None) -> None:
+        """The chain of the last ``run_mcmc`` as an NcmMSetCatalog FITS file:
+        one row per walker and step (NcmFit:m2lnL, then the parameters),
+        NCHAINS = nwalkers (ncm_fit_esmcmc.c writes its catalog this way)."""
+        from .catalog import MSetCatalog
+
+        if self.chain is None:
+            raise ValueError("no chain: run_mcmc with keep_chain=True first")
+        names = list(param_names) if param_names is not None else [f"APES:theta_{i}" for i in range(self.ndim)]
+        cat = MSetCatalog(names, param_symbols, nchains=self.nwalkers)
+        cat.add_rows(self.chain.reshape(-1, 1 + self.ndim))
+        cat.write(path)
+
     def get_state(self):
         th = np.empty((self.nwalkers, self.ndim))
         m2 = np.empty(self.nwalkers)

@@ -84,3 +84,19 @@ def test_apes_is_reproducible_for_a_seed():
     _, s1 = _run(2, 1, 8, seed=21)
     _, s2 = _run(2, 1, 8, seed=21)
     np.testing.assert_array_equal(s1.chain, s2.chain)
+
+
+def test_apes_chain_as_catalog(tmp_path):
+    # the run's chain in NcmMSetCatalog's FITS layout: rows ordered by step
+    # then walker, NcmFit:m2lnL first
+    from numcosmo_amd.catalog import MSetCatalog
+    _, s = _run(2, 0, 6, seed=31)
+    p = str(tmp_path / "apes.fits")
+    s.write_catalog(p, ["NcmModelMVND:mu_0", "NcmModelMVND:mu_1"])
+    cat = MSetCatalog.read(p)
+    assert cat.nchains == s.nwalkers and len(cat) == 6 * s.nwalkers
+    assert cat.param_names == ["NcmModelMVND:mu_0", "NcmModelMVND:mu_1"]
+    np.testing.assert_array_equal(cat.as_array(), s.chain.reshape(-1, 3))
+    th, m2 = s.get_state()
+    np.testing.assert_array_equal(cat.as_array()[-s.nwalkers:, 0], m2)
+    np.testing.assert_array_equal(cat.as_array()[-s.nwalkers:, 1:], th)
