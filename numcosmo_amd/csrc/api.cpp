@@ -936,7 +936,8 @@ ensure_mixed (nckde *h)
     HIPCHK (h->dA16.ensure (sizeof (_Float16) * n_pad * 2 * DP * DP));
     HIPCHK (h->dc32.ensure (sizeof (float) * n_pad * DP));
     HIPCHK (h->dcoef.ensure (sizeof (float) * 4 * n_pad));
-    HIPCHK (nck::mixed_build (h->dX.as<double> (), (int) h->n_k, d, h->D, DP, h->dWp.as<double> (),
+    HIPCHK (nck::mixed_build (h->dX.as<double> (), (int) h->n_k, d, h->D, DP,
+                              (h->kind == NCKDE_VKDE) ? h->dWp.as<double> () : nullptr,
                               h->dUg.as<double> (), h->db.as<double> (), h->bref, 1.0 / h->href, h->kernel, h->nu,
                               h->dmu.as<double> (), h->dYc.as<double> (),
                               h->dymax.as<unsigned long long> (), &h->ysc, h->dA16.as<_Float16> (),
@@ -957,12 +958,6 @@ ensure_mixed (nckde *h)
 static int
 eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s)
 {
-  if (h->kind != NCKDE_VKDE)
-  {
-    set_error ("nckde: the mixed-precision eval is implemented for NcmStatsDistVKDE; use NCKDE_PREC_FP64 for KDE");
-    return NCKDE_EUNSUPPORTED;
-  }
-
   TRY (ensure_mixed (h));
 
   const int DP      = nck::mfma_dim (h->d);
@@ -993,6 +988,7 @@ eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipS
   a.nu     = h->nu;
   a.kappa  = -0.5 * (h->nu + (double) (unsigned) h->d);
   a.lnh_d  = h->d * std::log (h->href);
+  a.lnnorm_shared = (h->kind == NCKDE_KDE) ? h->kernel_lnnorm : 0.0;
   a.f.DP   = DP;
   a.f.A    = h->dA16.p;
   a.f.c    = h->dc32.as<float> ();

@@ -133,8 +133,8 @@ launch_mixed_points (const MixedArgs &a, hipStream_t s)
 // number of tiles; padding kernels get A = 0, c = 0, b2 = -inf).
 //   A (half): [n_pad][2][DP][DP] (hi, lo);  c (float): [n_pad][DP];
 //   coef (float4): (q_i or r_i, b2_i, 1/s_i^2, 0).
-// Wp holds W_i = U_i^{-T} packed by rows with padded dimension D; Yc holds
-// y'_i = U_g^{-T} (theta_i - mu).
+// Wp holds W_i = U_i^{-T} packed by rows with padded dimension D (VKDE; NULL
+// for KDE, whose kernels all share U_g); Yc holds y'_i = U_g^{-T} (theta_i - mu).
 // --------------------------------------------------------------------------
 __global__ void
 k_mixed_factors (const double *__restrict__ Wp, int D, const double *__restrict__ Yc, const double *__restrict__ Ug,
@@ -167,22 +167,26 @@ k_mixed_factors (const double *__restrict__ Wp, int D, const double *__restrict_
     return;
   }
 
-  const double *W  = Wp + i * (size_t) (D * (D + 1) / 2);
+  // VKDE: B[r][q] = sum_{k=q..r} W[r][k] U_g[q][k] (W = U_i^{-T}); KDE
+  // (Wp == NULL): every kernel shares U_g, so B is the identity
+  const double *W  = Wp ? Wp + i * (size_t) (D * (D + 1) / 2) : nullptr;
   const double *yi = Yc + i * (size_t) d;
+  auto bval        = [&] (int r, int q) {
+    if (W == nullptr)
+      return (r == q) ? 1.0 : 0.0;
 
-  // B[r][q] = sum_{k=q..r} W[r][k] U_g[q][k] / h
+    double t = 0.0;
+
+    for (int k = q; k <= r; ++k)
+      t = fma (W[r * (r + 1) / 2 + k], Ug[q * d + k], t);
+
+    return t;
+  };
   double bmax = 0.0;
 
   for (int r = 0; r < d; ++r)
     for (int q = 0; q < d; ++q)
-    {
-      double t = 0.0;
-
-      for (int k = q; k <= r; ++k)
-        t = fma (W[r * (r + 1) / 2 + k], Ug[q * d + k], t);
-
-      bmax = fmax (bmax, fabs (t * inv_h));
-    }
+      bmax = fmax (bmax, fabs (bval (r, q) * inv_h));
 
   // s_i / ysc brings max |A_i| into [2^13, 2^14)
   const int e      = (bmax > 0.0) ? ilogb (bmax / ysc) : 0;
@@ -199,12 +203,7 @@ k_mixed_factors (const double *__restrict__ Wp, int D, const double *__restrict_
 
       if (r < d && q < d)
       {
-        double t = 0.0;
-
-        for (int k = q; k <= r; ++k)
-          t = fma (W[r * (r + 1) / 2 + k], Ug[q * d + k], t);
-
-        v  = t * inv_h * sy;
+        v  = bval (r, q) * inv_h * sy;
         cr = fma (v, yi[q] * ysc, cr);
       }
 
@@ -849,7 +848,7 @@ k_mixed_merge (MixedArgs a, double bref, int second)
   }
 
   const double lnsum = bref + LN2 * ((double) M0 + log2 (S));
-  const double m2lnp = -2.0 * (lnsum - a.lnh_d);
+  const double m2lnp = -2.0 * ((lnsum - a.lnnorm_shared) - a.lnh_d);
 
   a.out[p] = a.linear ? exp (-0.5 * m2lnp) : m2lnp;
 }

@@ -105,6 +105,7 @@ struct MixedArgs
 {
   int kernel, d, n, m, nsplit;
   double nu, kappa, lnh_d;
+  double lnnorm_shared = 0.0;  // KDE: the kernel's shared lnnorm (VKDE: 0, lnnorm_i sits in b_i)
   MixedFactors f;
   const double *X;        // points m x d (FP64, device)
   const double *mu;       // centring (d)

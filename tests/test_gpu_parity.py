@@ -264,15 +264,17 @@ MIXED_CASES = [
 ]
 
 
+@pytest.mark.parametrize("kind", [1, 0])
 @pytest.mark.parametrize("kernel,d,n,nu", MIXED_CASES)
-def test_eval_m2lnp_mixed_precision(gpu, oracle, kernel, d, n, nu):
+def test_eval_m2lnp_mixed_precision(gpu, oracle, kind, kernel, d, n, nu):
     """FP16-split MFMA distances with FP32 kernel sums: relative m2lnp error
-    |dm2lnp| / max(|m2lnp|, 1) <= 1e-6 against the FP64 oracle."""
-    _say(f"[eval mixed] kernel={kernel} d={d} n={n}")
+    |dm2lnp| / max(|m2lnp|, 1) <= 1e-6 against the FP64 oracle (VKDE, and
+    KDE, whose kernels share the global factor)."""
+    _say(f"[eval mixed] kind={kind} kernel={kernel} d={d} n={n}")
     g = np.random.default_rng(5000 + d)
     tgt = synth.Gaussian(d, g)
     X = tgt.draw(g, n)
-    sd, od = _pair(gpu, oracle, 1, kernel, d, X, nu)
+    sd, od = _pair(gpu, oracle, kind, kernel, d, X, nu)
     sd.prepare_interp(tgt.m2lnp(X)) if d <= 16 else sd.prepare()
     od.prepare()
     od.set_weights(sd.peek_weights())
