@@ -69,6 +69,7 @@ class APES:
         self._error = None
         self._cb = M2LNL_BATCH_FN(self._callback)
         self.chain = None
+        self._catalog = None
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -95,7 +96,8 @@ class APES:
         x0 = np.ascontiguousarray(initial_sample, dtype=np.float64)
         if x0.shape != (self.nwalkers, self.ndim):
             raise ValueError("Initial sample has wrong shape.")
-        check(lib().nckde_apes_init(self._h, x0, np.ascontiguousarray(self.m2lnL(x0))))
+        m2_0 = np.ascontiguousarray(self.m2lnL(x0))
+        check(lib().nckde_apes_init(self._h, x0, m2_0))
         chain = np.empty((niter, self.nwalkers, 1 + self.ndim)) if keep_chain else None
         st = lib().nckde_apes_run(self._h, int(niter), C.cast(self._cb, C.c_void_p), None,
                                   chain.ctypes.data if chain is not None else None)
@@ -104,18 +106,31 @@ class APES:
             raise e
         check(st)
         self.chain = chain
+        if keep_chain:
+            # as numcosmo_py's APES.run_mcmc: the catalog starts with the
+            # initial sample (numcosmo_py/sampling/apes.py:146-167)
+            from .catalog import MSetCatalog
+
+            if self._catalog is None:
+                self._catalog = MSetCatalog([f"APES:theta_{i}" for i in range(self.ndim)], nchains=self.nwalkers)
+                self._catalog.add_rows(np.hstack([m2_0[:, None], x0]))
+            self._catalog.add_rows(chain.reshape(-1, 1 + self.ndim))
+
+    def get_catalog(self):
+        """numcosmo_py APES.get_catalog: every walker row so far (the initial
+        sample first), as an NcmMSetCatalog-layout ``MSetCatalog``."""
+        if self._catalog is None:
+            raise ValueError("no catalog: run_mcmc with keep_chain=True first")
+        return self._catalog
 
     def write_catalog(self, path: str, param_names=None, param_symbols=None) -> None:
-        """The chain of the last ``run_mcmc`` as an NcmMSetCatalog FITS file:
-        one row per walker and step (NcmFit:m2lnL, then the parameters),
-        NCHAINS = nwalkers (ncm_fit_esmcmc.c writes its catalog this way)."""
-        from .catalog import MSetCatalog
-
-        if self.chain is None:
-            raise ValueError("no chain: run_mcmc with keep_chain=True first")
-        names = list(param_names) if param_names is not None else [f"APES:theta_{i}" for i in range(self.ndim)]
-        cat = MSetCatalog(names, param_symbols, nchains=self.nwalkers)
-        cat.add_rows(self.chain.reshape(-1, 1 + self.ndim))
+        """``get_catalog()`` as an NcmMSetCatalog FITS file: one row per
+        walker and step (NcmFit:m2lnL, then the parameters), NCHAINS =
+        nwalkers, the initial sample in the first step."""
+        cat = self.get_catalog()
+        if param_names is not None:
+            cat.param_names = list(param_names)
+            cat.param_symbols = list(param_symbols) if param_symbols is not None else list(param_names)
         cat.write(path)
 
     def get_state(self):

@@ -157,6 +157,31 @@ class MSetCatalog:
     def get_m2lnp_var(self) -> int:
         return self.m2lnp_var
 
+    def trim(self, tc: int, thin: int = 1) -> None:
+        """ncm_mset_catalog_trim (ncm_mset_catalog.c:5255-5315): drop the
+        first tc steps (tc x nchains rows) and keep every thin-th step after
+        them; FIRST_ID advances by tc x nchains."""
+        if thin < 1 or tc < 0:
+            raise ValueError("trim: thin >= 1 and tc >= 0")
+        if len(self._rows) % self.nchains != 0:
+            raise ValueError("trim: the catalog does not hold whole steps")
+        if thin == 1 and tc == 0:
+            return
+        steps = len(self._rows) // self.nchains
+        rows = self._rows
+        self._rows = [rows[t * self.nchains + i] for t in range(tc, steps, thin) for i in range(self.nchains)]
+        self.first_id += tc * self.nchains
+
+    def get_mean(self) -> np.ndarray:
+        """ncm_mset_catalog_get_mean: mean of the parameter columns."""
+        return self.as_array()[:, len(self.add_names):].mean(axis=0)
+
+    def get_covar(self) -> np.ndarray:
+        """ncm_mset_catalog_get_covar: covariance of the parameter columns
+        (bias 1/(n-1), as NcmStatsVec)."""
+        P = self.as_array()[:, len(self.add_names):]
+        return np.atleast_2d(np.cov(P, rowvar=False, ddof=1))
+
     # ---- FITS -------------------------------------------------------------------
     def write(self, path: str) -> None:
         data = self.as_array()

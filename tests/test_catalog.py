@@ -84,3 +84,15 @@ def test_card_strings_and_errors(tmp_path):
     bad.write_bytes(b"NOT A FITS FILE".ljust(2880))
     with pytest.raises(ValueError):
         K.MSetCatalog.read(str(bad))
+
+
+def test_trim_mean_covar():
+    cat = _cat(nrows=40, ndim=2, nchains=4)    # 10 steps of 4 chains
+    A = cat.as_array()
+    np.testing.assert_allclose(cat.get_mean(), A[:, 1:].mean(0), rtol=1e-14)
+    np.testing.assert_allclose(cat.get_covar(), np.cov(A[:, 1:], rowvar=False), rtol=1e-13)
+    cat.trim(3, 2)
+    assert len(cat) == 4 * 4 and cat.first_id == 12
+    np.testing.assert_array_equal(cat.as_array(), A.reshape(10, 4, 3)[3::2].reshape(-1, 3))
+    with pytest.raises(ValueError):
+        cat.trim(0, 0)

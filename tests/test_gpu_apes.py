@@ -94,9 +94,14 @@ def test_apes_chain_as_catalog(tmp_path):
     p = str(tmp_path / "apes.fits")
     s.write_catalog(p, ["NcmModelMVND:mu_0", "NcmModelMVND:mu_1"])
     cat = MSetCatalog.read(p)
-    assert cat.nchains == s.nwalkers and len(cat) == 6 * s.nwalkers
+    nw = s.nwalkers
+    assert cat.nchains == nw and len(cat) == 7 * nw   # the initial sample, then 6 steps
     assert cat.param_names == ["NcmModelMVND:mu_0", "NcmModelMVND:mu_1"]
-    np.testing.assert_array_equal(cat.as_array(), s.chain.reshape(-1, 3))
+    np.testing.assert_array_equal(cat.as_array()[nw:], s.chain.reshape(-1, 3))
+    np.testing.assert_array_equal(cat.as_array(), s.get_catalog().as_array())
+    cat.trim(1, 2)
+    assert len(cat) == 3 * nw and cat.first_id == nw
+    np.testing.assert_array_equal(cat.as_array(), s.chain[0::2].reshape(-1, 3))
     th, m2 = s.get_state()
-    np.testing.assert_array_equal(cat.as_array()[-s.nwalkers:, 0], m2)
-    np.testing.assert_array_equal(cat.as_array()[-s.nwalkers:, 1:], th)
+    np.testing.assert_array_equal(cat.as_array()[-nw:, 0], m2)
+    np.testing.assert_array_equal(cat.as_array()[-nw:, 1:], th)
