@@ -99,9 +99,9 @@ def test_apes_chain_as_catalog(tmp_path):
     assert cat.param_names == ["NcmModelMVND:mu_0", "NcmModelMVND:mu_1"]
     np.testing.assert_array_equal(cat.as_array()[nw:], s.chain.reshape(-1, 3))
     np.testing.assert_array_equal(cat.as_array(), s.get_catalog().as_array())
-    cat.trim(1, 2)
-    assert len(cat) == 3 * nw and cat.first_id == nw
-    np.testing.assert_array_equal(cat.as_array(), s.chain[0::2].reshape(-1, 3))
     th, m2 = s.get_state()
     np.testing.assert_array_equal(cat.as_array()[-nw:, 0], m2)
     np.testing.assert_array_equal(cat.as_array()[-nw:, 1:], th)
+    cat.trim(1, 2)
+    assert len(cat) == 3 * nw and cat.first_id == nw
+    np.testing.assert_array_equal(cat.as_array(), s.chain[0::2].reshape(-1, 3))
