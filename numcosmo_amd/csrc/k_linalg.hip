@@ -1059,6 +1059,7 @@ k_trsv_panel (const double *__restrict__ S, int np, int p0, int pb, double *__re
 {
   __shared__ double T[TS_NB][TS_NB + 1];
   __shared__ double y[TS_NB];
+  __shared__ double rdiag[TS_NB];
   const int tid = threadIdx.x;
 
   {
@@ -1088,11 +1089,19 @@ k_trsv_panel (const double *__restrict__ S, int np, int p0, int pb, double *__re
     const int i = tid;
     double bi   = (i < pb) ? rhs[p0 + i] : 0.0;
 
+    // reciprocals of the diagonal, then 64 serial steps with the pivot
+    // taken by readlane (j is wave-uniform) and a multiplication in place
+    // of the division
+    rdiag[i] = (i < pb) ? 1.0 / T[i][i] : 0.0;
+    __builtin_amdgcn_fence (__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier ();
+    __builtin_amdgcn_fence (__ATOMIC_ACQUIRE, "wavefront");
+
     for (int step = 0; step < pb; ++step)
     {
       const int j     = FWD ? step : pb - 1 - step;
-      const double bj = __shfl (bi, j, 64);
-      const double yj = bj / T[j][j];
+      const double bj = bcast_lane (bi, j);
+      const double yj = bj * rdiag[j];
 
       if (i == j)
         bi = yj;
