@@ -17,7 +17,7 @@
 //   Gauss  lnt2 = q_i |z'|^2 + b2_i,               q_i = -log2(e) / (2 s_i^2)
 //   ST     lnt2 = kappa log2(1 + r_i |z'|^2) + b2_i, r_i = 1 / (nu s_i^2)
 // and an online maximum M (FP32) with the scaled sum S in FP64.  Partial
-// (M, S) per kernel-range split are merged in order by k_mixed_merge.
+// (M, S) per kernel-range split are merged by k_mixed_merge in a fixed order.
 //
 // Reference: ncm_stats_dist_vkde.c:681-723 with the sum0 reduction of
 // ncm_stats_dist_kernel_gauss.c:246-290 / ncm_stats_dist_kernel_st.c:295-341.
@@ -70,8 +70,9 @@ mfma_dim (int d)
 // beyond the representable range, by a per-point power of two t_p), split
 // into FP16 hi/lo.  Layout: Yh[p][DP], Yl[p][DP]; tp[p] = (t_p, 1/t_p^2).
 // --------------------------------------------------------------------------
-__global__ void
-k_mixed_points (const double *__restrict__ X, int m, int d, int DP, const double *__restrict__ mu,
+template <int DP>
+__global__ void __launch_bounds__ (128)
+k_mixed_points (const double *__restrict__ X, int m, int d, const double *__restrict__ mu,
                 const double *__restrict__ Ug, double ysc, _Float16 *__restrict__ Yh, _Float16 *__restrict__ Yl,
                 float2 *__restrict__ tp)
 {
@@ -80,18 +81,27 @@ k_mixed_points (const double *__restrict__ X, int m, int d, int DP, const double
   if (p >= m)
     return;
 
-  double y[64];
+  // y = U_g^{-T} (x - mu) by forward substitution, in registers (DP is the
+  // padded dimension; the loops are unrolled, entries past d stay 0)
+  double y[DP];
   double ymax = 0.0;
 
-  for (int l = 0; l < d; ++l)
+#pragma unroll
+  for (int l = 0; l < DP; ++l)
   {
-    double t = X[p * d + l] - mu[l];
+    y[l] = 0.0;
 
-    for (int q = 0; q < l; ++q)
-      t = fma (-y[q], Ug[q * d + l], t);
+    if (l < d)
+    {
+      double t = X[p * d + l] - mu[l];
 
-    y[l] = t / Ug[l * d + l];
-    ymax = fmax (ymax, fabs (y[l]));
+#pragma unroll
+      for (int q = 0; q < l; ++q)
+        t = fma (-y[q], Ug[q * d + l], t);
+
+      y[l] = t / Ug[l * d + l];
+      ymax = fmax (ymax, fabs (y[l]));
+    }
   }
 
   double sc = ysc;
@@ -101,15 +111,44 @@ k_mixed_points (const double *__restrict__ X, int m, int d, int DP, const double
     sc *= 0.0625;
 
   const double t_p = sc / ysc;
+  _Float16 hi[DP], lo[DP];
 
+#pragma unroll
   for (int l = 0; l < DP; ++l)
   {
-    const double v     = (l < d) ? y[l] * sc : 0.0;
-    const _Float16 hi  = (_Float16) v;
-    const _Float16 lo  = (_Float16) (v - (double) hi);
+    const double v = y[l] * sc;
 
-    Yh[p * DP + l] = hi;
-    Yl[p * DP + l] = lo;
+    hi[l] = (_Float16) v;
+    lo[l] = (_Float16) (v - (double) hi[l]);
+  }
+
+  // whole-row vector stores (DP halves = 8 or more bytes)
+  if constexpr (DP % 8 == 0)
+  {
+#pragma unroll
+    for (int c = 0; c < DP / 8; ++c)
+    {
+      half8 vh, vl;
+
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+      {
+        vh[k] = hi[c * 8 + k];
+        vl[k] = lo[c * 8 + k];
+      }
+
+      *(half8 *) (Yh + p * DP + c * 8) = vh;
+      *(half8 *) (Yl + p * DP + c * 8) = vl;
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int l = 0; l < DP; ++l)
+    {
+      Yh[p * DP + l] = hi[l];
+      Yl[p * DP + l] = lo[l];
+    }
   }
 
   tp[p] = make_float2 ((float) t_p, (float) (1.0 / (t_p * t_p)));
@@ -121,9 +160,17 @@ launch_mixed_points (const MixedArgs &a, hipStream_t s)
   _Float16 *Yh = (_Float16 *) a.Yh;
   _Float16 *Yl = Yh + (size_t) a.m * a.f.DP;
   float2 *tp   = (float2 *) (Yl + (size_t) a.m * a.f.DP);
+  const dim3 grid ((a.m + 127) / 128), block (128);
 
-  hipLaunchKernelGGL (k_mixed_points, dim3 ((a.m + 127) / 128), dim3 (128), 0, s, a.X, a.m, a.d, a.f.DP, a.mu, a.G,
-                      a.ysc, Yh, Yl, tp);
+  switch (a.f.DP)
+  {
+    case 4: hipLaunchKernelGGL (k_mixed_points<4>, grid, block, 0, s, a.X, a.m, a.d, a.mu, a.G, a.ysc, Yh, Yl, tp); break;
+    case 8: hipLaunchKernelGGL (k_mixed_points<8>, grid, block, 0, s, a.X, a.m, a.d, a.mu, a.G, a.ysc, Yh, Yl, tp); break;
+    case 16: hipLaunchKernelGGL (k_mixed_points<16>, grid, block, 0, s, a.X, a.m, a.d, a.mu, a.G, a.ysc, Yh, Yl, tp); break;
+    case 32: hipLaunchKernelGGL (k_mixed_points<32>, grid, block, 0, s, a.X, a.m, a.d, a.mu, a.G, a.ysc, Yh, Yl, tp); break;
+    case 64: hipLaunchKernelGGL (k_mixed_points<64>, grid, block, 0, s, a.X, a.m, a.d, a.mu, a.G, a.ysc, Yh, Yl, tp); break;
+    default: return hipErrorInvalidValue;
+  }
 
   return hipGetLastError ();
 }
@@ -816,25 +863,42 @@ k_eval_mixed (MixedArgs a)
 constexpr float MIXED_FLOOR = -100.0f;
 
 // pass 1: log sum = b_ref + ln2 (M0 + log2 S); flag points that underflowed
-__global__ void
+// Merge of the nsplit partial (max, sum) pairs of every point: a block takes
+// 32 points with 8 lanes each; lane l sums the splits s = l, l+8, ... in
+// order (reads of consecutive points are coalesced), then the 8 lane
+// partials combine in lane order, so the result depends only on nsplit.
+constexpr int MG_PTS = 32, MG_LANES = 8;
+
+__global__ void __launch_bounds__ (MG_PTS * MG_LANES)
 k_mixed_merge (MixedArgs a, double bref, int second)
 {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float Ms[MG_LANES][MG_PTS];
+  __shared__ double Ss[MG_LANES][MG_PTS];
+  const int pl  = threadIdx.x % MG_PTS;
+  const int sl  = threadIdx.x / MG_PTS;
+  const int p   = blockIdx.x * MG_PTS + pl;
+  const bool on = (p < a.m) && !(second && !a.flag[p]);
+  float Mx      = -INFINITY;
+  double S      = 0.0;
 
-  if (p >= a.m)
+  if (on)
+    for (int s = sl; s < a.nsplit; s += MG_LANES)
+    {
+      Mx = fmaxf (Mx, a.partM[(size_t) s * a.m + p]);
+      S += a.partS[(size_t) s * a.m + p];
+    }
+
+  Ms[sl][pl] = Mx;
+  Ss[sl][pl] = S;
+  __syncthreads ();
+
+  if (sl != 0 || !on)
     return;
 
-  if (second && !a.flag[p])
-    return;
-
-  const double *pS = a.partS;
-  float Mx         = -INFINITY;
-  double S         = 0.0;
-
-  for (int s = 0; s < a.nsplit; ++s)
+  for (int l = 1; l < MG_LANES; ++l)
   {
-    Mx = fmaxf (Mx, a.partM[(size_t) s * a.m + p]);
-    S += pS[(size_t) s * a.m + p];
+    Mx = fmaxf (Mx, Ms[l][pl]);
+    S += Ss[l][pl];
   }
 
   const float M0 = second ? a.M0[p] : 0.0f;
@@ -931,7 +995,8 @@ launch_eval_mixed (const MixedArgs &a, hipStream_t s)
 hipError_t
 launch_mixed_merge (const MixedArgs &a, double bref, int second, hipStream_t s)
 {
-  hipLaunchKernelGGL (k_mixed_merge, dim3 ((a.m + 255) / 256), dim3 (256), 0, s, a, bref, second);
+  hipLaunchKernelGGL (k_mixed_merge, dim3 ((a.m + MG_PTS - 1) / MG_PTS), dim3 (MG_PTS * MG_LANES), 0, s, a, bref,
+                      second);
 
   return hipGetLastError ();
 }
