@@ -268,6 +268,14 @@ int nckde_apes_run (nckde_apes *a, unsigned nsteps, nckde_m2lnL_batch_fn like, v
 int nckde_apes_get_state (nckde_apes *a, double *theta, double *m2lnL);
 /* acceptance counters and seconds spent in (prepare, sample, eval, likelihood) */
 int nckde_apes_get_stats (nckde_apes *a, uint64_t *accepted, uint64_t *proposed, double *seconds);
+/* No reference counterpart (diagnostic hook): from the next nckde_apes_run
+ * on, one row of 2 dim + 9 doubles per walker and half-step goes to buf,
+ * up to max_rows rows: half (0, 1), walker, exploration active (0, 1),
+ * m2lnL, m2lnL*, m2lnp*, m2lnp_cur, jump, accepted (0, 1), theta[dim],
+ * theta*[dim].  buf = NULL stops the record.  The buffer stays the
+ * caller's. */
+int nckde_apes_set_trace (nckde_apes *a, double *buf, size_t max_rows);
+int nckde_apes_get_trace_rows (nckde_apes *a, size_t *rows);
 
 /* ---- served requests (continuous batching) -------------------------------
  *

@@ -85,6 +85,8 @@ SIGNATURES = {
     "nckde_apes_run": (C.c_int, [_h, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nckde_apes_get_state": (C.c_int, [_h, _dp, _dp]),
     "nckde_apes_get_stats": (C.c_int, [_h, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _dp]),
+    "nckde_apes_set_trace": (C.c_int, [_h, C.c_void_p, _sz]),
+    "nckde_apes_get_trace_rows": (C.c_int, [_h, C.POINTER(C.c_size_t)]),
     "nckde_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint, _sz]),
     "nckde_server_destroy": (C.c_int, [_h]),
     "nckde_server_submit": (C.c_int, [_h, C.c_void_p, C.POINTER(C.c_uint64)]),

@@ -69,6 +69,7 @@ class APES:
         self._error = None
         self._cb = M2LNL_BATCH_FN(self._callback)
         self.chain = None
+        self.trace = None
         self._catalog = None
 
     def __del__(self):
@@ -92,20 +93,29 @@ class APES:
             self._error = e
             return 1
 
-    def run_mcmc(self, initial_sample: np.ndarray, niter: int, keep_chain: bool = True) -> None:
+    def run_mcmc(self, initial_sample: np.ndarray, niter: int, keep_chain: bool = True,
+                 trace: bool = False) -> None:
         x0 = np.ascontiguousarray(initial_sample, dtype=np.float64)
         if x0.shape != (self.nwalkers, self.ndim):
             raise ValueError("Initial sample has wrong shape.")
         m2_0 = np.ascontiguousarray(self.m2lnL(x0))
         check(lib().nckde_apes_init(self._h, x0, m2_0))
         chain = np.empty((niter, self.nwalkers, 1 + self.ndim)) if keep_chain else None
+        # optional record of every move: rows (half, walker, exploration,
+        # m2lnL, m2lnL*, m2lnp*, m2lnp_cur, jump, accepted, theta, theta*)
+        tbuf = np.empty((niter * self.nwalkers, 2 * self.ndim + 9)) if trace else None
+        check(lib().nckde_apes_set_trace(self._h, tbuf.ctypes.data if trace else None, tbuf.shape[0] if trace else 0))
         st = lib().nckde_apes_run(self._h, int(niter), C.cast(self._cb, C.c_void_p), None,
                                   chain.ctypes.data if chain is not None else None)
+        rows = C.c_size_t()
+        check(lib().nckde_apes_get_trace_rows(self._h, C.byref(rows)))
+        check(lib().nckde_apes_set_trace(self._h, None, 0))
         if self._error is not None:
             e, self._error = self._error, None
             raise e
         check(st)
         self.chain = chain
+        self.trace = tbuf[:rows.value] if trace else None
         if keep_chain:
             # as numcosmo_py's APES.run_mcmc: the catalog starts with the
             # initial sample (numcosmo_py/sampling/apes.py:146-167)
@@ -132,6 +142,14 @@ class APES:
             cat.param_names = list(param_names)
             cat.param_symbols = list(param_symbols) if param_symbols is not None else list(param_names)
         cat.write(path)
+
+    def set_exploration(self, steps: int) -> None:
+        """ncm_fit_esmcmc_walker_apes_set_exploration: the next ``steps``
+        half-step setups count down an exploration phase in which the
+        acceptance uses the likelihood ratio alone (ncm_fit_esmcmc_walker_apes.c:815-816,
+        895-919); the countdown runs before each half's acceptance, so the
+        first ``steps - 1`` half-steps explore."""
+        check(lib().nckde_apes_set_exploration(self._h, int(steps)))
 
     def get_state(self):
         th = np.empty((self.nwalkers, self.ndim))

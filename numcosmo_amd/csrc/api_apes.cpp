@@ -65,6 +65,9 @@ struct nckde_apes
   uint64_t seed = 0, draw_ctr = 0;   // counter-based device draws of the KDE proposals
   nckde *sd[2] = {nullptr, nullptr};
   uint64_t accepted = 0, proposed = 0, offboard = 0;
+  // optional record of every move (nckde_apes_set_trace)
+  double *trace = nullptr;
+  size_t trace_cap = 0, trace_len = 0;
   double t_prepare = 0, t_sample = 0, t_eval = 0, t_like = 0;
   bool initialised = false;
 };
@@ -242,6 +245,12 @@ apes_half (nckde_apes *a, unsigned k0, unsigned k1, unsigned other0, int which, 
   a->t_sample += now_s () - t0;
   t0 = now_s ();
 
+  // the exploration countdown runs at the end of each half's setup, before
+  // its acceptance (ncm_fit_esmcmc_walker_apes.c:815-816), so N steps of
+  // exploration cover the first N - 1 half-steps
+  if (a->exploration > 0)
+    a->exploration--;
+
   // ---- transition densities: q(theta*) and q(theta) in one eval ----------
   std::vector<double> pts ((size_t) 2 * nh * d), q (2 * nh);
 
@@ -327,7 +336,27 @@ apes_half (nckde_apes *a, unsigned k0, unsigned k1, unsigned other0, int which, 
 
     ++a->proposed;
 
-    if (jumps[k] < prob)
+    const bool acc = jumps[k] < prob;
+
+    if (a->trace != nullptr && a->trace_len < a->trace_cap)
+    {
+      double *r = a->trace + a->trace_len * (2 * (size_t) d + 9);
+
+      r[0] = which;
+      r[1] = k;
+      r[2] = (a->exploration > 0) ? 1.0 : 0.0;
+      r[3] = a->m2lnL[k];
+      r[4] = like_star[j];
+      r[5] = m2lnp_star[j];
+      r[6] = m2lnp_cur[j];
+      r[7] = jumps[k];
+      r[8] = acc ? 1.0 : 0.0;
+      std::memcpy (r + 9, a->theta.data () + (size_t) k * d, sizeof (double) * d);
+      std::memcpy (r + 9 + d, prop.data () + (size_t) j * d, sizeof (double) * d);
+      a->trace_len++;
+    }
+
+    if (acc)
     {
       std::memcpy (a->theta.data () + (size_t) k * d, prop.data () + (size_t) j * d, sizeof (double) * d);
       a->m2lnL[k] = like_star[j];
@@ -543,9 +572,6 @@ nckde_apes_run (nckde_apes *a, unsigned nsteps, nckde_m2lnL_batch_fn like, void 
     TRYA (apes_half (a, 0, a->n2, a->n2, 0, jumps.data (), like, user));
     TRYA (apes_half (a, a->n2, a->n, 0, 1, jumps.data (), like, user));
 
-    if (a->exploration > 0)
-      a->exploration--;
-
     if (chain)
     {
       double *c = chain + (size_t) s * a->n * row;
@@ -557,6 +583,36 @@ nckde_apes_run (nckde_apes *a, unsigned nsteps, nckde_m2lnL_batch_fn like, void 
       }
     }
   }
+
+  return NCKDE_OK;
+}
+
+int
+nckde_apes_set_trace (nckde_apes *a, double *buf, size_t max_rows)
+{
+  if (a == nullptr)
+  {
+    set_error ("nckde_apes: NULL handle");
+    return NCKDE_EINVAL;
+  }
+
+  a->trace     = buf;
+  a->trace_cap = (buf != nullptr) ? max_rows : 0;
+  a->trace_len = 0;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_apes_get_trace_rows (nckde_apes *a, size_t *rows)
+{
+  if (a == nullptr || rows == nullptr)
+  {
+    set_error ("nckde_apes: NULL argument");
+    return NCKDE_EINVAL;
+  }
+
+  *rows = a->trace_len;
 
   return NCKDE_OK;
 }

@@ -436,3 +436,55 @@ class StatsDist:
         idx = np.empty((self.get_n_kernels(), k), dtype=np.int32)
         lib().ok_get_knn(self._h, idx)
         return idx
+
+
+# ---- APES transition densities (pure Python; small ensembles) -----------------
+ERF_BOUND = 3.0
+
+
+def log_normal_gaussian_integral(zl, zu):
+    """ln |integral of the standard normal over [zl, zu]| by the branches of
+    ncm_util.c:646-690 (erfc in either tail, log1p when both ends lie
+    beyond ERF_BOUND / sqrt 2 on opposite sides, erf otherwise)."""
+    import math
+    if zl == zu:
+        return -math.inf
+    ul, uu = (zl, zu) if zl < zu else (zu, zl)
+    ul *= math.sqrt(0.5)
+    uu *= math.sqrt(0.5)
+    if ul > ERF_BOUND:
+        return math.log(abs(0.5 * (math.erfc(ul) - math.erfc(uu))))
+    if uu < -ERF_BOUND:
+        return math.log(abs(0.5 * (math.erfc(-ul) - math.erfc(-uu))))
+    if uu > ERF_BOUND and ul < -ERF_BOUND:
+        return math.log1p(-0.5 * (math.erfc(uu) + math.erfc(-ul)))
+    return math.log(abs(0.5 * (math.erf(uu) - math.erf(ul))))
+
+
+def apes_rw_std(full_cov):
+    """Random-walk scales of ncm_fit_esmcmc_walker_apes.c:647-690: a quarter
+    of the square root of the diagonal of the sample covariance."""
+    v = np.diag(np.asarray(full_cov, dtype=np.float64))
+    if np.any(v <= 0.0):
+        raise OracleError("non-positive covariance diagonal")
+    return 0.25 * np.sqrt(v)
+
+
+def apes_transition_m2lnp(sd, theta, thetastar, lb, ub, std, rw_prob):
+    """-2 ln q(theta -> thetastar) of ncm_fit_esmcmc_walker_apes.c:819-859:
+    the density ``sd`` at thetastar mixed, with weight rw_prob, with a
+    Gaussian random walk centred at theta and truncated to [lb, ub]."""
+    import math
+    m_sd = sd.eval_m2lnp(np.asarray(thetastar, dtype=np.float64))
+    if rw_prob <= 0.0:
+        return m_sd
+    m_rw = 0.0
+    for i in range(len(theta)):
+        ln_norm = 0.5 * math.log(2.0 * math.pi) + math.log(std[i]) + \
+            log_normal_gaussian_integral((lb[i] - theta[i]) / std[i], (ub[i] - theta[i]) / std[i])
+        m_rw += ((thetastar[i] - theta[i]) / std[i]) ** 2 + 2.0 * ln_norm
+    m_rw += -2.0 * math.log(rw_prob)
+    m_sd += -2.0 * math.log1p(-rw_prob)
+    if m_sd < m_rw:
+        return m_sd - 2.0 * math.log1p(math.exp(-0.5 * (m_rw - m_sd)))
+    return m_rw - 2.0 * math.log1p(math.exp(-0.5 * (m_sd - m_rw)))
