@@ -8,7 +8,8 @@
 //   chi2_i = |z'_i|^2 / s_i^2.
 // A_i and ysc y(x) are split into FP16 hi + lo parts (22 significant bits);
 // z' is accumulated in FP32 by v_mfma_f32_32x32x16_f16 from three products
-// (A_hi y_hi + A_hi y_lo + A_lo y_hi), with -c_i as the accumulator input.
+// (A_hi y_hi + A_hi y_lo + A_lo y_hi), with -c_i (stored negated) as the
+// accumulator input.
 // Each 32-row MFMA tile holds 32/DP kernels (DP = padded dimension 4, 8, 16)
 // or one kernel over 1 or 2 row tiles (DP = 32, 64); for DP < 16 the hi/lo
 // operands are packed along K so no K lanes are wasted.
@@ -261,7 +262,8 @@ k_mixed_factors (const double *__restrict__ Wp, int D, const double *__restrict_
       Al[r * DP + q] = lo;
     }
 
-    ci[r] = (float) cr;
+    // stored negated: the MFMA accumulator starts from -B y_i directly
+    ci[r] = (float) -cr;
   }
 
   const double inv_s2 = 1.0 / (s_i * s_i);
@@ -339,36 +341,33 @@ k_colmean (const double *__restrict__ X, int n, int d, double *__restrict__ mu)
 // 32 points whose B fragments stay in registers; every wave of the grid walks
 // a range of kernel groups.
 // --------------------------------------------------------------------------
+// The group's factors start at a wave-uniform address (scalar registers);
+// each lane adds a 32-bit offset, so the loads take the scalar-base form and
+// need no 64-bit vector address arithmetic.
 template <int DP>
 __device__ inline half8
 load_a_frag (const _Float16 *__restrict__ A, int tg, int rt, int kb, int lohi, int lane)
 {
-  using P       = Plan<DP>;
-  const int g   = lane >> 5;
-  const int R   = lane & 31;
+  using P            = Plan<DP>;
+  const unsigned g   = lane >> 5;
+  const unsigned R   = lane & 31;
+  const _Float16 *Ag = A + (size_t) tg * P::KPT * 2 * DP * DP;
   half8 v;
 
   if (DP >= 16)
   {
-    const int kern = (DP == 64) ? tg : tg * P::KPT + R / DP;
-    const int row  = (DP == 64) ? rt * 32 + R : R % DP;
-    const _Float16 *base = A + ((size_t) kern * 2 + lohi) * DP * DP + (size_t) row * DP + kb * 16 + g * 8;
+    const unsigned kl  = R / DP;   // kernel within the group (0 for DP >= 32)
+    const unsigned row = (DP == 64) ? rt * 32 + R : R % DP;
 
-    v = *(const half8 *) base;
+    v = *(const half8 *) (Ag + ((kl * 2 + lohi) * DP * DP + row * DP + kb * 16 + g * 8));
   }
   else if (DP == 8)
   {
-    const int kern = tg * P::KPT + R / 8;
-    const int row  = R % 8;
-    const _Float16 *base = A + ((size_t) kern * 2 + g) * 64 + row * 8;  // g = 0: hi, 1: lo
-
-    v = *(const half8 *) base;
+    v = *(const half8 *) (Ag + (((R / 8) * 2 + g) * 64 + (R % 8) * 8));  // g = 0: hi, 1: lo
   }
   else
   {
-    const int kern = tg * P::KPT + R / 4;
-    const int row  = R % 4;
-    const _Float16 *bh = A + (size_t) kern * 2 * 16 + row * 4;
+    const _Float16 *bh = Ag + ((R / 4) * 2 * 16 + (R % 4) * 4);
     const half4 h      = *(const half4 *) bh;
     const half4 l      = *(const half4 *) (bh + 16);
 
@@ -455,8 +454,10 @@ __device__ inline void
 load_group (GroupData<DP> &G, const _Float16 *__restrict__ A, const float *__restrict__ cv,
             const float4 *__restrict__ coef, int tg, int lane)
 {
-  using P     = Plan<DP>;
-  const int g = lane >> 5;
+  using P          = Plan<DP>;
+  const unsigned g = lane >> 5;
+  const float *cg  = cv + (size_t) tg * P::KPT * DP;   // wave-uniform group bases
+  const float4 *fg = coef + (size_t) tg * P::KPT;
 
 #pragma unroll
   for (int rt = 0; rt < P::NRT; ++rt)
@@ -470,19 +471,17 @@ load_group (GroupData<DP> &G, const _Float16 *__restrict__ A, const float *__res
         G.Al[rt][kb] = load_a_frag<DP> (A, tg, rt, kb, 1, lane);
     }
 
-    // -c for the 16 rows held by this lane: rows 8 j + 4 g + r
+    // the (negated) centre terms of the 16 rows held by this lane: rows 8 j + 4 g + r
 #pragma unroll
     for (int j = 0; j < 4; ++j)
     {
-      const int Rr    = 8 * j + 4 * g;
-      const int kern  = (DP == 64) ? tg : tg * P::KPT + Rr / DP;
-      const int row   = (DP == 64) ? rt * 32 + Rr : Rr % DP;
-      const float4 c4 = *(const float4 *) (cv + (size_t) kern * DP + row);
+      const unsigned Rr = 8 * j + 4 * g;
+      const float4 c4   = *(const float4 *) (cg + ((DP == 64) ? rt * 32 + Rr : Rr));
 
-      G.cinit[rt][4 * j + 0] = -c4.x;
-      G.cinit[rt][4 * j + 1] = -c4.y;
-      G.cinit[rt][4 * j + 2] = -c4.z;
-      G.cinit[rt][4 * j + 3] = -c4.w;
+      G.cinit[rt][4 * j + 0] = c4.x;
+      G.cinit[rt][4 * j + 1] = c4.y;
+      G.cinit[rt][4 * j + 2] = c4.z;
+      G.cinit[rt][4 * j + 3] = c4.w;
     }
   }
 
@@ -499,7 +498,7 @@ load_group (GroupData<DP> &G, const _Float16 *__restrict__ A, const float *__res
     else if (DP == 16)
       slot = g;
 
-    G.cf[q] = coef[(size_t) tg * P::KPT + slot];
+    G.cf[q] = fg[slot];
   }
 }
 
