@@ -42,7 +42,7 @@ struct Plan
   static constexpr int KPT    = (DP >= 32) ? 1 : 32 / DP;  // kernels per 32-row tile
   static constexpr int NRT    = (DP == 64) ? 2 : 1;        // row tiles per kernel group
   static constexpr int NKB    = (DP >= 32) ? DP / 16 : 1;  // 16-wide K blocks
-  static constexpr int NLOCAL = (DP <= 8) ? 4 : (DP == 16 ? 2 : 1);
+  static constexpr int NLOCAL = (DP == 4) ? 4 : (DP == 8 ? 2 : 1);   // kernels per lane
 };
 
 int
@@ -344,6 +344,39 @@ k_colmean (const double *__restrict__ X, int n, int d, double *__restrict__ mu)
 // The group's factors start at a wave-uniform address (scalar registers);
 // each lane adds a 32-bit offset, so the loads take the scalar-base form and
 // need no 64-bit vector address arithmetic.
+//
+// Row map of a 32-row tile.  The 32x32 accumulator gives lane l the rows
+// 8 j + 4 (l / 32) + r (j, r < 4).  Kernel rows are placed so that these
+// rows hold whole kernels and no lane needs a partner's partial sum:
+//   DP = 4:  row R = kernel R / 4, dim R % 4 (kernels 2 j + g);
+//   DP = 8:  row 8 j + 4 g + r = kernel 2 g + j / 2, dim 4 (j % 2) + r;
+//   DP = 16: row 8 j + 4 g + r = kernel g, dim 4 j + r.
+// At DP >= 32 one kernel fills the tile and the half-waves add partials.
+template <int DP>
+__device__ inline void
+tile_row (unsigned R, unsigned &kl, unsigned &dim)
+{
+  if (DP == 4)
+  {
+    kl  = R >> 2;
+    dim = R & 3;
+  }
+  else if (DP == 8)
+  {
+    kl  = 2 * ((R >> 2) & 1) + (R >> 4);
+    dim = ((R >> 3) & 1) * 4 + (R & 3);
+  }
+  else if (DP == 16)
+  {
+    kl  = (R >> 2) & 1;
+    dim = (R >> 3) * 4 + (R & 3);
+  }
+  else
+  {
+    kl  = 0;
+    dim = R;
+  }
+}
 template <int DP>
 __device__ inline half8
 load_a_frag (const _Float16 *__restrict__ A, int tg, int rt, int kb, int lohi, int lane)
@@ -354,20 +387,22 @@ load_a_frag (const _Float16 *__restrict__ A, int tg, int rt, int kb, int lohi, i
   const _Float16 *Ag = A + (size_t) tg * P::KPT * 2 * DP * DP;
   half8 v;
 
+  unsigned kl, row;
+
+  tile_row<DP> (R, kl, row);
+
   if (DP >= 16)
   {
-    const unsigned kl  = R / DP;   // kernel within the group (0 for DP >= 32)
-    const unsigned row = (DP == 64) ? rt * 32 + R : R % DP;
-
+    row += (DP == 64) ? rt * 32 : 0;
     v = *(const half8 *) (Ag + ((kl * 2 + lohi) * DP * DP + row * DP + kb * 16 + g * 8));
   }
   else if (DP == 8)
   {
-    v = *(const half8 *) (Ag + (((R / 8) * 2 + g) * 64 + (R % 8) * 8));  // g = 0: hi, 1: lo
+    v = *(const half8 *) (Ag + ((kl * 2 + g) * 64 + row * 8));  // g = 0: hi, 1: lo
   }
   else
   {
-    const _Float16 *bh = Ag + ((R / 4) * 2 * 16 + (R % 4) * 4);
+    const _Float16 *bh = Ag + (kl * 2 * 16 + row * 4);
     const half4 h      = *(const half4 *) bh;
     const half4 l      = *(const half4 *) (bh + 16);
 
@@ -475,8 +510,11 @@ load_group (GroupData<DP> &G, const _Float16 *__restrict__ A, const float *__res
 #pragma unroll
     for (int j = 0; j < 4; ++j)
     {
-      const unsigned Rr = 8 * j + 4 * g;
-      const float4 c4   = *(const float4 *) (cg + ((DP == 64) ? rt * 32 + Rr : Rr));
+      unsigned kl, row;
+
+      tile_row<DP> (8 * j + 4 * g, kl, row);
+
+      const float4 c4 = *(const float4 *) (cg + kl * DP + ((DP == 64) ? rt * 32 + row : row));
 
       G.cinit[rt][4 * j + 0] = c4.x;
       G.cinit[rt][4 * j + 1] = c4.y;
@@ -494,7 +532,7 @@ load_group (GroupData<DP> &G, const _Float16 *__restrict__ A, const float *__res
     if (DP == 4)
       slot = 2 * q + g;
     else if (DP == 8)
-      slot = 2 * (q & 1) + g;
+      slot = 2 * g + (q & 1);
     else if (DP == 16)
       slot = g;
 
@@ -572,7 +610,7 @@ group_step (const GroupData<DP> &cur, const half8 (&Bh)[NPT][Plan<DP>::NKB], con
 #pragma unroll
       for (int r = 0; r < 16; ++r)
       {
-        const int j = (DP <= 8) ? r / 4 : ((DP == 16) ? r / 8 : 0);
+        const int j = (DP == 4) ? r / 4 : ((DP == 8) ? r / 8 : 0);
 
         part[j] = fmaf (acc[t][rt][r], acc[t][rt][r], part[j]);
       }
@@ -588,19 +626,15 @@ group_step (const GroupData<DP> &cur, const half8 (&Bh)[NPT][Plan<DP>::NKB], con
     }
     else if (DP == 8)
     {
+      // whole kernels per lane: slots 2 g + q
 #pragma unroll
       for (int q = 0; q < 2; ++q)
-      {
-        const float tot = halfwave_pair_sum (part[2 * q], part[2 * q + 1]);
-
-        lse_push<KERNEL, OFFSET> (tot * sc2, cur.cf[q], kappa, M0[t], Mb[t], S32[t]);
-      }
+        lse_push<KERNEL, OFFSET> (part[q] * sc2, cur.cf[q], kappa, M0[t], Mb[t], S32[t]);
     }
     else if (DP == 16)
     {
-      const float tot = halfwave_pair_sum (part[0], part[1]);
-
-      lse_push<KERNEL, OFFSET> (tot * sc2, cur.cf[0], kappa, M0[t], Mb[t], S32[t]);
+      // one whole kernel per lane: slot g
+      lse_push<KERNEL, OFFSET> (part[0] * sc2, cur.cf[0], kappa, M0[t], Mb[t], S32[t]);
     }
     else
     {
