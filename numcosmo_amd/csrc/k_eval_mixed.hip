@@ -177,20 +177,24 @@ launch_mixed_points (const MixedArgs &a, hipStream_t s)
 }
 
 // --------------------------------------------------------------------------
-// Kernel factors (FP64 -> split), one lane per kernel slot (padded to a whole
+// Kernel factors (FP64 -> split), one wave per kernel slot (padded to a whole
 // number of tiles; padding kernels get A = 0, c = 0, b2 = -inf).
 //   A (half): [n_pad][2][DP][DP] (hi, lo);  c (float): [n_pad][DP];
 //   coef (float4): (q_i or r_i, b2_i, 1/s_i^2, 0).
 // Wp holds W_i = U_i^{-T} packed by rows with padded dimension D (VKDE; NULL
 // for KDE, whose kernels all share U_g); Yc holds y'_i = U_g^{-T} (theta_i - mu).
+// Lanes take the entries of the DP x DP block in row-major order (coalesced
+// half stores); lane r < d sums row r of c serially in q, so every value is
+// the one a serial loop over (r, q) gives.
 // --------------------------------------------------------------------------
-__global__ void
+__global__ void __launch_bounds__ (256)
 k_mixed_factors (const double *__restrict__ Wp, int D, const double *__restrict__ Yc, const double *__restrict__ Ug,
                  const double *__restrict__ b, double bref, int n, int n_pad, int d, int DP, double inv_h,
                  double ysc, int kernel, double nu, _Float16 *__restrict__ A, float *__restrict__ c,
                  float4 *__restrict__ coef)
 {
-  const long i = blockIdx.x * (long) blockDim.x + threadIdx.x;
+  const long i   = blockIdx.x * 4L + (threadIdx.x >> 6);   // wave-uniform
+  const int lane = threadIdx.x & 63;
 
   if (i >= n_pad)
     return;
@@ -201,16 +205,17 @@ k_mixed_factors (const double *__restrict__ Wp, int D, const double *__restrict_
 
   if (i >= n)
   {
-    for (int e = 0; e < DP * DP; ++e)
+    for (int e = lane; e < DP * DP; e += 64)
     {
       Ah[e] = (_Float16) 0.0f;
       Al[e] = (_Float16) 0.0f;
     }
 
-    for (int r = 0; r < DP; ++r)
-      ci[r] = 0.0f;
+    if (lane < DP)
+      ci[lane] = 0.0f;
 
-    coef[i] = make_float4 (0.0f, -INFINITY, 0.0f, 0.0f);
+    if (lane == 0)
+      coef[i] = make_float4 (0.0f, -INFINITY, 0.0f, 0.0f);
 
     return;
   }
@@ -232,44 +237,49 @@ k_mixed_factors (const double *__restrict__ Wp, int D, const double *__restrict_
   };
   double bmax = 0.0;
 
-  for (int r = 0; r < d; ++r)
-    for (int q = 0; q < d; ++q)
-      bmax = fmax (bmax, fabs (bval (r, q) * inv_h));
+  for (int e = lane; e < d * d; e += 64)
+    bmax = fmax (bmax, fabs (bval (e / d, e % d) * inv_h));
+
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1)
+    bmax = fmax (bmax, __shfl_xor (bmax, o, 64));
 
   // s_i / ysc brings max |A_i| into [2^13, 2^14)
   const int e      = (bmax > 0.0) ? ilogb (bmax / ysc) : 0;
   const double s_i = ldexp (1.0, 13 - e);
   const double sy  = s_i / ysc;
 
-  for (int r = 0; r < DP; ++r)
+  for (int f = lane; f < DP * DP; f += 64)
+  {
+    const int r    = f / DP;
+    const int q    = f % DP;
+    const double v = (r < d && q < d) ? bval (r, q) * inv_h * sy : 0.0;
+    const _Float16 hi = (_Float16) v;
+    const _Float16 lo = (_Float16) (v - (double) hi);
+
+    Ah[f] = hi;
+    Al[f] = lo;
+  }
+
+  if (lane < DP)
   {
     double cr = 0.0;
 
-    for (int q = 0; q < DP; ++q)
-    {
-      double v = 0.0;
-
-      if (r < d && q < d)
-      {
-        v  = bval (r, q) * inv_h * sy;
-        cr = fma (v, yi[q] * ysc, cr);
-      }
-
-      const _Float16 hi = (_Float16) v;
-      const _Float16 lo = (_Float16) (v - (double) hi);
-
-      Ah[r * DP + q] = hi;
-      Al[r * DP + q] = lo;
-    }
+    if (lane < d)
+      for (int q = 0; q < d; ++q)
+        cr = fma (bval (lane, q) * inv_h * sy, yi[q] * ysc, cr);
 
     // stored negated: the MFMA accumulator starts from -B y_i directly
-    ci[r] = (float) -cr;
+    ci[lane] = (float) -cr;
   }
 
-  const double inv_s2 = 1.0 / (s_i * s_i);
-  const double qr     = (kernel == NCKDE_KERNEL_GAUSS) ? -0.5 * LOG2E * inv_s2 : inv_s2 / nu;
+  if (lane == 0)
+  {
+    const double inv_s2 = 1.0 / (s_i * s_i);
+    const double qr     = (kernel == NCKDE_KERNEL_GAUSS) ? -0.5 * LOG2E * inv_s2 : inv_s2 / nu;
 
-  coef[i] = make_float4 ((float) qr, (float) ((b[i] - bref) * LOG2E), (float) inv_s2, 0.0f);
+    coef[i] = make_float4 ((float) qr, (float) ((b[i] - bref) * LOG2E), (float) inv_s2, 0.0f);
+  }
 }
 
 // b2_i refresh after the weights change
@@ -1074,7 +1084,7 @@ mixed_build (const double *X, int n, int d, int D, int DP, const double *Wp, con
   const int n_pad  = ((n + kpt - 1) / kpt) * kpt;
 
   *ysc_out = ysc;
-  hipLaunchKernelGGL (k_mixed_factors, dim3 ((n_pad + 63) / 64), dim3 (64), 0, s, Wp, D, Yc, Ug, b, bref, n, n_pad,
+  hipLaunchKernelGGL (k_mixed_factors, dim3 ((n_pad + 3) / 4), dim3 (256), 0, s, Wp, D, Yc, Ug, b, bref, n, n_pad,
                       d, DP, inv_h, ysc, kernel, nu, A, c, (float4 *) coef);
 
   return hipGetLastError ();
