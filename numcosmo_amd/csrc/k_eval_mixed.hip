@@ -668,8 +668,13 @@ group_step (const GroupData<DP> &cur, const half8 (&Bh)[NPT][Plan<DP>::NKB], con
 
 // Tiling chosen by measurement on MI355X: up to DP = 16 a wave keeps more
 // point tiles and loads each kernel group just before its MFMA chains (the
-// occupancy of the smaller register set hides the loads); at DP >= 32 the next
-// group is prefetched into a second register set.
+// occupancy of the smaller register set hides the loads).  At DP = 32 the
+// next kernel is prefetched into a second register set.  At DP = 64 one
+// kernel's factors (16 KiB) feed all four waves of a block: they are staged
+// once per block in LDS, double-buffered, the next kernel's global loads in
+// flight while the current one is computed (1.5x faster than the register
+// prefetch at d = 64; at d = 32 the barrier per kernel costs more than the
+// shared loads save).
 constexpr int
 tiles_per_wave (int DP)
 {
@@ -677,9 +682,108 @@ tiles_per_wave (int DP)
 }
 
 constexpr bool
-prefetch_groups (int DP)
+lds_groups (int DP)
 {
-  return DP >= 32;
+  return DP == 64;
+}
+
+// LDS stage of one kernel (DP = 64): A hi rows then lo rows, each padded to
+// DP + 8 halves (16 B) so that the 32 rows read by a fragment load spread
+// over the banks; then c (DP floats) and the coefficients.
+template <int DP>
+struct Stage
+{
+  static constexpr int RS  = DP + 8;                  // row stride, halves
+  static constexpr int ASZ = 2 * DP * RS;             // halves per stage
+  static constexpr int CPT = 2 * DP * DP / 8 / 256;   // 16 B chunks per thread
+};
+
+typedef unsigned int u32x4 __attribute__ ((ext_vector_type (4)));
+typedef float f32x4 __attribute__ ((ext_vector_type (4)));
+
+// staging registers (plain vector types, so they stay in registers)
+template <int DP>
+struct StageFetch
+{
+  u32x4 a[(Stage<DP>::CPT > 0) ? Stage<DP>::CPT : 1];
+  f32x4 c, f;
+};
+
+template <int DP>
+__device__ inline void
+stage_load (StageFetch<DP> &s, const _Float16 *__restrict__ A, const float *__restrict__ cv,
+            const float4 *__restrict__ coef, int tg, int tid)
+{
+  const u32x4 *src = (const u32x4 *) (A + (size_t) tg * 2 * DP * DP);
+
+#pragma unroll
+  for (int q = 0; q < Stage<DP>::CPT; ++q)
+    s.a[q] = src[tid + q * 256];
+
+  // every thread loads (in bounds) so the stage stays in registers
+  s.c = ((const f32x4 *) (cv + (size_t) tg * DP))[tid % (DP / 4)];
+  s.f = ((const f32x4 *) coef)[tg];
+}
+
+template <int DP>
+__device__ inline void
+stage_store (const StageFetch<DP> &s, _Float16 *sA, float *sC, float4 *sF, int tid)
+{
+  constexpr int CPR = DP / 8;   // chunks per row
+
+#pragma unroll
+  for (int q = 0; q < Stage<DP>::CPT; ++q)
+  {
+    const int e = tid + q * 256;
+
+    *(u32x4 *) (sA + (e / CPR) * Stage<DP>::RS + (e % CPR) * 8) = s.a[q];
+  }
+
+  if (tid < DP / 4)
+    ((f32x4 *) sC)[tid] = s.c;
+
+  if (tid == 0)
+    *(f32x4 *) sF = s.f;
+}
+
+// the fragments of a staged kernel, as load_group reads them from global memory
+template <int DP>
+__device__ inline void
+load_group_lds (GroupData<DP> &G, const _Float16 *sA, const float *sC, const float4 *sF, int lane)
+{
+  using P            = Plan<DP>;
+  constexpr int RS   = Stage<DP>::RS;
+  const unsigned g   = lane >> 5;
+  const unsigned R   = lane & 31;
+
+#pragma unroll
+  for (int rt = 0; rt < P::NRT; ++rt)
+  {
+    const unsigned row = (DP == 64) ? rt * 32 + R : R;
+
+#pragma unroll
+    for (int kb = 0; kb < P::NKB; ++kb)
+    {
+      G.Ah[rt][kb] = *(const half8 *) (sA + row * RS + kb * 16 + g * 8);
+      G.Al[rt][kb] = *(const half8 *) (sA + (DP + row) * RS + kb * 16 + g * 8);
+    }
+
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+    {
+      const unsigned Rr = 8 * j + 4 * g;
+      const float4 c4   = *(const float4 *) (sC + ((DP == 64) ? rt * 32 + Rr : Rr));
+
+      G.cinit[rt][4 * j + 0] = c4.x;
+      G.cinit[rt][4 * j + 1] = c4.y;
+      G.cinit[rt][4 * j + 2] = c4.z;
+      G.cinit[rt][4 * j + 3] = c4.w;
+    }
+  }
+
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    G.cf[q] = *sF;
 }
 
 template <int DP, int KERNEL, int NPT, bool OFFSET, bool SCALED, int NST>
@@ -687,7 +791,8 @@ __device__ inline void
 group_loop (int g0, int g1, const _Float16 *__restrict__ A, const float *__restrict__ cv,
             const float4 *__restrict__ coef, const half8 (&Bh)[NPT][Plan<DP>::NKB],
             const half8 (&Bl)[NPT][Plan<DP>::NKB], const float (&tscale)[NPT], const float (&it2)[NPT],
-            const float (&M0)[NST], float (&Mb)[NST], double (&S64)[NST], float kappa, int lane)
+            const float (&M0)[NST], float (&Mb)[NST], double (&S64)[NST], float kappa, int lane,
+            _Float16 *sA, float *sC, float4 *sF)
 {
   const int g = lane >> 5;
   float S32[NST];
@@ -696,9 +801,7 @@ group_loop (int g0, int g1, const _Float16 *__restrict__ A, const float *__restr
   for (int q = 0; q < NST; ++q)
     S32[q] = 0.0f;
 
-  // two register sets alternate roles: while one group is computed the next
-  // one is loaded into the other set (no register copies)
-  if (!prefetch_groups (DP))
+  if constexpr (DP <= 16)
   {
     for (int tg = g0; tg < g1; ++tg)
     {
@@ -722,35 +825,88 @@ group_loop (int g0, int g1, const _Float16 *__restrict__ A, const float *__restr
 #pragma unroll
     for (int q = 0; q < NST; ++q)
       S64[q] += (double) S32[q];
+  }
+  else if constexpr (!lds_groups (DP))
+  {
+    // two register sets alternate roles: while one group is computed the
+    // next one is loaded into the other set (no register copies)
+    GroupData<DP> ga, gb;
+    int tg = g0;
 
-    return;
+    if (tg < g1)
+      load_group<DP> (ga, A, cv, coef, tg, lane);
+
+    while (tg < g1)
+    {
+      if (tg + 1 < g1)
+        load_group<DP> (gb, A, cv, coef, tg + 1, lane);
+
+      group_step<DP, KERNEL, NPT, OFFSET, SCALED, NST> (ga, Bh, Bl, tscale, it2, M0, Mb, S32, kappa, g);
+      ++tg;
+
+      if (tg >= g1)
+        break;
+
+      if (tg + 1 < g1)
+        load_group<DP> (ga, A, cv, coef, tg + 1, lane);
+
+      group_step<DP, KERNEL, NPT, OFFSET, SCALED, NST> (gb, Bh, Bl, tscale, it2, M0, Mb, S32, kappa, g);
+      ++tg;
+
+      // fold the FP32 partial sums into FP64 every 16 groups
+      if (((tg - g0) & 15) == 0)
+      {
+#pragma unroll
+        for (int q = 0; q < NST; ++q)
+        {
+          S64[q] += (double) S32[q];
+          S32[q] = 0.0f;
+        }
+      }
+    }
+
+#pragma unroll
+    for (int q = 0; q < NST; ++q)
+      S64[q] += (double) S32[q];
+  }
+  else
+  {
+  // staged groups: g0 and g1 are block-uniform, so every wave of the block
+  // runs the same iterations and meets the same barriers.  Stage st is read
+  // in iteration tg while the other one is written; the barrier at the end of
+  // each iteration orders those reads before the next overwrite.
+  const int tid = threadIdx.x;
+  StageFetch<DP> nx;
+
+  if (g0 < g1)
+  {
+    stage_load<DP> (nx, A, cv, coef, g0, tid);
+    stage_store<DP> (nx, sA, sC, sF, tid);
   }
 
-  GroupData<DP> ga, gb;
-  int tg = g0;
+  __syncthreads ();
 
-  if (tg < g1)
-    load_group<DP> (ga, A, cv, coef, tg, lane);
-
-  while (tg < g1)
+  for (int tg = g0; tg < g1; ++tg)
   {
-    if (tg + 1 < g1)
-      load_group<DP> (gb, A, cv, coef, tg + 1, lane);
-
-    group_step<DP, KERNEL, NPT, OFFSET, SCALED, NST> (ga, Bh, Bl, tscale, it2, M0, Mb, S32, kappa, g);
-    ++tg;
-
-    if (tg >= g1)
-      break;
+    const int st = (tg - g0) & 1;
 
     if (tg + 1 < g1)
-      load_group<DP> (ga, A, cv, coef, tg + 1, lane);
+      stage_load<DP> (nx, A, cv, coef, tg + 1, tid);
 
-    group_step<DP, KERNEL, NPT, OFFSET, SCALED, NST> (gb, Bh, Bl, tscale, it2, M0, Mb, S32, kappa, g);
-    ++tg;
+    {
+      GroupData<DP> gd;
+
+      load_group_lds<DP> (gd, sA + st * Stage<DP>::ASZ, sC + st * DP, sF + st, lane);
+      group_step<DP, KERNEL, NPT, OFFSET, SCALED, NST> (gd, Bh, Bl, tscale, it2, M0, Mb, S32, kappa, g);
+    }
+
+    if (tg + 1 < g1)
+      stage_store<DP> (nx, sA + (st ^ 1) * Stage<DP>::ASZ, sC + (st ^ 1) * DP, sF + (st ^ 1), tid);
+
+    __syncthreads ();
 
     // fold the FP32 partial sums into FP64 every 16 groups
-    if (((tg - g0) & 15) == 0)
+    if (((tg + 1 - g0) & 15) == 0)
     {
 #pragma unroll
       for (int q = 0; q < NST; ++q)
@@ -764,6 +920,7 @@ group_loop (int g0, int g1, const _Float16 *__restrict__ A, const float *__restr
 #pragma unroll
   for (int q = 0; q < NST; ++q)
     S64[q] += (double) S32[q];
+  }
 }
 
 template <int DP, int KERNEL, int NPT, bool OFFSET>
@@ -811,7 +968,8 @@ k_eval_mixed (MixedArgs a)
         any |= a.flag[p];
     }
 
-    if (!__any (any))
+    // staged kernels keep the whole block (its barriers need every wave)
+    if (lds_groups (DP) ? !__syncthreads_or (any) : !__any (any))
       return;
 
 #pragma unroll
@@ -860,12 +1018,20 @@ k_eval_mixed (MixedArgs a)
     S64[q] = 0.0;
   }
 
-  if (__any (scaled))
+  // LDS stages of the factors (DP >= 32), two per block
+  __shared__ __attribute__ ((aligned (16))) _Float16 sA[lds_groups (DP) ? 2 * Stage<DP>::ASZ : 8];
+  __shared__ __attribute__ ((aligned (16))) float sC[lds_groups (DP) ? 2 * DP : 4];
+  __shared__ float4 sF[2];
+
+  // the scaled/unscaled choice is block-wide where the loop has barriers
+  const bool any_scaled = lds_groups (DP) ? (bool) __syncthreads_or (scaled) : (bool) __any (scaled);
+
+  if (any_scaled)
     group_loop<DP, KERNEL, NPT, OFFSET, true, NST> (g0, g1, A, cv, coef, Bh, Bl, tscale, it2, M0, Mb, S64, kappa,
-                                                    lane);
+                                                    lane, sA, sC, sF);
   else
     group_loop<DP, KERNEL, NPT, OFFSET, false, NST> (g0, g1, A, cv, coef, Bh, Bl, tscale, it2, M0, Mb, S64, kappa,
-                                                     lane);
+                                                     lane, sA, sC, sF);
 
   if (DP >= 32)
   {
