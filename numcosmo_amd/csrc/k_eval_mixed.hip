@@ -347,7 +347,7 @@ k_colmean (const double *__restrict__ X, int n, int d, double *__restrict__ mu)
 }
 
 // --------------------------------------------------------------------------
-// The evaluation kernel.  256 threads = 4 waves; each wave owns NPT tiles of
+// The evaluation kernel.  4 waves per block (8 at DP = 64); each wave owns NPT tiles of
 // 32 points whose B fragments stay in registers; every wave of the grid walks
 // a range of kernel groups.
 // --------------------------------------------------------------------------
@@ -687,6 +687,13 @@ lds_groups (int DP)
   return DP == 64;
 }
 
+// waves per block: at DP = 64 eight waves share each staged kernel
+constexpr int
+waves_per_block (int DP)
+{
+  return (DP == 64) ? 8 : 4;
+}
+
 // LDS stage of one kernel (DP = 64): A hi rows then lo rows, each padded to
 // DP + 8 halves (16 B) so that the 32 rows read by a fragment load spread
 // over the banks; then c (DP floats) and the coefficients.
@@ -695,7 +702,8 @@ struct Stage
 {
   static constexpr int RS  = DP + 8;                  // row stride, halves
   static constexpr int ASZ = 2 * DP * RS;             // halves per stage
-  static constexpr int CPT = 2 * DP * DP / 8 / 256;   // 16 B chunks per thread
+  static constexpr int NT  = 64 * waves_per_block (DP);   // threads per block
+  static constexpr int CPT = 2 * DP * DP / 8 / NT;        // 16 B chunks per thread
 };
 
 typedef unsigned int u32x4 __attribute__ ((ext_vector_type (4)));
@@ -718,7 +726,7 @@ stage_load (StageFetch<DP> &s, const _Float16 *__restrict__ A, const float *__re
 
 #pragma unroll
   for (int q = 0; q < Stage<DP>::CPT; ++q)
-    s.a[q] = src[tid + q * 256];
+    s.a[q] = src[tid + q * Stage<DP>::NT];
 
   // every thread loads (in bounds) so the stage stays in registers
   s.c = ((const f32x4 *) (cv + (size_t) tg * DP))[tid % (DP / 4)];
@@ -734,7 +742,7 @@ stage_store (const StageFetch<DP> &s, _Float16 *sA, float *sC, float4 *sF, int t
 #pragma unroll
   for (int q = 0; q < Stage<DP>::CPT; ++q)
   {
-    const int e = tid + q * 256;
+    const int e = tid + q * Stage<DP>::NT;
 
     *(u32x4 *) (sA + (e / CPR) * Stage<DP>::RS + (e % CPR) * 8) = s.a[q];
   }
@@ -924,7 +932,7 @@ group_loop (int g0, int g1, const _Float16 *__restrict__ A, const float *__restr
 }
 
 template <int DP, int KERNEL, int NPT, bool OFFSET>
-__global__ void __launch_bounds__ (256)
+__global__ void __launch_bounds__ (64 * waves_per_block (DP))
 k_eval_mixed (MixedArgs a)
 {
   using P        = Plan<DP>;
@@ -942,7 +950,7 @@ k_eval_mixed (MixedArgs a)
   const float4 *coef = (const float4 *) a.f.coef;
   const float kappa  = (float) a.kappa;
 
-  const int tile0   = (blockIdx.x * 4 + wave) * NPT;
+  const int tile0   = (blockIdx.x * waves_per_block (DP) + wave) * NPT;
   const int ngroups = (a.n + P::KPT - 1) / P::KPT;
   const int chunk   = (ngroups + a.nsplit - 1) / a.nsplit;
   const int g0      = blockIdx.y * chunk;
@@ -1131,10 +1139,10 @@ static hipError_t
 eval_t (const MixedArgs &a, hipStream_t s)
 {
   constexpr int NPT = tiles_per_wave (DP);
-  const int pts     = 4 * NPT * 32;
+  const int pts     = waves_per_block (DP) * NPT * 32;
   dim3 grid ((a.m + pts - 1) / pts, a.nsplit);
 
-  hipLaunchKernelGGL ((k_eval_mixed<DP, KERNEL, NPT, OFFSET>), grid, dim3 (256), 0, s, a);
+  hipLaunchKernelGGL ((k_eval_mixed<DP, KERNEL, NPT, OFFSET>), grid, dim3 (64 * waves_per_block (DP)), 0, s, a);
 
   return hipGetLastError ();
 }
@@ -1184,7 +1192,7 @@ eval_d (const MixedArgs &a, hipStream_t s)
 int
 mixed_points_per_block (int DP)
 {
-  return 4 * tiles_per_wave (DP) * 32;
+  return waves_per_block (DP) * tiles_per_wave (DP) * 32;
 }
 
 hipError_t
