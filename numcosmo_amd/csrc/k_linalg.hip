@@ -838,6 +838,7 @@ k_potrf_diag64 (const double *__restrict__ S, int np, int kb, int *__restrict__ 
     return;
 
   double a[CH_NB];
+  double rdl = 0.0;   // lane j keeps 1 / u_jj
 
 #pragma unroll
   for (int r = 0; r < CH_NB; ++r)
@@ -856,9 +857,14 @@ k_potrf_diag64 (const double *__restrict__ S, int np, int kb, int *__restrict__ 
       return;
     }
 
-    const double ujj = sqrt (djj);
+    // the pivot and its reciprocal are wave-uniform: row j is scaled by a
+    // multiplication (no per-lane division, no branch), and the reciprocal
+    // is the one the panel solves use
+    const double ujj  = sqrt (djj);
+    const double rinv = 1.0 / ujj;
 
-    a[j] = (lane == j) ? ujj : a[j] / ujj;
+    a[j] = (lane == j) ? ujj : a[j] * rinv;
+    rdl  = (lane == j) ? rinv : rdl;
     R[lane] = a[j];
     __builtin_amdgcn_fence (__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier ();
@@ -873,13 +879,10 @@ k_potrf_diag64 (const double *__restrict__ S, int np, int kb, int *__restrict__ 
 
 #pragma unroll
   for (int r = 0; r < CH_NB; ++r)
-  {
     if (r <= lane)
       Ud[r * CH_NB + lane] = a[r];
 
-    if (r == lane)
-      rd[lane] = 1.0 / a[r];
-  }
+  rd[lane] = rdl;
 }
 
 // U_kk^T x = S[kb:kb+64][c] for the columns c of the row panel by forward
