@@ -870,9 +870,19 @@ k_potrf_diag64 (const double *__restrict__ S, int np, int kb, int *__restrict__ 
     __builtin_amdgcn_wave_barrier ();
     __builtin_amdgcn_fence (__ATOMIC_ACQUIRE, "wavefront");
 
+    // all reads of row j first, then the updates: one wait for the whole
+    // row instead of one per read
+    double rr[CH_NB];
+
 #pragma unroll
     for (int i = j + 1; i < CH_NB; ++i)
-      a[i] = fma (-R[i], a[j], a[i]);
+      rr[i] = R[i];
+
+    __builtin_amdgcn_sched_barrier (0);
+
+#pragma unroll
+    for (int i = j + 1; i < CH_NB; ++i)
+      a[i] = fma (-rr[i], a[j], a[i]);
 
     __builtin_amdgcn_wave_barrier ();
   }
