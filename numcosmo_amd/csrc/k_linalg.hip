@@ -1110,16 +1110,40 @@ k_trsv_panel (const double *__restrict__ S, int np, int p0, int pb, double *__re
     __builtin_amdgcn_wave_barrier ();
     __builtin_amdgcn_fence (__ATOMIC_ACQUIRE, "wavefront");
 
-    for (int step = 0; step < pb; ++step)
+    if (pb == TS_NB)
     {
-      const int j     = FWD ? step : pb - 1 - step;
-      const double bj = bcast_lane (bi, j);
-      const double yj = bj * rdiag[j];
+      // full panel: the lane's row of T in registers, zero outside the
+      // triangle being solved, so each step is a readlane of lane j's
+      // scaled value and one fma or select (the same arithmetic as below)
+      const double rdl = rdiag[i];
+      double t[TS_NB];
 
-      if (i == j)
-        bi = yj;
-      else if (FWD ? (i > j) : (i < j))
-        bi = fma (-T[i][j], yj, bi);
+#pragma unroll
+      for (int j = 0; j < TS_NB; ++j)
+        t[j] = (FWD ? (j < i) : (j > i)) ? T[i][j] : 0.0;
+
+#pragma unroll
+      for (int step = 0; step < TS_NB; ++step)
+      {
+        const int j     = FWD ? step : TS_NB - 1 - step;
+        const double yj = bcast_lane (bi * rdl, j);
+
+        bi = (i == j) ? yj : fma (-t[j], yj, bi);
+      }
+    }
+    else
+    {
+      for (int step = 0; step < pb; ++step)
+      {
+        const int j     = FWD ? step : pb - 1 - step;
+        const double bj = bcast_lane (bi, j);
+        const double yj = bj * rdiag[j];
+
+        if (i == j)
+          bi = yj;
+        else if (FWD ? (i > j) : (i < j))
+          bi = fma (-T[i][j], yj, bi);
+      }
     }
 
     y[i] = bi;
