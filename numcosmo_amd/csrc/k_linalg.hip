@@ -820,18 +820,21 @@ bcast_lane (double v, int src)
 
 // Full panels (nb = 64) take two launches.
 // k_potrf_diag64 (one wave): the diagonal block, one column per lane in
-// registers.  Step j takes the pivot from lane j by readlane, scales row j
-// of U, publishes it through LDS (one write per lane, wide broadcast reads;
-// a single wave's LDS operations complete in order, so no barrier) and
-// applies the rank-one update.  Entries below the diagonal take part in the
-// update unmasked: they only ever combine with each other, are never read
-// as factor entries and are not stored.  The factor goes to Ud (upper
-// triangle, row-major 64 x 64) and the reciprocals of its diagonal to rd.
+// registers.  Row j of U is published through LDS (one write per lane, wide
+// broadcast reads; a single wave's LDS operations complete in order, so no
+// barrier) and applied as a rank-one update.  The next pivot is taken as
+// soon as its own column is updated: its readlane, square root and
+// reciprocal (all wave-uniform) then overlap the rest of the update, and the
+// scaled row goes to the other half of a double-buffered LDS row.  Entries
+// below the diagonal take part in the update unmasked: they only ever
+// combine with each other, are never read as factor entries and are not
+// stored.  The factor goes to Ud (upper triangle, row-major 64 x 64) and the
+// reciprocals of its diagonal to rd.
 __global__ void __launch_bounds__ (64)
 k_potrf_diag64 (const double *__restrict__ S, int np, int kb, int *__restrict__ info, double *__restrict__ Ud,
                 double *__restrict__ rd)
 {
-  __shared__ double R[CH_NB];
+  __shared__ double R[2][CH_NB];
   const int lane = threadIdx.x;
 
   if (*info != 0)
@@ -844,47 +847,63 @@ k_potrf_diag64 (const double *__restrict__ S, int np, int kb, int *__restrict__ 
   for (int r = 0; r < CH_NB; ++r)
     a[r] = (r <= lane) ? S[(size_t) (kb + r) * np + kb + lane] : 0.0;
 
-#pragma unroll
-  for (int j = 0; j < CH_NB; ++j)
   {
-    const double djj = bcast_lane (a[j], j);
+    const double d0 = bcast_lane (a[0], 0);
 
-    if (!(djj > 0.0))
+    if (!(d0 > 0.0))
     {
       if (lane == 0)
-        *info = kb + j + 1;
+        *info = kb + 1;
 
       return;
     }
 
-    // the pivot and its reciprocal are wave-uniform: row j is scaled by a
-    // multiplication (no per-lane division, no branch), and the reciprocal
-    // is the one the panel solves use
-    const double ujj  = sqrt (djj);
-    const double rinv = 1.0 / ujj;
+    const double u0 = sqrt (d0);
+    const double r0 = 1.0 / u0;
 
-    a[j] = (lane == j) ? ujj : a[j] * rinv;
-    rdl  = (lane == j) ? rinv : rdl;
-    R[lane] = a[j];
+    a[0]    = (lane == 0) ? u0 : a[0] * r0;
+    rdl     = (lane == 0) ? r0 : rdl;
+    R[0][lane] = a[0];
+  }
+
+#pragma unroll
+  for (int j = 0; j + 1 < CH_NB; ++j)
+  {
     __builtin_amdgcn_fence (__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier ();
     __builtin_amdgcn_fence (__ATOMIC_ACQUIRE, "wavefront");
 
-    // all reads of row j first, then the updates: one wait for the whole
-    // row instead of one per read
+    // row j (scaled) from LDS, all reads issued together
     double rr[CH_NB];
 
 #pragma unroll
     for (int i = j + 1; i < CH_NB; ++i)
-      rr[i] = R[i];
+      rr[i] = R[j & 1][i];
 
     __builtin_amdgcn_sched_barrier (0);
 
+    // column j + 1 first: its pivot is needed next
+    a[j + 1]        = fma (-rr[j + 1], a[j], a[j + 1]);
+    const double dn = bcast_lane (a[j + 1], j + 1);
+
+    if (!(dn > 0.0))
+    {
+      if (lane == 0)
+        *info = kb + j + 2;
+
+      return;
+    }
+
+    const double un = sqrt (dn);
+    const double rn = 1.0 / un;
+
 #pragma unroll
-    for (int i = j + 1; i < CH_NB; ++i)
+    for (int i = j + 2; i < CH_NB; ++i)
       a[i] = fma (-rr[i], a[j], a[i]);
 
-    __builtin_amdgcn_wave_barrier ();
+    a[j + 1] = (lane == j + 1) ? un : a[j + 1] * rn;
+    rdl      = (lane == j + 1) ? rn : rdl;
+    R[(j + 1) & 1][lane] = a[j + 1];
   }
 
 #pragma unroll
