@@ -189,10 +189,12 @@ launch_transpose (const double *in, int rows, int cols, double *out, hipStream_t
 
 // --------------------------------------------------------------------------
 // Symmetric rank-K products on the FP64 matrix cores (v_mfma_f64_16x16x4f64)
-// over 128 x 128 tiles of the upper triangle.  256 threads = 4 waves in a
-// 2 x 2 layout, each wave a 64 x 64 block of 4 x 4 MFMA tiles; K staged
-// through LDS in slices of 16, the next slice's global loads in flight
-// during the current slice's MFMAs.
+// over BM x BM tiles of the upper triangle.  256 threads = 4 waves in a
+// 2 x 2 layout, each wave a BM/2 x BM/2 block of 16 x 16 MFMA tiles; K
+// staged through LDS in slices of 16, the next slice's global loads in
+// flight during the current slice's MFMAs.  The Gram matrix (large K) uses
+// BM = 128; the Cholesky update (K = 64) uses BM = 64, whose 4x more blocks
+// and 4x shorter MFMA chain per block suit the shrinking trailing matrix.
 //   mode 0: M = AT AT^T (AT is n x K row-major): the Gram matrix A^T A of
 //           the NNLS normal equations (cblas_dsyrk, ncm_matrix.c:1547-1575);
 //           both triangles are written.
@@ -206,14 +208,17 @@ launch_transpose (const double *in, int rows, int cols, double *out, hipStream_t
 // --------------------------------------------------------------------------
 typedef double f64x4 __attribute__ ((ext_vector_type (4)));
 
-constexpr int SY_BM = 128;
+constexpr int SY_BM = 128;   // tile of the Gram matrix (MODE 0)
 constexpr int SY_BK = 16;
 
-template <int MODE>
+// BM: square tile of one block (128 or 64); every thread stages
+// VPT = 16 BM / 256 values of each operand per K slice, 16 threads per K row
+template <int MODE, int BM>
 __device__ inline void
-syrk_load (const double *__restrict__ P, int n, int K, int ld, int kb, int r0, int k0, double v[8])
+syrk_load (const double *__restrict__ P, int n, int K, int ld, int kb, int r0, int k0, double (&v)[BM / 16])
 {
-  const int tid = threadIdx.x;
+  constexpr int VPT = BM / 16;
+  const int tid     = threadIdx.x;
 
   if (MODE == 0)
   {
@@ -227,19 +232,20 @@ syrk_load (const double *__restrict__ P, int n, int K, int ld, int kb, int r0, i
   else
   {
     const int kq = k0 + tid / 16;
-    const int a  = r0 + (tid % 16) * 8;
+    const int a  = r0 + (tid % 16) * VPT;
 
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
+    for (int q = 0; q < VPT; ++q)
       v[q] = (a + q < n && kq < K) ? P[(size_t) (kb + kq) * ld + a + q] : 0.0;
   }
 }
 
-template <int MODE>
+template <int MODE, int BM>
 __device__ inline void
-syrk_store (double (*T)[SY_BM + 4], const double v[8])
+syrk_store (double (*T)[BM + 4], const double (&v)[BM / 16])
 {
-  const int tid = threadIdx.x;
+  constexpr int VPT = BM / 16;
+  const int tid     = threadIdx.x;
 
   if (MODE == 0)
   {
@@ -251,21 +257,25 @@ syrk_store (double (*T)[SY_BM + 4], const double v[8])
   }
   else
   {
-    const int kk = tid / 16, rr = (tid % 16) * 8;
+    const int kk = tid / 16, rr = (tid % 16) * VPT;
 
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
+    for (int q = 0; q < VPT; ++q)
       T[kk][rr + q] = v[q];
   }
 }
 
-template <int MODE>
+template <int MODE, int BM>
 __global__ void __launch_bounds__ (256)
 k_syrk_mfma (const double *__restrict__ P, int n, int K, int ld, int off, int kb, double *__restrict__ M,
              int ntiles)
 {
-  __shared__ double As[SY_BK][SY_BM + 4];
-  __shared__ double Bs[SY_BK][SY_BM + 4];
+  static_assert (BM == 128 || (BM == 64 && MODE == 1), "the Gram matrix uses 128-wide tiles");
+  constexpr int WM = BM / 2;    // tile of one wave
+  constexpr int NI = WM / 16;   // 16 x 16 MFMA blocks per wave dimension
+
+  __shared__ double As[SY_BK][BM + 4];
+  __shared__ double Bs[SY_BK][BM + 4];
 
   const int per = gridDim.x / 8;
   int t         = (blockIdx.x % 8) * per + blockIdx.x / 8;
@@ -273,7 +283,7 @@ k_syrk_mfma (const double *__restrict__ P, int n, int K, int ld, int off, int kb
   if (t >= ntiles)
     return;
 
-  const int nt = (n - off + SY_BM - 1) / SY_BM;
+  const int nt = (n - off + BM - 1) / BM;
   int ti       = 0;
 
   while (t >= nt - ti)
@@ -283,54 +293,54 @@ k_syrk_mfma (const double *__restrict__ P, int n, int K, int ld, int off, int kb
   }
 
   const int tj   = ti + t;
-  const int r0   = off + ti * SY_BM, c0 = off + tj * SY_BM;
+  const int r0   = off + ti * BM, c0 = off + tj * BM;
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
-  const int wr   = (w >> 1) * 64, wc = (w & 1) * 64;
+  const int wr   = (w >> 1) * WM, wc = (w & 1) * WM;
   const int li   = lane % 16, lk = lane / 16;
 
-  f64x4 acc[4][4];
+  f64x4 acc[NI][NI];
 
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < NI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < NI; ++j)
       acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-  double va[8], vb[8];
+  double va[BM / 16], vb[BM / 16];
 
-  syrk_load<MODE> (P, n, K, ld, kb, r0, 0, va);
-  syrk_load<MODE> (P, n, K, ld, kb, c0, 0, vb);
+  syrk_load<MODE, BM> (P, n, K, ld, kb, r0, 0, va);
+  syrk_load<MODE, BM> (P, n, K, ld, kb, c0, 0, vb);
 
   for (int k0 = 0; k0 < K; k0 += SY_BK)
   {
     __syncthreads ();
-    syrk_store<MODE> (As, va);
-    syrk_store<MODE> (Bs, vb);
+    syrk_store<MODE, BM> (As, va);
+    syrk_store<MODE, BM> (Bs, vb);
     __syncthreads ();
 
     if (k0 + SY_BK < K)
     {
-      syrk_load<MODE> (P, n, K, ld, kb, r0, k0 + SY_BK, va);
-      syrk_load<MODE> (P, n, K, ld, kb, c0, k0 + SY_BK, vb);
+      syrk_load<MODE, BM> (P, n, K, ld, kb, r0, k0 + SY_BK, va);
+      syrk_load<MODE, BM> (P, n, K, ld, kb, c0, k0 + SY_BK, vb);
     }
 
 #pragma unroll
     for (int ks = 0; ks < SY_BK / 4; ++ks)
     {
       const int kq = ks * 4 + lk;
-      double fa[4], fb[4];
+      double fa[NI], fb[NI];
 
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < NI; ++i)
       {
         fa[i] = As[kq][wr + i * 16 + li];
         fb[i] = Bs[kq][wc + i * 16 + li];
       }
 
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < NI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < NI; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64 (fa[i], fb[j], acc[i][j], 0, 0, 0);
     }
   }
@@ -339,9 +349,9 @@ k_syrk_mfma (const double *__restrict__ P, int n, int K, int ld, int off, int kb
   if (MODE == 0)
   {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < NI; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
         {
@@ -358,15 +368,15 @@ k_syrk_mfma (const double *__restrict__ P, int n, int K, int ld, int off, int kb
   else
   {
     // read-modify-write of the tile, one 16-column block at a time with all
-    // 16 loads of a lane in flight (out-of-range entries read element 0)
+    // loads of a lane in flight (out-of-range entries read element 0)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < NI; ++j)
     {
       const int b = c0 + wc + j * 16 + li;
-      double old[4][4];
+      double old[NI][4];
 
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
         {
@@ -377,7 +387,7 @@ k_syrk_mfma (const double *__restrict__ P, int n, int K, int ld, int off, int kb
         }
 
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
         {
@@ -390,16 +400,16 @@ k_syrk_mfma (const double *__restrict__ P, int n, int K, int ld, int off, int kb
   }
 }
 
-template <int MODE>
+template <int MODE, int BM = SY_BM>
 static void
 launch_syrk_mfma (const double *P, int n, int K, int ld, int off, int kb, double *M, hipStream_t s)
 {
-  const int nt     = (n - off + SY_BM - 1) / SY_BM;
+  const int nt     = (n - off + BM - 1) / BM;
   const int ntiles = nt * (nt + 1) / 2;
   const int grid   = (ntiles + 7) / 8 * 8;
 
   if (ntiles > 0)
-    hipLaunchKernelGGL (k_syrk_mfma<MODE>, dim3 (grid), dim3 (256), 0, s, P, n, K, ld, off, kb, M, ntiles);
+    hipLaunchKernelGGL ((k_syrk_mfma<MODE, BM>), dim3 (grid), dim3 (256), 0, s, P, n, K, ld, off, kb, M, ntiles);
 }
 
 hipError_t
@@ -1055,7 +1065,7 @@ launch_potrf (double *S, int np, int *info, double *diagbuf, hipStream_t s)
     }
 
     if (rest > 0)
-      launch_syrk_mfma<1> (S, np, nb, np, kb + nb, kb, S, s);
+      launch_syrk_mfma<1, 64> (S, np, nb, np, kb + nb, kb, S, s);
   }
 
   hipLaunchKernelGGL (k_copy_diag, dim3 ((np + CH_NB - 1) / CH_NB), dim3 (256), 0, s, S, np, diagbuf);
