@@ -192,9 +192,10 @@ launch_transpose (const double *in, int rows, int cols, double *out, hipStream_t
 // over BM x BM tiles of the upper triangle.  256 threads = 4 waves in a
 // 2 x 2 layout, each wave a BM/2 x BM/2 block of 16 x 16 MFMA tiles; K
 // staged through LDS in slices of 16, the next slice's global loads in
-// flight during the current slice's MFMAs.  The Gram matrix (large K) uses
-// BM = 128; the Cholesky update (K = 64) uses BM = 64, whose 4x more blocks
-// and 4x shorter MFMA chain per block suit the shrinking trailing matrix.
+// flight during the current slice's MFMAs.  The Gram matrix uses BM = 128
+// when that gives at least 1024 tiles and BM = 64 below; the Cholesky update
+// (K = 64) uses BM = 64, whose 4x more blocks and 4x shorter MFMA chain per
+// block suit the shrinking trailing matrix.
 //   mode 0: M = AT AT^T (AT is n x K row-major): the Gram matrix A^T A of
 //           the NNLS normal equations (cblas_dsyrk, ncm_matrix.c:1547-1575);
 //           both triangles are written.
@@ -222,11 +223,12 @@ syrk_load (const double *__restrict__ P, int n, int K, int ld, int kb, int r0, i
 
   if (MODE == 0)
   {
-    const int a  = r0 + tid / 2;
-    const int kq = k0 + (tid % 2) * 8;
+    // 256 / BM threads per row, VPT consecutive K entries each
+    const int a  = r0 + tid / (256 / BM);
+    const int kq = k0 + (tid % (256 / BM)) * VPT;
 
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
+    for (int q = 0; q < VPT; ++q)
       v[q] = (a < n && kq + q < K) ? P[(size_t) a * ld + kq + q] : 0.0;
   }
   else
@@ -249,10 +251,10 @@ syrk_store (double (*T)[BM + 4], const double (&v)[BM / 16])
 
   if (MODE == 0)
   {
-    const int rr = tid / 2, kk = (tid % 2) * 8;
+    const int rr = tid / (256 / BM), kk = (tid % (256 / BM)) * VPT;
 
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
+    for (int q = 0; q < VPT; ++q)
       T[kk + q][rr] = v[q];
   }
   else
@@ -270,7 +272,7 @@ __global__ void __launch_bounds__ (256)
 k_syrk_mfma (const double *__restrict__ P, int n, int K, int ld, int off, int kb, double *__restrict__ M,
              int ntiles)
 {
-  static_assert (BM == 128 || (BM == 64 && MODE == 1), "the Gram matrix uses 128-wide tiles");
+  static_assert (BM == 128 || BM == 64, "128- or 64-wide tiles");
   constexpr int WM = BM / 2;    // tile of one wave
   constexpr int NI = WM / 16;   // 16 x 16 MFMA blocks per wave dimension
 
@@ -415,7 +417,15 @@ launch_syrk_mfma (const double *P, int n, int K, int ld, int off, int kb, double
 hipError_t
 launch_gram_nt (const double *AT, int n, int K, double *M, hipStream_t s)
 {
-  launch_syrk_mfma<0> (AT, n, K, K, 0, 0, M, s);
+  // 128-wide tiles reuse each staged operand twice as often; below about
+  // four blocks per CU their tail leaves the chip half idle, and 64-wide
+  // tiles (4x the blocks) balance better
+  const int nt128 = (n + 127) / 128;
+
+  if (nt128 * (nt128 + 1) / 2 >= 1024)
+    launch_syrk_mfma<0, 128> (AT, n, K, K, 0, 0, M, s);
+  else
+    launch_syrk_mfma<0, 64> (AT, n, K, K, 0, 0, M, s);
 
   return hipGetLastError ();
 }
