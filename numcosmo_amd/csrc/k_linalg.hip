@@ -192,10 +192,11 @@ launch_transpose (const double *in, int rows, int cols, double *out, hipStream_t
 // over BM x BM tiles of the upper triangle.  256 threads = 4 waves in a
 // 2 x 2 layout, each wave a BM/2 x BM/2 block of 16 x 16 MFMA tiles; K
 // staged through LDS in slices of 16, the next slice's global loads in
-// flight during the current slice's MFMAs.  The Gram matrix uses BM = 128
-// when that gives at least 1024 tiles and BM = 64 below; the Cholesky update
-// (K = 64) uses BM = 64, whose 4x more blocks and 4x shorter MFMA chain per
-// block suit the shrinking trailing matrix.
+// flight during the current slice's MFMAs.  BM = 128 serves large products
+// (at least 1024 tiles for the Gram matrix, 16384 for the Cholesky update)
+// and BM = 64 the smaller ones: 4x more blocks, and for the Cholesky update
+// (K = 64) a 4x shorter MFMA chain per block, which suits the shrinking
+// trailing matrix.
 //   mode 0: M = AT AT^T (AT is n x K row-major): the Gram matrix A^T A of
 //           the NNLS normal equations (cblas_dsyrk, ncm_matrix.c:1547-1575);
 //           both triangles are written.
@@ -1075,7 +1076,17 @@ launch_potrf (double *S, int np, int *info, double *diagbuf, hipStream_t s)
     }
 
     if (rest > 0)
-      launch_syrk_mfma<1, 64> (S, np, nb, np, kb + nb, kb, S, s);
+    {
+      // 128-wide tiles while they number at least 16384 (trailing size above
+      // about 23000: more reuse per staged operand), 64-wide ones below
+      // (measured faster at n = 20000 and below)
+      const int nt128 = (rest + 127) / 128;
+
+      if (nt128 * (nt128 + 1) / 2 >= 16384)
+        launch_syrk_mfma<1, 128> (S, np, nb, np, kb + nb, kb, S, s);
+      else
+        launch_syrk_mfma<1, 64> (S, np, nb, np, kb + nb, kb, S, s);
+    }
   }
 
   hipLaunchKernelGGL (k_copy_diag, dim3 ((np + CH_NB - 1) / CH_NB), dim3 (256), 0, s, S, np, diagbuf);
