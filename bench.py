@@ -402,7 +402,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-north-star", action="store_true")
+    ap.add_argument("--no-d16", action="store_true", help="skip the d=16, n=20000 VKDE eval")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc passes")
     ap.add_argument("--apes-steps", type=int, default=40, help="APES ensemble steps (0: skip)")
@@ -437,8 +437,8 @@ def main(argv=None):
 
     main_case = run_case("C2", "banana", 8, 4000, 100000, 3.0, args.steps, args.warmup, True, 4000, 2, O, cores)
     ns = None
-    if not args.no_north_star:
-        ns = run_case("north-star", "gaussian", 16, 20000, 200000, 3.0, max(3, args.steps // 4), 1, False,
+    if not args.no_d16:
+        ns = run_case("d16", "gaussian", 16, 20000, 200000, 3.0, max(3, args.steps // 4), 1, False,
                       200, 4, O, cores)
 
     c5 = c5_case(1000000, 100000) if args.c5 else None
@@ -491,7 +491,7 @@ def main(argv=None):
         "fp64": main_case.get("fp64"),
         "mixed": main_case.get("mixed"),
         "prepare_interp_s": main_case["prepare_s"],
-        "north_star": ns,
+        "vkde_d16_n20000": ns,
         "latency": latency,
         "apes": apes,
         "c5": c5,
