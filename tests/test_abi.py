@@ -83,3 +83,17 @@ def test_request_struct_layout_matches_the_header(tmp_path):
     want = [C.sizeof(Request)] + [getattr(Request, f).offset for f in fields_r]
     want += [C.sizeof(RequestInfo)] + [getattr(RequestInfo, f).offset for f in fields_i]
     assert got == want
+
+
+def test_apes_entry_points_reject_null_handles():
+    """Handle checks of the APES walker run on the host: a NULL handle or
+    output pointer is an error with a message, never a dereference."""
+    from numcosmo_amd._lib import lib
+    L = lib()
+    rows = ctypes.c_size_t(7)
+    assert L.nckde_apes_set_trace(None, None, 0) == -1
+    assert b"NULL" in L.nckde_last_error()
+    assert L.nckde_apes_get_trace_rows(None, ctypes.byref(rows)) == -1
+    assert rows.value == 7
+    assert L.nckde_apes_run(None, 1, None, None, None) != 0
+    assert L.nckde_apes_destroy(None) == 0
