@@ -616,6 +616,7 @@ k_knn (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, uint64
   __shared__ int s_nlt;
   __shared__ int s_eq_taken;
   __shared__ int s_scan[KNN_THREADS];
+  __shared__ unsigned s_wsum[KNN_THREADS / 64];
 
   const int qb  = blockIdx.x;
   const int tid = threadIdx.x;
@@ -673,22 +674,44 @@ k_knn (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, uint64
 
     __syncthreads ();
 
-    if (tid == 0)
+    // the digit holding rank s_kk: the one bin whose exclusive prefix count
+    // is below the rank and whose inclusive count reaches it (a block scan
+    // of the 256 bins, one per thread: wave scans, then the wave totals)
     {
-      int kk       = s_kk;
-      unsigned cum = 0;
-      int dg       = 0;
+      const int lane    = tid & 63;
+      const int wave    = tid >> 6;
+      const unsigned h  = hist[tid];
+      unsigned v        = h;
 
-      for (dg = 0; dg < 256; ++dg)
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1)
       {
-        if (cum + hist[dg] >= (unsigned) kk)
-          break;
+        const unsigned t = __shfl_up (v, o, 64);
 
-        cum += hist[dg];
+        if (lane >= o)
+          v += t;
       }
 
-      s_kk     = kk - (int) cum;
-      s_prefix = prefix | ((uint64_t) dg << shift);
+      if (lane == 63)
+        s_wsum[wave] = v;
+
+      const int kk = s_kk;
+
+      __syncthreads ();
+
+      unsigned base = 0;
+
+      for (int w = 0; w < wave; ++w)
+        base += s_wsum[w];
+
+      const unsigned incl = base + v;
+      const unsigned excl = incl - h;
+
+      if (excl < (unsigned) kk && incl >= (unsigned) kk)
+      {
+        s_kk     = kk - (int) excl;
+        s_prefix = prefix | ((uint64_t) tid << shift);
+      }
     }
 
     mask |= (uint64_t) 255 << shift;
