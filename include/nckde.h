@@ -89,8 +89,23 @@ int nckde_set_cov_type (nckde *h, int cov_type);
 int nckde_set_cov_fixed (nckde *h, const double *cov);
 /* ncm_stats_dist_kde_set_nearPD_maxiter (ncm_stats_dist_kde.h:79) */
 int nckde_set_nearPD_maxiter (nckde *h, unsigned maxiter);
-/* ncm_stats_dist_set_use_threads (ncm_stats_dist.h:115): accepted, no effect */
+/* ncm_stats_dist_set_use_threads / get (ncm_stats_dist.h:115-116): stored;
+ * the device path does not depend on it */
 int nckde_set_use_threads (nckde *h, int use_threads);
+int nckde_get_use_threads (nckde *h, int *use_threads);
+/* ncm_stats_dist_get_shrink (ncm_stats_dist.h:107) */
+int nckde_get_shrink (nckde *h, double *shrink);
+/* ncm_stats_dist_vkde_get_local_frac (ncm_stats_dist_vkde.h:59) */
+int nckde_get_local_frac (nckde *h, double *local_frac);
+/* ncm_stats_dist_vkde_get_use_rot_href (ncm_stats_dist_vkde.h:62) */
+int nckde_get_use_rot_href (nckde *h, int *use_rot_href);
+/* ncm_stats_dist_kde_get_nearPD_maxiter (ncm_stats_dist_kde.h:80) */
+int nckde_get_nearPD_maxiter (nckde *h, unsigned *maxiter);
+/* ncm_stats_dist_kde_get_cov_type (ncm_stats_dist_kde.h:83) */
+int nckde_get_cov_type (nckde *h, int *cov_type);
+/* ncm_stats_dist_kde_peek_cov_fixed (ncm_stats_dist_kde.h:86): copies the d x d
+ * matrix; NCKDE_ESTATE when none was set (the reference returns NULL) */
+int nckde_get_cov_fixed (nckde *h, double *cov);
 /* ncm_stats_dist_get_over_smooth (ncm_stats_dist.h:101); the CV_SPLIT,
  * CV_SPLIT_NOFIT and CV_LOO fits change it */
 int nckde_get_over_smooth (nckde *h, double *over_smooth);
@@ -185,6 +200,11 @@ int nckde_get_weights (nckde *h, double *w);
  * upper triangle (cov = U^T U); the strict lower triangle holds what LAPACK
  * leaves there (the covariance's lower triangle). */
 int nckde_get_cov_decomp (nckde *h, size_t i, double *U);
+/* ncm_stats_dist_get_Ki (ncm_stats_dist.h:139, ncm_stats_dist.c:1810-1828):
+ * kernel i's centre y_i[d] (its sample row), covariance cov_i[d*d] =
+ * href^2 U_i^T U_i, normalisation n_i = exp(lnnorm_i) (lnnorm as
+ * nckde_get_lnnorm) and weight w_i.  Outputs are the caller's buffers. */
+int nckde_get_Ki (nckde *h, size_t i, double *y_i, double *cov_i, double *n_i, double *w_i);
 /* ncm_stats_dist_peek_full_cov (ncm_stats_dist.h:135) */
 int nckde_get_full_cov (nckde *h, double *cov);
 /* ncm_stats_dist_peek_full_cov_decomp (ncm_stats_dist.h:134) */
@@ -325,7 +345,8 @@ int nckde_server_create (nckde_server **out, unsigned slots, size_t hbm_budget);
 int nckde_server_destroy (nckde_server *s);
 /* queue a request; returns at once with its ticket */
 int nckde_server_submit (nckde_server *s, const nckde_request *req, uint64_t *ticket);
-/* block until the request is done; each ticket is waited once */
+/* block until the request is done; each ticket is waited once (a second wait
+ * on it, also from another thread, returns NCKDE_EINVAL) */
 int nckde_server_wait (nckde_server *s, uint64_t ticket, nckde_request_info *info);
 /* seconds since nckde_server_create (the clock of nckde_request_info) */
 double nckde_server_clock (nckde_server *s);

@@ -34,6 +34,13 @@ SIGNATURES = {
     "nckde_set_cov_fixed": (C.c_int, [_h, _dp]),
     "nckde_set_nearPD_maxiter": (C.c_int, [_h, C.c_uint]),
     "nckde_set_use_threads": (C.c_int, [_h, C.c_int]),
+    "nckde_get_use_threads": (C.c_int, [_h, C.POINTER(C.c_int)]),
+    "nckde_get_shrink": (C.c_int, [_h, C.POINTER(C.c_double)]),
+    "nckde_get_local_frac": (C.c_int, [_h, C.POINTER(C.c_double)]),
+    "nckde_get_use_rot_href": (C.c_int, [_h, C.POINTER(C.c_int)]),
+    "nckde_get_nearPD_maxiter": (C.c_int, [_h, C.POINTER(C.c_uint)]),
+    "nckde_get_cov_type": (C.c_int, [_h, C.POINTER(C.c_int)]),
+    "nckde_get_cov_fixed": (C.c_int, [_h, _dp]),
     "nckde_get_over_smooth": (C.c_int, [_h, C.POINTER(C.c_double)]),
     "nckde_set_split_frac": (C.c_int, [_h, C.c_double]),
     "nckde_get_split_frac": (C.c_int, [_h, C.POINTER(C.c_double)]),
@@ -62,6 +69,7 @@ SIGNATURES = {
     "nckde_get_lnnorm": (C.c_int, [_h, _sz, C.POINTER(C.c_double)]),
     "nckde_get_weights": (C.c_int, [_h, _dp]),
     "nckde_get_cov_decomp": (C.c_int, [_h, _sz, _dp]),
+    "nckde_get_Ki": (C.c_int, [_h, _sz, _dp, _dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "nckde_get_full_cov": (C.c_int, [_h, _dp]),
     "nckde_get_full_cov_decomp": (C.c_int, [_h, _dp]),
     "nckde_get_sample": (C.c_int, [_h, _dp]),

@@ -9,6 +9,7 @@
 #include "nckde_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -91,6 +92,7 @@ struct nckde
   unsigned cv_nobj        = 0;       // objective evaluations / NNLS fits of the last CV fit
   double cv_seconds       = 0.0;     // wall time of the last CV fit
   int use_rot_href        = 0;
+  bool use_threads        = false;   // "use-threads" property: stored, the device path ignores it
   int cov_type            = NCKDE_COV_SAMPLE;
   unsigned nearPD_maxiter = 200;
 
@@ -109,6 +111,7 @@ struct nckde
 
   hipStream_t stream      = nullptr;
   hipStream_t last_stream = nullptr;  // stream of the last device-form eval
+  hipEvent_t ready_ev     = nullptr;  // marks the handle stream's work for evals on other streams
   bool profiling          = false;    // events around the main eval kernel
   bool pev_recorded       = false;
   hipEvent_t pev[2]       = {nullptr, nullptr};
@@ -415,7 +418,13 @@ do_prepare (nckde *h)
 {
   const int d = h->d;
 
-  h->prepared  = false;
+  // a device-form eval still running on a caller's stream reads the factors
+  // this prepare overwrites
+  if (h->last_stream != nullptr && h->last_stream != h->stream)
+    HIPCHK (hipStreamSynchronize (h->last_stream));
+
+  h->last_stream = nullptr;
+  h->prepared    = false;
   h->n_obs     = h->n_samples;
   // the split cross-validations keep the first ceil(n split_frac) points as
   // kernels and the rest as held-out observations
@@ -955,11 +964,10 @@ ensure_mixed (nckde *h)
   return NCKDE_OK;
 }
 
+// ensure_mixed must have run (eval_device does it before ordering the streams)
 static int
 eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s)
 {
-  TRY (ensure_mixed (h));
-
   const int DP      = nck::mfma_dim (h->d);
   const int kpt     = (DP >= 32) ? 1 : 32 / DP;
   const int ngroups = (int) ((h->n_k + kpt - 1) / kpt);
@@ -1025,14 +1033,25 @@ eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, 
   if (m == 0)
     return NCKDE_OK;
 
-  if (precision == NCKDE_PREC_MIXED)
-    return eval_mixed (h, dX, m, dout, linear, s);
-
-  if (precision != NCKDE_PREC_FP64)
+  if (precision != NCKDE_PREC_FP64 && precision != NCKDE_PREC_MIXED)
   {
     set_error ("nckde: unknown precision %d", precision);
     return NCKDE_EINVAL;
   }
+
+  // the mixed factors are built (or their weights refreshed) on the handle's
+  // stream; an eval on another stream must not start before that work ends
+  if (precision == NCKDE_PREC_MIXED)
+    TRY (ensure_mixed (h));
+
+  if (s != h->stream)
+  {
+    HIPCHK (hipEventRecord (h->ready_ev, h->stream));
+    HIPCHK (hipStreamWaitEvent (s, h->ready_ev, 0));
+  }
+
+  if (precision == NCKDE_PREC_MIXED)
+    return eval_mixed (h, dX, m, dout, linear, s);
 
   const int nblk = (int) ((m + 255) / 256);
   int nsplit     = (2048 + nblk - 1) / nblk;
@@ -1534,6 +1553,12 @@ struct Reader
 
     return v;
   }
+
+  size_t
+  left () const
+  {
+    return ok ? len - pos : 0;
+  }
 };
 
 const char SER_MAGIC[8] = {'N', 'C', 'K', 'D', 'E', '0', '2', '\0'};
@@ -1601,6 +1626,14 @@ nckde_create (nckde **out, int kind, int kernel, double nu, unsigned dim, int cv
     return NCKDE_EHIP;
   }
 
+  if (hipEventCreateWithFlags (&h->ready_ev, hipEventDisableTiming) != hipSuccess)
+  {
+    set_error ("nckde_create: hipEventCreate failed");
+    (void) hipStreamDestroy (h->stream);
+    delete h;
+    return NCKDE_EHIP;
+  }
+
   *out = h;
 
   return NCKDE_OK;
@@ -1621,6 +1654,9 @@ nckde_destroy (nckde *h)
   for (hipEvent_t &e : h->pev)
     if (e)
       (void) hipEventDestroy (e);
+
+  if (h->ready_ev)
+    (void) hipEventDestroy (h->ready_ev);
 
   delete h;
 
@@ -1815,9 +1851,143 @@ nckde_set_nearPD_maxiter (nckde *h, unsigned v)
 int
 nckde_set_use_threads (nckde *h, int v)
 {
-  (void) v;
+  TRY (check_handle (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
 
-  return check_handle (h);
+  h->use_threads = (v != 0);
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_use_threads (nckde *h, int *v)
+{
+  TRY (check_handle (h));
+
+  if (v == nullptr)
+  {
+    set_error ("nckde_get_use_threads: NULL output");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  *v = h->use_threads ? 1 : 0;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_shrink (nckde *h, double *v)
+{
+  TRY (check_handle (h));
+
+  if (v == nullptr)
+  {
+    set_error ("nckde_get_shrink: NULL output");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  *v = h->shrink;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_local_frac (nckde *h, double *v)
+{
+  TRY (check_handle (h));
+
+  if (v == nullptr)
+  {
+    set_error ("nckde_get_local_frac: NULL output");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  *v = h->local_frac;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_use_rot_href (nckde *h, int *v)
+{
+  TRY (check_handle (h));
+
+  if (v == nullptr)
+  {
+    set_error ("nckde_get_use_rot_href: NULL output");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  *v = h->use_rot_href ? 1 : 0;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_nearPD_maxiter (nckde *h, unsigned *v)
+{
+  TRY (check_handle (h));
+
+  if (v == nullptr)
+  {
+    set_error ("nckde_get_nearPD_maxiter: NULL output");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  *v = h->nearPD_maxiter;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_cov_type (nckde *h, int *v)
+{
+  TRY (check_handle (h));
+
+  if (v == nullptr)
+  {
+    set_error ("nckde_get_cov_type: NULL output");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  *v = h->cov_type;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_cov_fixed (nckde *h, double *cov)
+{
+  TRY (check_handle (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  if (!h->has_cov_fixed)
+  {
+    set_error ("nckde_get_cov_fixed: no fixed covariance was set");
+    return NCKDE_ESTATE;
+  }
+
+  if (cov == nullptr)
+  {
+    set_error ("nckde_get_cov_fixed: NULL output");
+    return NCKDE_EINVAL;
+  }
+
+  std::copy (h->cov_fixed.begin (), h->cov_fixed.end (), cov);
+
+  return NCKDE_OK;
 }
 
 // ncm_stats_dist_kde_set_cov_type / _set_cov_fixed (ncm_stats_dist_kde.c:
@@ -2174,6 +2344,51 @@ nckde_get_cov_decomp (nckde *h, size_t i, double *U)
 }
 
 int
+nckde_get_Ki (nckde *h, size_t i, double *y_i, double *cov_i, double *n_i, double *w_i)
+{
+  TRY (check_prepared (h));
+
+  if (y_i == nullptr || cov_i == nullptr || n_i == nullptr || w_i == nullptr)
+  {
+    set_error ("nckde_get_Ki: NULL output");
+    return NCKDE_EINVAL;
+  }
+
+  const size_t d = (size_t) h->d;
+  std::vector<double> U (d * d);
+  double lnnorm = 0.0, href = 0.0;
+
+  TRY (nckde_get_cov_decomp (h, i, U.data ()));
+  TRY (nckde_get_lnnorm (h, i, &lnnorm));
+  TRY (nckde_get_href (h, &href));
+
+  {
+    std::lock_guard<std::mutex> lk (h->mtx);
+
+    std::copy (h->X.begin () + i * d, h->X.begin () + (i + 1) * d, y_i);
+    HIPCHK (hipMemcpy (w_i, h->dw.as<double> () + i, sizeof (double), hipMemcpyDeviceToHost));
+  }
+
+  // cov_i = href^2 U^T U over the upper triangle of the factor (the
+  // reference's triang_to_sym also zeroes the stored factor's lower triangle;
+  // the stored factor is left as it is here)
+  for (size_t a = 0; a < d; ++a)
+    for (size_t b = 0; b < d; ++b)
+    {
+      double acc = 0.0;
+
+      for (size_t k = 0; k <= std::min (a, b); ++k)
+        acc += U[k * d + a] * U[k * d + b];
+
+      cov_i[a * d + b] = acc * href * href;
+    }
+
+  *n_i = std::exp (lnnorm);
+
+  return NCKDE_OK;
+}
+
+int
 nckde_get_full_cov (nckde *h, double *cov)
 {
   TRY (check_prepared (h));
@@ -2377,17 +2592,14 @@ nckde_serialize (nckde *h, int with_state, void *buf, size_t cap, size_t *len)
   return NCKDE_OK;
 }
 
-int
-nckde_deserialize (nckde **out, const void *buf, size_t len)
+} // extern "C"
+
+// bytes left to read must cover every array before it is sized from the
+// header's counts (a corrupt or truncated buffer then fails cleanly instead
+// of asking for an allocation its length cannot back)
+static int
+deserialize_impl (nckde **out, const void *buf, size_t len)
 {
-  if (out == nullptr || buf == nullptr)
-  {
-    set_error ("nckde_deserialize: NULL argument");
-    return NCKDE_EINVAL;
-  }
-
-  *out = nullptr;
-
   Reader r {(const unsigned char *) buf, len};
   char magic[8];
 
@@ -2430,6 +2642,12 @@ nckde_deserialize (nckde **out, const void *buf, size_t len)
 
   if (has_fixed)
   {
+    if (sizeof (double) * dd > r.left ())
+    {
+      set_error ("nckde_deserialize: truncated fixed covariance");
+      return NCKDE_EINVAL;
+    }
+
     std::vector<double> cf (dd);
 
     r.get (cf.data (), sizeof (double) * dd);
@@ -2447,7 +2665,7 @@ nckde_deserialize (nckde **out, const void *buf, size_t len)
 
   const uint64_t ns = r.val<uint64_t> ();
 
-  if (!r.ok || ns > (len / sizeof (double)) / (size_t) d)
+  if (!r.ok || ns > (r.left () / sizeof (double)) / (size_t) d)
   {
     set_error ("nckde_deserialize: truncated sample");
     return NCKDE_EINVAL;
@@ -2471,12 +2689,36 @@ nckde_deserialize (nckde **out, const void *buf, size_t len)
       return NCKDE_EINVAL;
     }
 
+    // every count is bounded by ns <= len / (8 d) here, so none of these
+    // products overflows; together they must fit in what is left
+    size_t need = sizeof (double) * (n_obs * d + 2 * dd + n_k);
+
+    if (kind == NCKDE_VKDE)
+    {
+      if (k > r.left () / sizeof (int32_t) / std::max<uint64_t> (n_k, 1))
+        need = SIZE_MAX;
+      else
+        need += sizeof (double) * (2 * n_k * dd + n_k) + sizeof (int32_t) * n_k * k;
+    }
+
+    if (need > r.left ())
+    {
+      set_error ("nckde_deserialize: truncated prepared state");
+      return NCKDE_EINVAL;
+    }
+
     h->n_obs = n_obs;
     h->n_k   = n_k;
     h->k     = k;
     h->D     = nck::padded_dim (d);
 
     auto dev = [&] (DBuf &b, size_t bytes) -> int {
+      if (bytes > r.left ())
+      {
+        set_error ("nckde_deserialize: truncated prepared state");
+        return NCKDE_EINVAL;
+      }
+
       std::vector<unsigned char> tmp (bytes);
 
       r.get (tmp.data (), bytes);
@@ -2538,6 +2780,35 @@ nckde_deserialize (nckde **out, const void *buf, size_t len)
   *out = guard.release ();
 
   return NCKDE_OK;
+}
+
+extern "C" {
+
+int
+nckde_deserialize (nckde **out, const void *buf, size_t len)
+{
+  if (out == nullptr || buf == nullptr)
+  {
+    set_error ("nckde_deserialize: NULL argument");
+    return NCKDE_EINVAL;
+  }
+
+  *out = nullptr;
+
+  try
+  {
+    return deserialize_impl (out, buf, len);
+  }
+  catch (const std::bad_alloc &)
+  {
+    set_error ("nckde_deserialize: host allocation failed");
+    return NCKDE_ENOMEM;
+  }
+  catch (const std::exception &e)
+  {
+    set_error ("nckde_deserialize: %s", e.what ());
+    return NCKDE_EINVAL;
+  }
 }
 
 int
@@ -2639,6 +2910,12 @@ nckde_request_bytes (const nckde_request *req, size_t *bytes)
 namespace nck
 {
 
+static std::array<DBuf *, N_WORK_BUFS>
+work_bufs (nckde *h)
+{
+  return {&h->dkeys, &h->dAT, &h->dM, &h->dS, &h->dlu, &h->dscratch, &h->drw, &h->drobust, &h->dpd};
+}
+
 // Frees the transient buffers of prepare / prepare_interp (kNN keys, IM^T,
 // normal matrices); the prepared factors and weights stay.
 int
@@ -2648,15 +2925,28 @@ trim_work (nckde *h)
   std::lock_guard<std::mutex> lk (h->mtx);
 
   HIPCHK (hipStreamSynchronize (h->stream));
-  h->dkeys.release ();
-  h->dAT.release ();
-  h->dM.release ();
-  h->dS.release ();
-  h->dlu.release ();
-  h->dscratch.release ();
-  h->drw.release ();
-  h->drobust.release ();
-  h->dpd.release ();
+
+  for (DBuf *b : work_bufs (h))
+    b->release ();
+
+  return NCKDE_OK;
+}
+
+int
+swap_work (nckde *h, WorkSet &w)
+{
+  TRY (check_handle (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  HIPCHK (hipStreamSynchronize (h->stream));
+
+  const auto bufs = work_bufs (h);
+
+  for (int i = 0; i < N_WORK_BUFS; ++i)
+  {
+    std::swap (bufs[i]->p, w.b[i].p);
+    std::swap (bufs[i]->bytes, w.b[i].bytes);
+  }
 
   return NCKDE_OK;
 }

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <deque>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -32,6 +33,13 @@ struct Job
   uint64_t ticket = 0;
   nckde_request_info info;
   bool done = false;
+};
+
+// Refit buffers a worker keeps between its prepare_interp requests
+struct WorkerState
+{
+  nck::WorkSet set;
+  bool active = false;  // running a request (its set may be lent to a session)
 };
 
 double
@@ -56,38 +64,79 @@ struct nckde_server
   int device = 0;
   std::chrono::steady_clock::time_point t0;
   std::vector<std::thread> workers;
+  std::vector<std::unique_ptr<WorkerState>> wstate;
 
-  // first admissible pending request, or nullptr; mu held
-  Job *
-  pick ()
+  // device bytes of the refit buffers held by idle workers; mu held
+  size_t
+  held_idle () const
   {
-    std::unordered_set<const nckde *> blocked;
+    size_t t = 0;
 
-    for (auto it = pending.begin (); it != pending.end (); ++it)
+    for (const auto &w : wstate)
+      if (!w->active)
+        t += w->set.bytes ();
+
+    return t;
+  }
+
+  // bytes charged for running j on worker wid: the request's own bytes plus,
+  // for requests that do not use it, the refit set the worker holds
+  size_t
+  charge (const Job *j, unsigned wid) const
+  {
+    return j->info.bytes + ((j->r.op == NCKDE_REQ_PREPARE_INTERP) ? 0 : wstate[wid]->set.bytes ());
+  }
+
+  // first admissible pending request for worker wid, or nullptr; mu held.
+  // When only the idle workers' refit sets keep a request out, they are
+  // freed (the one place where serving frees device memory).
+  Job *
+  pick (unsigned wid)
+  {
+    for (int pass = 0; pass < 2; ++pass)
     {
-      Job *j           = *it;
-      const nckde *h   = j->r.h;
+      std::unordered_set<const nckde *> blocked;
+      const size_t held = held_idle () - wstate[wid]->set.bytes ();
+      bool held_blocks  = false;
 
-      if (busy.count (h) || blocked.count (h))
+      for (auto it = pending.begin (); it != pending.end (); ++it)
       {
+        Job *j         = *it;
+        const nckde *h = j->r.h;
+
+        if (busy.count (h) || blocked.count (h))
+        {
+          blocked.insert (h);
+          continue;
+        }
+
+        const size_t c = charge (j, wid);
+
+        if (running == 0 || in_flight + held + c <= budget)
+        {
+          pending.erase (it);
+          return j;
+        }
+
+        if (in_flight + c <= budget)
+          held_blocks = true;
+
         blocked.insert (h);
-        continue;
       }
 
-      if (running == 0 || in_flight + j->info.bytes <= budget)
-      {
-        pending.erase (it);
-        return j;
-      }
+      if (!held_blocks || pass == 1)
+        break;
 
-      blocked.insert (h);
+      for (auto &w : wstate)
+        if (!w->active)
+          w->set.release ();
     }
 
     return nullptr;
   }
 
   static int
-  execute (const nckde_request &r)
+  execute (const nckde_request &r, nck::WorkSet &ws)
   {
     unsigned d = 0;
     int st     = nckde_get_dim (r.h, &d);
@@ -111,7 +160,21 @@ struct nckde_server
             return st;
         }
 
-        if ((st = nckde_prepare_interp (r.h, r.m2lnp, r.m)) != NCKDE_OK)
+        // the refit runs on the worker's buffers, which go back to the
+        // worker afterwards (also when the refit fails)
+        if ((st = nck::swap_work (r.h, ws)) != NCKDE_OK)
+          return st;
+
+        st = nckde_prepare_interp (r.h, r.m2lnp, r.m);
+
+        {
+          const int st2 = nck::swap_work (r.h, ws);
+
+          if (st == NCKDE_OK)
+            st = st2;
+        }
+
+        if (st != NCKDE_OK)
           return st;
 
         if (r.out != nullptr)
@@ -128,7 +191,7 @@ struct nckde_server
             r.out[i] = std::numeric_limits<double>::quiet_NaN ();
         }
 
-        return nck::trim_work (r.h);
+        return NCKDE_OK;
       default:
         set_error ("nckde_server: unknown op %d", r.op);
         return NCKDE_EINVAL;
@@ -136,14 +199,15 @@ struct nckde_server
   }
 
   void
-  worker ()
+  worker (unsigned wid)
   {
     (void) hipSetDevice (device);
     std::unique_lock<std::mutex> lk (mu);
+    WorkerState &me = *wstate[wid];
 
     while (true)
     {
-      Job *j = pick ();
+      Job *j = pick (wid);
 
       if (j == nullptr)
       {
@@ -154,15 +218,19 @@ struct nckde_server
         continue;
       }
 
+      const size_t c = charge (j, wid);
+
       busy.insert (j->r.h);
-      in_flight += j->info.bytes;
+      in_flight += c;
       ++running;
+      me.active       = true;
       j->info.t_start = seconds_since (t0);
       lk.unlock ();
 
-      const int st = execute (j->r);
+      const int st = execute (j->r, me.set);
 
       lk.lock ();
+      me.active = false;
       j->info.t_end  = seconds_since (t0);
       j->info.status = st;
 
@@ -170,7 +238,7 @@ struct nckde_server
         snprintf (j->info.message, sizeof (j->info.message), "%s", nckde_last_error ());
 
       busy.erase (j->r.h);
-      in_flight -= j->info.bytes;
+      in_flight -= c;
       --running;
       j->done = true;
       cv_done.notify_all ();
@@ -205,7 +273,10 @@ nckde_server_create (nckde_server **out, unsigned slots, size_t hbm_budget)
   s->t0     = std::chrono::steady_clock::now ();
 
   for (unsigned i = 0; i < slots; ++i)
-    s->workers.emplace_back ([s] { s->worker (); });
+    s->wstate.emplace_back (new WorkerState ());
+
+  for (unsigned i = 0; i < slots; ++i)
+    s->workers.emplace_back ([s, i] { s->worker (i); });
 
   *out = s;
 
@@ -229,6 +300,9 @@ nckde_server_destroy (nckde_server *s)
 
   for (auto &kv : s->jobs)
     delete kv.second;
+
+  for (auto &w : s->wstate)
+    w->set.release ();
 
   delete s;
 
@@ -305,10 +379,12 @@ nckde_server_wait (nckde_server *s, uint64_t ticket, nckde_request_info *info)
     return NCKDE_EINVAL;
   }
 
+  // the ticket leaves the table before the wait, so a second waiter on it
+  // gets "unknown ticket" instead of a job another thread frees
   Job *j = it->second;
 
-  s->cv_done.wait (lk, [j] { return j->done; });
   s->jobs.erase (it);
+  s->cv_done.wait (lk, [j] { return j->done; });
   lk.unlock ();
 
   const int st = j->info.status;

@@ -55,6 +55,38 @@ struct DBuf
 // Frees a handle's transient prepare buffers (defined in api.cpp).
 int trim_work (nckde *h);
 
+// The transient buffers of prepare / prepare_interp (kNN keys, IM^T, normal
+// matrices, factor and solve scratch).  A request server keeps one set per
+// worker and lends it to the session it refits, so that refits reuse device
+// memory instead of allocating and freeing it per request.
+constexpr int N_WORK_BUFS = 9;
+struct WorkSet
+{
+  DBuf b[N_WORK_BUFS];
+
+  size_t
+  bytes () const
+  {
+    size_t t = 0;
+
+    for (const DBuf &x : b)
+      t += x.bytes;
+
+    return t;
+  }
+
+  void
+  release ()
+  {
+    for (DBuf &x : b)
+      x.release ();
+  }
+};
+
+// Exchanges the handle's transient buffers with w (waits for the handle's
+// stream first; defined in api.cpp).
+int swap_work (nckde *h, WorkSet &w);
+
 // Padded dimension used by the templated per-pair kernels.
 int padded_dim (int d);
 // Padded dimension of the MFMA tile (multiples of 4 up to 64, see k_eval_mixed.hip).

@@ -117,7 +117,6 @@ class _StatsDist:
         self.kernel = sdk
         self.d = sdk.dim
         self._over_smooth = 1.0
-        self._shrink = 0.01
         self.precision = Precision.FP64
 
     @classmethod
@@ -180,22 +179,52 @@ class _StatsDist:
 
     def set_shrink(self, v):
         check(lib().nckde_set_shrink(self._h, float(v)))
-        self._shrink = float(v)
 
     def get_shrink(self):
-        return self._shrink
+        """ncm_stats_dist_get_shrink (ncm_stats_dist.h:107)"""
+        v = C.c_double()
+        check(lib().nckde_get_shrink(self._h, C.byref(v)))
+        return v.value
 
     def set_use_threads(self, v):
         check(lib().nckde_set_use_threads(self._h, int(bool(v))))
 
+    def get_use_threads(self):
+        """ncm_stats_dist_get_use_threads (ncm_stats_dist.h:116)"""
+        v = C.c_int()
+        check(lib().nckde_get_use_threads(self._h, C.byref(v)))
+        return bool(v.value)
+
     def set_cov_type(self, cov_type):
         check(lib().nckde_set_cov_type(self._h, int(cov_type)))
+
+    def get_cov_type(self):
+        """ncm_stats_dist_kde_get_cov_type (ncm_stats_dist_kde.h:83)"""
+        v = C.c_int()
+        check(lib().nckde_get_cov_type(self._h, C.byref(v)))
+        return v.value
 
     def set_cov_fixed(self, cov):
         check(lib().nckde_set_cov_fixed(self._h, _f64(cov)))
 
+    def peek_cov_fixed(self):
+        """ncm_stats_dist_kde_peek_cov_fixed (ncm_stats_dist_kde.h:86); None
+        when no fixed covariance was set (the reference returns NULL)"""
+        out = np.empty((self.d, self.d))
+        st = lib().nckde_get_cov_fixed(self._h, out)
+        if st == -3:
+            return None
+        check(st)
+        return out
+
     def set_nearPD_maxiter(self, v):
         check(lib().nckde_set_nearPD_maxiter(self._h, int(v)))
+
+    def get_nearPD_maxiter(self):
+        """ncm_stats_dist_kde_get_nearPD_maxiter (ncm_stats_dist_kde.h:80)"""
+        v = C.c_uint()
+        check(lib().nckde_get_nearPD_maxiter(self._h, C.byref(v)))
+        return v.value
 
     # ---- sample --------------------------------------------------------
     def add_obs(self, x):
@@ -346,10 +375,11 @@ class _StatsDist:
     def get_Ki(self, i):
         """ncm_stats_dist_get_Ki (ncm_stats_dist.c:1810-1828): kernel centre,
         covariance h^2 U^T U, normalisation exp(lnnorm_i) and weight."""
-        U = np.triu(self.peek_cov_decomp(i))
-        h = self.get_href()
-        y = self.peek_sample_array()[i].copy()
-        return y, (U.T @ U) * h * h, float(np.exp(self.get_lnnorm(i))), float(self.peek_weights()[i])
+        y = np.empty(self.d)
+        cov = np.empty((self.d, self.d))
+        n_i, w_i = C.c_double(), C.c_double()
+        check(lib().nckde_get_Ki(self._h, int(i), y, cov, C.byref(n_i), C.byref(w_i)))
+        return y, cov, n_i.value, w_i.value
 
 
 class StatsDistKDE(_StatsDist):
@@ -377,7 +407,7 @@ def deserialize(blob: bytes):
 
     # header of nckde_serialize: magic[8], int32 x 10, double x 5 (layout in csrc/api.cpp)
     kind, kernel, d, cv, rot, _ct, _mi, _hf, _st, _pf = struct.unpack_from("=10i", blob, 8)
-    nu, over_smooth, shrink, local_frac, _sf = struct.unpack_from("=5d", blob, 8 + 40)
+    nu, over_smooth, _shrink, _local_frac, _sf = struct.unpack_from("=5d", blob, 8 + 40)
     h = C.c_void_p()
     buf = C.create_string_buffer(blob, len(blob))
     check(lib().nckde_deserialize(C.byref(h), buf, len(blob)))
@@ -387,11 +417,7 @@ def deserialize(blob: bytes):
     sd.kernel = StatsDistKernelST(d, nu) if kernel == 1 else StatsDistKernelGauss(d)
     sd.d = d
     sd._over_smooth = over_smooth
-    sd._shrink = shrink
     sd.precision = Precision.FP64
-    if kind == 1:
-        sd._local_frac = local_frac
-        sd._use_rot_href = bool(rot)
     return sd
 
 
@@ -401,19 +427,21 @@ class StatsDistVKDE(StatsDistKDE):
 
     def __init__(self, sdk, cv_type=StatsDistCV.NONE):
         super().__init__(sdk, cv_type)
-        self._local_frac = 0.05
-        self._use_rot_href = False
 
     def set_local_frac(self, v):
         check(lib().nckde_set_local_frac(self._h, float(v)))
-        self._local_frac = float(v)
 
     def get_local_frac(self):
-        return self._local_frac
+        """ncm_stats_dist_vkde_get_local_frac (ncm_stats_dist_vkde.h:59)"""
+        v = C.c_double()
+        check(lib().nckde_get_local_frac(self._h, C.byref(v)))
+        return v.value
 
     def set_use_rot_href(self, v):
         check(lib().nckde_set_use_rot_href(self._h, int(bool(v))))
-        self._use_rot_href = bool(v)
 
     def get_use_rot_href(self):
-        return self._use_rot_href
+        """ncm_stats_dist_vkde_get_use_rot_href (ncm_stats_dist_vkde.h:62)"""
+        v = C.c_int()
+        check(lib().nckde_get_use_rot_href(self._h, C.byref(v)))
+        return bool(v.value)
