@@ -217,6 +217,14 @@ int nckde_get_invUsample (nckde *h, double *Y);
  * (squared distance, index) order */
 int nckde_get_knn (nckde *h, size_t *k, int32_t *idx);
 
+/* No reference counterpart (diagnostic hook for checking a fit): the last
+ * NNLS fit's solution x (n_kernels, before shrink and normalisation), its
+ * gradient g = A^T (f - A x) (n_kernels) and residual f - A x (sample size),
+ * with A = diag(1/f) IM and f_j = exp(-(m2lnp_j - min m2lnp)/2), as the
+ * reference's ncm_nnls_solve leaves them (ncm_nnls.c:767-871).  Any output
+ * may be NULL.  NCKDE_ESTATE when the last prepare fitted no NNLS. */
+int nckde_get_nnls_state (nckde *h, double *x, double *g, double *r);
+
 /* ---- serialisation ------------------------------------------------------
  * The reference serialises an object's configuration with NcmSerialize
  * (ncm_serialize.h; exercised by test_ncm_stats_dist.c:955-957: a

@@ -69,6 +69,7 @@ SIGNATURES = {
     "nckde_get_lnnorm": (C.c_int, [_h, _sz, C.POINTER(C.c_double)]),
     "nckde_get_weights": (C.c_int, [_h, _dp]),
     "nckde_get_cov_decomp": (C.c_int, [_h, _sz, _dp]),
+    "nckde_get_nnls_state": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nckde_get_Ki": (C.c_int, [_h, _sz, _dp, _dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "nckde_get_full_cov": (C.c_int, [_h, _dp]),
     "nckde_get_full_cov_decomp": (C.c_int, [_h, _dp]),

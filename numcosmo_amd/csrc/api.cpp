@@ -107,6 +107,8 @@ struct nckde
   size_t n_obs = 0, n_k = 0, k = 0;
   int D = 0;
   double href = 0.0, kernel_lnnorm = 0.0, rnorm = 0.0;
+  // last NNLS fit: solution (before shrink), gradient A^T r and residual f - A x
+  std::vector<double> nnls_x, nnls_g, nnls_r;
   bool wcum_ready = false;
 
   hipStream_t stream      = nullptr;
@@ -425,6 +427,9 @@ do_prepare (nckde *h)
 
   h->last_stream = nullptr;
   h->prepared    = false;
+  h->nnls_x.clear ();
+  h->nnls_g.clear ();
+  h->nnls_r.clear ();
   h->n_obs     = h->n_samples;
   // the split cross-validations keep the first ceil(n split_frac) points as
   // kernels and the rest as held-out observations
@@ -787,6 +792,14 @@ nnls_solve (nckde *h)
 
   HIPCHK (hipMemcpyAsync (h->dw.p, h->dx.p, sizeof (double) * n, hipMemcpyDeviceToDevice, s));
   h->rnorm = rnorm;
+
+  // the fit's final solution, gradient and residual (diagnostic peek)
+  h->nnls_x.resize (n);
+  h->nnls_r.resize (nobs);
+  h->nnls_g = g;
+  HIPCHK (hipMemcpyAsync (h->nnls_x.data (), h->dx.p, sizeof (double) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK (hipMemcpyAsync (h->nnls_r.data (), h->dr.p, sizeof (double) * nobs, hipMemcpyDeviceToHost, s));
+  HIPCHK (hipStreamSynchronize (s));
 
   return NCKDE_OK;
 }
@@ -2339,6 +2352,30 @@ nckde_get_cov_decomp (nckde *h, size_t i, double *U)
     HIPCHK (hipMemcpy (U, h->dU.as<double> () + i * dd, sizeof (double) * dd, hipMemcpyDeviceToHost));
   else
     HIPCHK (hipMemcpy (U, h->dUg.p, sizeof (double) * dd, hipMemcpyDeviceToHost));
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_nnls_state (nckde *h, double *x, double *g, double *r)
+{
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  if (h->nnls_x.empty ())
+  {
+    set_error ("nckde_get_nnls_state: the last prepare ran no NNLS fit");
+    return NCKDE_ESTATE;
+  }
+
+  if (x)
+    std::copy (h->nnls_x.begin (), h->nnls_x.end (), x);
+
+  if (g)
+    std::copy (h->nnls_g.begin (), h->nnls_g.end (), g);
+
+  if (r)
+    std::copy (h->nnls_r.begin (), h->nnls_r.end (), r);
 
   return NCKDE_OK;
 }

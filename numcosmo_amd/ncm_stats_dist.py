@@ -364,6 +364,21 @@ class _StatsDist:
         check(lib().nckde_get_invUsample(self._h, Y))
         return Y
 
+    def get_k(self):
+        """VKDE: number of neighbours of each local covariance."""
+        k = C.c_size_t()
+        check(lib().nckde_get_knn(self._h, C.byref(k), None))
+        return k.value
+
+    def peek_nnls_state(self):
+        """(x, g, r) of the last NNLS fit: the solution before shrink, the
+        gradient A^T (f - A x) and the residual f - A x (nckde_get_nnls_state;
+        no reference counterpart)."""
+        nk, ns = self.get_n_kernels(), self.get_sample_size()
+        x, g, r = np.empty(nk), np.empty(nk), np.empty(ns)
+        check(lib().nckde_get_nnls_state(self._h, x.ctypes.data, g.ctypes.data, r.ctypes.data))
+        return x, g, r
+
     def peek_knn(self):
         k = C.c_size_t()
         check(lib().nckde_get_knn(self._h, C.byref(k), None))

@@ -2273,6 +2273,47 @@ ok_set_weights (ok_sd *sd, const double *w)
   sd->wcum_ready = 0;
 }
 
+/* Checker hook: a prepared VKDE (CV_NONE) from factors computed elsewhere,
+ * so that the evaluation and sampling paths (ncm_stats_dist_vkde.c:631-723,
+ * ncm_stats_dist.c:1565-1627) can be checked at sizes where the oracle's own
+ * prepare would take too long.  X is n x d (kernel centres = observations),
+ * U the n upper Cholesky factors (d x d row-major, cov_i = U_i^T U_i), w the
+ * weights, href the bandwidth.  The lnnorms are the oracle's own
+ * (ncm_stats_dist_kernel_*.c) of the given factors. */
+void
+ok_set_prepared_vkde (ok_sd *sd, int n, const double *X, const double *U, const double *w, double href)
+{
+  const int d     = sd->d;
+  const size_t dd = (size_t) d * d;
+  int i;
+
+  ok_clear_prepared (sd);
+  free (sd->X);
+  sd->X         = malloc (sizeof (double) * (size_t) n * d);
+  sd->n_alloc   = n;
+  sd->n_samples = n;
+  memcpy (sd->X, X, sizeof (double) * (size_t) n * d);
+
+  sd->kind      = OK_VKDE;
+  sd->n_obs     = n;
+  sd->n_kernels = n;
+  sd->k         = 0;
+  sd->href      = href;
+  sd->weights   = malloc (sizeof (double) * n);
+  sd->wcum      = malloc (sizeof (double) * (n + 1));
+  sd->U_arr     = malloc (sizeof (double) * (size_t) n * dd);
+  sd->lnnorms   = malloc (sizeof (double) * n);
+  memcpy (sd->weights, w, sizeof (double) * n);
+  memcpy (sd->U_arr, U, sizeof (double) * (size_t) n * dd);
+
+  #pragma omp parallel for schedule(static)
+  for (i = 0; i < n; i++)
+    sd->lnnorms[i] = ok_kernel_lnnorm (sd->kernel, sd->nu, d, U + (size_t) i * dd);
+
+  sd->wcum_ready = 0;
+  sd->prepared   = 1;
+}
+
 int ok_get_dim (ok_sd *sd) { return sd->d; }
 int ok_get_sample_size (ok_sd *sd) { return sd->n_obs; }
 int ok_get_n_kernels (ok_sd *sd) { return sd->n_kernels; }

@@ -83,6 +83,9 @@ int    ok_sample (ok_sd *sd, double u, const double *z, double chi2, double *x);
 /* replaces the weights of a prepared object (checker hook: evaluate with
  * weights fitted elsewhere) */
 void   ok_set_weights (ok_sd *sd, const double *w);
+/* checker hook: a prepared VKDE from given centres X (n x d), upper factors U
+ * (n x d x d), weights w and bandwidth (lnnorms recomputed by the oracle) */
+void   ok_set_prepared_vkde (ok_sd *sd, int n, const double *X, const double *U, const double *w, double href);
 
 /* peeks (valid after prepare) */
 int    ok_get_dim (ok_sd *sd);

@@ -81,6 +81,7 @@ def lib():
             "ok_kernel_choose": (C.c_int, [C.c_void_p, C.c_double]),
             "ok_sample": (C.c_int, [C.c_void_p, C.c_double, _dp, C.c_double, _dp]),
             "ok_set_weights": (None, [C.c_void_p, _dp]),
+            "ok_set_prepared_vkde": (None, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_double]),
             "ok_get_dim": (C.c_int, [C.c_void_p]),
             "ok_get_sample_size": (C.c_int, [C.c_void_p]),
             "ok_get_n_kernels": (C.c_int, [C.c_void_p]),
@@ -385,6 +386,13 @@ class StatsDist:
 
     def set_weights(self, w):
         lib().ok_set_weights(self._h, _f64(w))
+
+    def set_prepared_vkde(self, X, U, w, href):
+        """Checker hook: a prepared VKDE holding the given centres X (n x d),
+        upper factors U (n x d x d), weights and bandwidth; the oracle
+        recomputes the lnnorms from U."""
+        X = _f64(X)
+        lib().ok_set_prepared_vkde(self._h, X.shape[0], X, _f64(U), _f64(w), float(href))
 
     # peeks
     def get_dim(self): return lib().ok_get_dim(self._h)

@@ -116,3 +116,93 @@ def test_vkde_cauchy_d16_mixture_half_ensemble(oracle):
     _say("[config] VKDE Cauchy d=16 mixture n=3200 (APES half-ensemble)")
     tgt, X, m2, Xe = synth.workload("mixture", 3200, 6400, 16, 3)
     _vkde_full(oracle, ST, 1.0, 16, X, m2, Xe, 1000)
+
+
+def _all_factors(sd, n):
+    return np.stack([sd.peek_cov_decomp(i) for i in range(n)])
+
+
+def test_vkde_st5_d64_cond_n50000(oracle):
+    """The large VKDE at its full size: Student-t nu=5, d=64, n=50000, Gaussian
+    target with condition number 1e4 (k = 2500 neighbours).
+
+    The oracle's own prepare at this size takes CPU hours, so the fit is
+    pinned piecewise:
+    - whitening: the oracle's global covariance, factor and whitened sample
+      (a KDE prepare of the same sample), bit for bit;
+    - 256 sampled kernels: the oracle's brute-force kNN on the whitened sample,
+      then its covariance and Cholesky of those neighbours, bit for bit;
+    - eval: FP64 (<= 1e-12) and mixed (<= 1e-6) against the oracle evaluating
+      the device's factors and weights (lnnorms recomputed by the oracle);
+    - NNLS: the KKT conditions of the device fit, with the device's gradient
+      A^T r checked on sampled kernels against the oracle's interpolation
+      column times the residual, and the residual on sampled rows against
+      the oracle's density at those sample points.
+    """
+    import numcosmo_amd as na
+    d, n, nu = 64, 50000, 5.0
+    _say(f"[config] VKDE Student-t nu={nu} d={d} n={n} cond 1e4: device prepare_interp")
+    tgt, X, m2, Xe = synth.workload("cond", n, 2000, d, 5)
+    sd = na.StatsDistVKDE.new(na.StatsDistKernelST.new(d, nu))
+    sd.add_obs_array(X)
+    sd.prepare_interp(m2)
+    assert sd.get_n_kernels() == n
+    k = sd.get_k()
+    assert k == 2500
+
+    _say("   whitening against the oracle (KDE prepare of the same sample)")
+    ok = oracle.StatsDist(oracle.KDE, oracle.ST, d, nu)
+    ok.add_obs_array(X)
+    ok.prepare()
+    np.testing.assert_array_equal(sd.peek_full_cov(), ok.get_full_cov())
+    np.testing.assert_array_equal(np.triu(sd.peek_full_cov_decomp()), np.triu(ok.get_full_cov_decomp()))
+    Y = sd.peek_invUsample()
+    np.testing.assert_array_equal(Y, ok.get_invUsample())
+    del ok
+
+    _say("   256 sampled kernels: kNN lists and local factors")
+    g = np.random.default_rng(55)
+    qs = np.sort(g.choice(n, size=256, replace=False))
+    knn = sd.peek_knn()
+    for q in qs:
+        idx = oracle.knn(Y, int(q), k)[0]
+        np.testing.assert_array_equal(knn[q], idx)
+        cov = oracle.cov_welford(X, idx)
+        U = oracle.cholesky_decomp_fallback(cov)
+        np.testing.assert_array_equal(np.triu(sd.peek_cov_decomp(int(q))), np.triu(U))
+    del knn
+
+    _say("   eval against the oracle holding the device factors and weights")
+    U = _all_factors(sd, n)
+    w = sd.peek_weights()
+    od = oracle.StatsDist(oracle.VKDE, oracle.ST, d, nu)
+    od.set_prepared_vkde(X, U, w, sd.get_href())
+    ref = od.eval_m2lnp_batch(Xe)
+    e64 = np.max(_rel(sd.eval_m2lnp_batch(Xe, precision=na.Precision.FP64), ref))
+    emx = np.max(_rel(sd.eval_m2lnp_batch(Xe, precision=na.Precision.MIXED), ref))
+    _say(f"   FP64 vs oracle {e64:.2e}, mixed vs oracle {emx:.2e} on {len(Xe)} points")
+    assert e64 <= 1e-12
+    assert emx <= 1e-6
+
+    _say("   NNLS: KKT of the device fit")
+    x, gr, r = sd.peek_nnls_state()
+    f = np.exp(-0.5 * (m2 - m2.min()))
+    # residual r_i = 1 - (IM x)_i / f_i on sampled rows, where (IM x)_i is the
+    # linear density of the unshrunk solution x at theta_i
+    rows = np.sort(g.choice(n, size=64, replace=False))
+    od.set_weights(x)
+    r_rows = 1.0 - od.eval_batch(X[rows]) / f[rows]
+    assert np.allclose(r[rows], r_rows, rtol=1e-9, atol=1e-12)
+    # gradient g_j = sum_i A_ij r_i on sampled kernels, A_ij = IM_ij / f_i:
+    # column j of IM is the linear density of kernel j alone at every theta_i
+    for j in np.sort(g.choice(n, size=16, replace=False)):
+        oj = oracle.StatsDist(oracle.VKDE, oracle.ST, d, nu)
+        oj.set_prepared_vkde(X[j:j + 1], U[j:j + 1], np.ones(1), sd.get_href())
+        Aj = oj.eval_batch(X) / f
+        gj = Aj @ r
+        assert gj == pytest.approx(gr[j], rel=1e-8, abs=1e-10 * np.abs(Aj).sum())
+    scale = np.abs(gr).max()
+    act = x > 0
+    kkt = max(np.abs(gr[act]).max(initial=0.0), gr[~act].max(initial=0.0)) / max(scale, 1e-300)
+    _say(f"   KKT residual {kkt:.2e}, {int(act.sum())} active weights")
+    assert kkt < 1e-6

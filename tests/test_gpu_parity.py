@@ -51,6 +51,11 @@ CASES = [
     (1, GAUSS, 16, 1000, 0.0),
     (0, GAUSS, 4, 500, 0.0),
     (0, ST, 5, 400, 4.0),
+    # above d=16: padded dimensions 24, 32 and 64 of the FP64 eval and IM kernels
+    (1, ST, 24, 1200, 3.0),
+    (1, GAUSS, 32, 1500, 0.0),
+    (1, ST, 64, 1800, 5.0),
+    (0, GAUSS, 40, 900, 0.0),
 ]
 
 
@@ -108,7 +113,8 @@ def _kkt_rel(A, x):
 
 
 @pytest.mark.parametrize("kind,kernel,d,n,nu", [
-    (1, ST, 2, 200, 3.0), (1, GAUSS, 3, 400, 0.0), (1, ST, 8, 800, 3.0), (0, GAUSS, 4, 300, 0.0)])
+    (1, ST, 2, 200, 3.0), (1, GAUSS, 3, 400, 0.0), (1, ST, 8, 800, 3.0), (0, GAUSS, 4, 300, 0.0),
+    (1, ST, 24, 600, 3.0), (1, GAUSS, 32, 800, 0.0), (1, ST, 64, 1400, 5.0)])
 def test_prepare_interp_weights(gpu, oracle, kind, kernel, d, n, nu):
     _say(f"[prepare_interp] kind={kind} kernel={kernel} d={d} n={n}")
     g = np.random.default_rng(3000 + d + n)

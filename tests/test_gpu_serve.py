@@ -112,6 +112,63 @@ def test_results_alone_equal_results_in_the_mix():
         assert _same(got[k][0], ref), f"request {k} ({it[0]}, session {it[1]}) differs from its run alone"
 
 
+def _factors(sd):
+    n = sd.get_n_kernels()
+    return np.stack([sd.peek_cov_decomp(i) for i in range(n)])
+
+
+def test_served_stream_at_full_range_against_the_oracle(oracle):
+    """Sessions over the whole served range (n log-uniform on [500, 20000],
+    d in {4, 8, 16, 32}), with one session per d at n >= 10^4: the mix equals
+    the requests run alone, and every eval result (FP64 and mixed) is checked
+    against the oracle holding the session's device factors and weights at
+    that request, on a subset of its points."""
+    from numcosmo_amd.serve import Server
+    stream = synth.ServedStream(n_req=48, rate=0, n_sessions=8, n_lo=500, n_hi=20000, dims=(4, 8, 16, 32),
+                                seed=21, m_factor=10)
+    for s, (n, d) in enumerate([(10500, 4), (12000, 8), (16000, 16), (20000, 32)]):
+        stream.sessions[s] = {"n": n, "d": d, "seed": 9000 + s}
+    # every large session gets an eval in each precision
+    for k, s in enumerate([0, 1, 2, 3]):
+        stream.session[2 * k], stream.op[2 * k] = s, 0
+        stream.session[2 * k + 1], stream.op[2 * k + 1] = s, 0
+    plan = _plan(stream, precisions=(1, 0))
+    _say(f"[serve C4] {len(plan)} requests over {len(stream.sessions)} sessions "
+         f"(n {[S['n'] for S in stream.sessions]}, d {[S['d'] for S in stream.sessions]})")
+    sds = _sessions(stream)
+    with Server(slots=4, hbm_budget=64 << 30) as srv:
+        tickets = [_submit(srv, sds, it) for it in plan]
+        got = [srv.wait(t) for t in tickets]
+
+    _say("[serve C4] replay alone, with the oracle on eval results")
+    sds2 = _sessions(stream)
+    cache = {}
+    refits = {}
+    worst = {0: 0.0, 1: 0.0}
+    for k, it in enumerate(plan):
+        ref = _alone(sds2, it)
+        assert _same(got[k][0], ref), f"request {k} ({it[0]}, session {it[1]}) differs from its run alone"
+        op, s, p = it
+        if op == "prepare_interp":
+            refits[s] = refits.get(s, 0) + 1
+            continue
+        if op != "eval":
+            continue
+        key = (s, refits.get(s, 0))
+        if key not in cache:
+            sd = sds2[s]
+            od = oracle.StatsDist(oracle.VKDE, oracle.ST, sd.d, stream.nu)
+            od.set_prepared_vkde(sd.peek_sample_array()[:sd.get_n_kernels()], _factors(sd), sd.peek_weights(),
+                                 sd.get_href())
+            cache = {key: od}
+        sub = np.arange(0, len(p[0]), max(1, len(p[0]) // 64))
+        oref = cache[key].eval_m2lnp_batch(p[0][sub])
+        err = np.max(np.abs(ref[sub] - oref) / np.maximum(np.abs(oref), 1.0))
+        worst[p[1]] = max(worst[p[1]], err)
+        assert err <= (1e-6 if p[1] == 1 else 1e-12), (k, s, p[1], err)
+    _say(f"   eval vs oracle: FP64 max rel err {worst[0]:.2e}, mixed {worst[1]:.2e}")
+
+
 def test_admission_respects_the_device_budget():
     """With a budget below every request, requests run one at a time and
     still give the same results."""

@@ -540,3 +540,30 @@ def test_robust_too_few_points(oracle):
 def synth_spd(g, d):
     from numcosmo_amd import synth
     return synth.spd(g, d)
+
+
+def test_prepared_vkde_hook_reproduces_the_prepared_object(oracle):
+    """ok_set_prepared_vkde (the checker hook that loads factors computed
+    elsewhere) gives an object whose eval, lnnorms, kernel choice and sample
+    are bit-identical to the object those factors came from."""
+    from numcosmo_amd import synth
+    g = np.random.default_rng(77)
+    d, n = 5, 400
+    X = synth.Banana(d).draw(g, n)
+    od = oracle.StatsDist(oracle.VKDE, oracle.ST, d, 3.0)
+    od.add_obs_array(X)
+    od.prepare()
+    w = g.uniform(size=n)
+    w /= w.sum()
+    od.set_weights(w)
+    U = np.stack([od.get_cov_decomp(i) for i in range(n)])
+    o2 = oracle.StatsDist(oracle.VKDE, oracle.ST, d, 3.0)
+    o2.set_prepared_vkde(X, U, w, od.get_href())
+    Xe = synth.Banana(d).draw(g, 300)
+    np.testing.assert_array_equal(o2.eval_m2lnp_batch(Xe), od.eval_m2lnp_batch(Xe))
+    np.testing.assert_array_equal(o2.eval_batch(Xe), od.eval_batch(Xe))
+    assert all(o2.get_lnnorm(i) == od.get_lnnorm(i) for i in range(0, n, 7))
+    u = g.uniform(size=50)
+    assert [o2.kernel_choose(p) for p in u] == [od.kernel_choose(p) for p in u]
+    z = g.normal(size=d)
+    assert np.array_equal(o2.sample(u[0], z, 4.0)[1], od.sample(u[0], z, 4.0)[1])
