@@ -814,14 +814,124 @@ ok_lu_solve (int n, double *S, double *b)
   return 0;
 }
 
+/* Minimum-norm solution of the underdetermined A_P x = f (nrows < np), the
+ * other half of dgels 'N' (LAPACK dgels: LQ factorisation of A_P).  Done as
+ * the Householder QR of B = A_P^T (np x nrows, B = Q R): A_P = R^T Q^T, so
+ * x = Q [R^{-T} f; 0].  Reflector signs as dlarfg (beta = -sign(a) |.|). */
+static void
+ok_lq_minnorm (int nrows, int ncols, const double *A, const int *P, int np, const double *f, double *x)
+{
+  double *B   = malloc (sizeof (double) * (size_t) np * nrows);
+  double *v0  = malloc (sizeof (double) * nrows);
+  double *vtv = malloc (sizeof (double) * nrows);
+  int i, j, k;
+
+  for (j = 0; j < np; j++)
+    for (i = 0; i < nrows; i++)
+      B[(size_t) j * nrows + i] = A[(size_t) i * ncols + P[j]];
+
+  for (k = 0; k < nrows; k++)
+  {
+    double nrm = 0.0, alpha, t = 0.0;
+
+    for (j = k; j < np; j++)
+      nrm = nrm + B[(size_t) j * nrows + k] * B[(size_t) j * nrows + k];
+
+    nrm    = sqrt (nrm);
+    vtv[k] = 0.0;
+    v0[k]  = 0.0;
+
+    if (nrm == 0.0)
+      continue;
+
+    alpha = (B[(size_t) k * nrows + k] >= 0.0) ? -nrm : nrm;
+    B[(size_t) k * nrows + k] -= alpha;
+
+    for (j = k; j < np; j++)
+      t = t + B[(size_t) j * nrows + k] * B[(size_t) j * nrows + k];
+
+    vtv[k] = t;
+
+    for (i = k + 1; i < nrows; i++)
+    {
+      double sd = 0.0;
+
+      for (j = k; j < np; j++)
+        sd = sd + B[(size_t) j * nrows + k] * B[(size_t) j * nrows + i];
+
+      sd = 2.0 * sd / t;
+
+      for (j = k; j < np; j++)
+        B[(size_t) j * nrows + i] -= sd * B[(size_t) j * nrows + k];
+    }
+
+    v0[k]                     = B[(size_t) k * nrows + k];
+    B[(size_t) k * nrows + k] = alpha;
+  }
+
+  /* R^T y = f (forward), y into x[0..nrows-1], zeros below */
+  for (k = 0; k < nrows; k++)
+  {
+    double t = f[k];
+
+    for (i = 0; i < k; i++)
+      t = t - B[(size_t) i * nrows + k] * x[i];
+
+    x[k] = (B[(size_t) k * nrows + k] != 0.0) ? t / B[(size_t) k * nrows + k] : 0.0;
+  }
+
+  for (j = nrows; j < np; j++)
+    x[j] = 0.0;
+
+  /* x = H_0 H_1 ... H_{nrows-1} [y; 0] */
+  for (k = nrows - 1; k >= 0; k--)
+  {
+    double sd;
+
+    if (vtv[k] == 0.0)
+      continue;
+
+    sd = v0[k] * x[k];
+
+    for (j = k + 1; j < np; j++)
+      sd = sd + B[(size_t) j * nrows + k] * x[j];
+
+    sd = 2.0 * sd / vtv[k];
+
+    x[k] -= sd * v0[k];
+
+    for (j = k + 1; j < np; j++)
+      x[j] -= sd * B[(size_t) j * nrows + k];
+  }
+
+  free (B);
+  free (v0);
+  free (vtv);
+}
+
 /* Least squares min |A_P x - f| by Householder QR (stands in for dgels,
- * ncm_nnls.c:608-640).  A is nrows x ncols row-major, P the column list. */
+ * ncm_nnls.c:608-640).  A is nrows x ncols row-major, P the column list;
+ * nrows < np goes to the minimum-norm solution as dgels does.
+ * A column that is exactly zero below the current row (an exactly
+ * dependent column, e.g. a duplicate) is skipped with zero weight and the
+ * next reflector starts on the same row, so the others solve the reduced
+ * problem; dgels would stop there with info > 0 (a zero diagonal of R). */
 static void
 ok_qr_lstsq (int nrows, int ncols, const double *A, const int *P, int np, const double *f, double *x)
 {
-  double *Q  = malloc (sizeof (double) * (size_t) nrows * np);
-  double *rh = malloc (sizeof (double) * nrows);
-  int i, j, k;
+  double *Q, *rh;
+  int *prow;
+  int i, j, k, r = 0;
+
+  if (nrows < np)
+  {
+    ok_lq_minnorm (nrows, ncols, A, P, np, f, x);
+    return;
+  }
+
+  Q    = malloc (sizeof (double) * (size_t) nrows * np);
+  rh   = malloc (sizeof (double) * nrows);
+  prow = malloc (sizeof (int) * np);
 
   for (i = 0; i < nrows; i++)
     for (j = 0; j < np; j++)
@@ -829,72 +939,87 @@ ok_qr_lstsq (int nrows, int ncols, const double *A, const int *P, int np, const 
 
   memcpy (rh, f, sizeof (double) * nrows);
 
-  for (k = 0; k < np && k < nrows; k++)
+  for (k = 0; k < np; k++)
   {
     double nrm = 0.0, alpha, vtv = 0.0;
 
-    for (i = k; i < nrows; i++)
-      nrm = nrm + Q[(size_t) i * np + k] * Q[(size_t) i * np + k];
+    prow[k] = -1;
 
-    nrm   = sqrt (nrm);
-    alpha = (Q[(size_t) k * np + k] > 0.0) ? -nrm : nrm;
+    if (r == nrows)
+      continue;
+
+    for (i = r; i < nrows; i++)
+      nrm = nrm + Q[(size_t) i * np + k] * Q[(size_t) i * np + k];
 
     if (nrm == 0.0)
       continue;
 
-    Q[(size_t) k * np + k] -= alpha;
+    nrm   = sqrt (nrm);
+    alpha = (Q[(size_t) r * np + k] >= 0.0) ? -nrm : nrm;
 
-    for (i = k; i < nrows; i++)
+    Q[(size_t) r * np + k] -= alpha;
+
+    for (i = r; i < nrows; i++)
       vtv = vtv + Q[(size_t) i * np + k] * Q[(size_t) i * np + k];
 
     for (j = k + 1; j < np; j++)
     {
-      double s = 0.0;
+      double sd = 0.0;
 
-      for (i = k; i < nrows; i++)
-        s = s + Q[(size_t) i * np + k] * Q[(size_t) i * np + j];
+      for (i = r; i < nrows; i++)
+        sd = sd + Q[(size_t) i * np + k] * Q[(size_t) i * np + j];
 
-      s = 2.0 * s / vtv;
+      sd = 2.0 * sd / vtv;
 
-      for (i = k; i < nrows; i++)
-        Q[(size_t) i * np + j] -= s * Q[(size_t) i * np + k];
+      for (i = r; i < nrows; i++)
+        Q[(size_t) i * np + j] -= sd * Q[(size_t) i * np + k];
     }
 
     {
-      double s = 0.0;
+      double sd = 0.0;
 
-      for (i = k; i < nrows; i++)
-        s = s + Q[(size_t) i * np + k] * rh[i];
+      for (i = r; i < nrows; i++)
+        sd = sd + Q[(size_t) i * np + k] * rh[i];
 
-      s = 2.0 * s / vtv;
+      sd = 2.0 * sd / vtv;
 
-      for (i = k; i < nrows; i++)
-        rh[i] -= s * Q[(size_t) i * np + k];
+      for (i = r; i < nrows; i++)
+        rh[i] -= sd * Q[(size_t) i * np + k];
     }
 
-    Q[(size_t) k * np + k] = alpha;
+    Q[(size_t) r * np + k] = alpha;
+    prow[k]                = r++;
   }
 
-  for (k = ((np < nrows) ? np : nrows) - 1; k >= 0; k--)
+  for (k = np - 1; k >= 0; k--)
   {
-    double t = rh[k];
+    const int row = prow[k];
+    double t;
 
-    for (j = k + 1; j < np && j < nrows; j++)
-      t = t - Q[(size_t) k * np + j] * x[j];
+    if (row < 0)
+    {
+      x[k] = 0.0;
+      continue;
+    }
 
-    x[k] = (Q[(size_t) k * np + k] != 0.0) ? t / Q[(size_t) k * np + k] : 0.0;
+    t = rh[row];
+
+    for (j = k + 1; j < np; j++)
+      if (prow[j] >= 0)
+        t = t - Q[(size_t) row * np + j] * x[j];
+
+    x[k] = t / Q[(size_t) row * np + k];
   }
-
-  for (k = nrows; k < np; k++)
-    x[k] = 0.0;
 
   free (Q);
   free (rh);
+  free (prow);
 }
 
 typedef struct
 {
   int nrows, ncols;
+  int umethod; /* OK_NNLS_NORMAL, OK_NNLS_NORMAL_LU or OK_NNLS_QR */
   const double *A;
   const double *f;
   double *M;   /* A^T A, full */
@@ -904,17 +1029,24 @@ typedef struct
   int *P;      /* sorted index list */
 } ok_nnls_ws;
 
-/* ncm_nnls.c:655-666 -> :574-605 -> :608-640: normal equations by Cholesky,
- * else the symmetric-indefinite solve, else least squares by QR.
- * Writes the solution on the sorted index list into sol[0..np-1]. */
-static void
-ok_nnls_unconstrained (ok_nnls_ws *ws, const char *inP, double *sol)
-{
-  int np = 0, a, c;
+/* Solves that ended on each rung (Cholesky, symmetric solve, QR) since the
+ * last ok_nnls_rung_counts (..., 1): a test can show that a fit took a
+ * fallback rung. */
+static __thread unsigned ok_rungs[3];
 
-  for (a = 0; a < ws->ncols; a++)
-    if (inP[a])
-      ws->P[np++] = a;
+void
+ok_nnls_rung_counts (unsigned *out, int reset)
+{
+  memcpy (out, ok_rungs, sizeof (ok_rungs));
+
+  if (reset)
+    memset (ok_rungs, 0, sizeof (ok_rungs));
+}
+
+static void
+ok_gather_normal (ok_nnls_ws *ws, int np)
+{
+  int a, c;
 
   for (a = 0; a < np; a++)
   {
@@ -923,22 +1055,71 @@ ok_nnls_unconstrained (ok_nnls_ws *ws, const char *inP, double *sol)
 
     ws->bs[a] = ws->b[ws->P[a]];
   }
+}
 
-  if (ok_chol_solve (np, ws->S, ws->bs) > 0)
+/* ncm_nnls.c:671-706 with :655-666 -> :574-605 -> :608-640.
+ * NORMAL: normal equations by Cholesky, else the symmetric-indefinite
+ * solve, else least squares by QR; NORMAL_LU starts at the symmetric solve;
+ * QR solves min |A_P x - f| directly.
+ * Writes the solution on the sorted index list into sol[0..np-1]. */
+static void
+ok_nnls_unconstrained (ok_nnls_ws *ws, const char *inP, double *sol)
+{
+  int np = 0, a;
+
+  for (a = 0; a < ws->ncols; a++)
+    if (inP[a])
+      ws->P[np++] = a;
+
+  if (ws->umethod == OK_NNLS_QR)
   {
-    for (a = 0; a < np; a++)
-    {
-      for (c = 0; c < np; c++)
-        ws->S[(size_t) a * np + c] = ws->M[(size_t) ws->P[a] * ws->ncols + ws->P[c]];
+    ok_qr_lstsq (ws->nrows, ws->ncols, ws->A, ws->P, np, ws->f, ws->bs);
+    ok_rungs[2]++;
+  }
+  else
+  {
+    int chol_fail = 1;
 
-      ws->bs[a] = ws->b[ws->P[a]];
+    if (ws->umethod == OK_NNLS_NORMAL)
+    {
+      ok_gather_normal (ws, np);
+      chol_fail = (ok_chol_solve (np, ws->S, ws->bs) > 0);
+
+      if (!chol_fail)
+        ok_rungs[0]++;
     }
 
-    if (ok_lu_solve (np, ws->S, ws->bs) > 0)
-      ok_qr_lstsq (ws->nrows, ws->ncols, ws->A, ws->P, np, ws->f, ws->bs);
+    if (chol_fail)
+    {
+      ok_gather_normal (ws, np);
+
+      if (ok_lu_solve (np, ws->S, ws->bs) > 0)
+      {
+        ok_qr_lstsq (ws->nrows, ws->ncols, ws->A, ws->P, np, ws->f, ws->bs);
+        ok_rungs[2]++;
+      }
+      else
+      {
+        ok_rungs[1]++;
+      }
+    }
   }
 
   memcpy (sol, ws->bs, sizeof (double) * np);
+}
+
+/* min |A x - f| over all columns by the Householder QR rung (dgels). */
+void
+ok_lstsq_qr (int nrows, int ncols, const double *A, const double *f, double *x)
+{
+  int *P = malloc (sizeof (int) * ncols);
+  int j;
+
+  for (j = 0; j < ncols; j++)
+    P[j] = j;
+
+  ok_qr_lstsq (nrows, ncols, A, P, ncols, f, x);
+  free (P);
 }
 
 /* ncm_nnls.c:728-751: solve on P, drop indices whose solution is < 1e-300
@@ -1041,6 +1222,12 @@ ok_nnls_mgrad (ok_nnls_ws *ws, const double *r, double *g)
 double
 ok_nnls (int nrows, int ncols, const double *A, const double *f, double *x)
 {
+  return ok_nnls_m (nrows, ncols, A, f, x, OK_NNLS_NORMAL);
+}
+
+double
+ok_nnls_m (int nrows, int ncols, const double *A, const double *f, double *x, int umethod)
+{
   const double reltol = DBL_EPSILON;
   ok_nnls_ws ws;
   char *P     = malloc (ncols);
@@ -1053,9 +1240,10 @@ ok_nnls (int nrows, int ncols, const double *A, const double *f, double *x)
   double rnorm;
   int i, j, k;
 
-  ws.nrows = nrows;
-  ws.ncols = ncols;
-  ws.A     = A;
+  ws.nrows   = nrows;
+  ws.ncols   = ncols;
+  ws.umethod = umethod;
+  ws.A       = A;
   ws.f     = f;
   ws.M     = malloc (sizeof (double) * (size_t) ncols * ncols);
   ws.b     = malloc (sizeof (double) * ncols);
@@ -1063,30 +1251,34 @@ ok_nnls (int nrows, int ncols, const double *A, const double *f, double *x)
   ws.bs    = malloc (sizeof (double) * ncols);
   ws.P     = malloc (sizeof (int) * ncols);
 
-  /* M = A^T A (dsyrk, ncm_matrix.c:1547-1575), b = A^T f (dgemv) */
-  #pragma omp parallel for schedule(dynamic, 8) private(j, k)
-  for (i = 0; i < ncols; i++)
+  /* M = A^T A (dsyrk, ncm_matrix.c:1547-1575), b = A^T f (dgemv); the QR
+   * method works on A alone (ncm_nnls.c:795-797) */
+  if (umethod != OK_NNLS_QR)
   {
-    for (j = i; j < ncols; j++)
+    #pragma omp parallel for schedule(dynamic, 8) private(j, k)
+    for (i = 0; i < ncols; i++)
+    {
+      for (j = i; j < ncols; j++)
+      {
+        double s = 0.0;
+
+        for (k = 0; k < nrows; k++)
+          s = s + A[(size_t) k * ncols + i] * A[(size_t) k * ncols + j];
+
+        ws.M[(size_t) i * ncols + j] = s;
+        ws.M[(size_t) j * ncols + i] = s;
+      }
+    }
+
+    for (j = 0; j < ncols; j++)
     {
       double s = 0.0;
 
       for (k = 0; k < nrows; k++)
-        s = s + A[(size_t) k * ncols + i] * A[(size_t) k * ncols + j];
+        s = s + A[(size_t) k * ncols + j] * f[k];
 
-      ws.M[(size_t) i * ncols + j] = s;
-      ws.M[(size_t) j * ncols + i] = s;
+      ws.b[j] = s;
     }
-  }
-
-  for (j = 0; j < ncols; j++)
-  {
-    double s = 0.0;
-
-    for (k = 0; k < nrows; k++)
-      s = s + A[(size_t) k * ncols + j] * f[k];
-
-    ws.b[j] = s;
   }
 
   memset (P, 1, ncols);
@@ -1193,7 +1385,7 @@ struct ok_sd
   int kind, kernel, d;
   double nu;
   double over_smooth, shrink, local_frac, split_frac;
-  int use_rot_href, cov_type, nearPD_maxiter, use_threads, cv_type;
+  int use_rot_href, cov_type, nearPD_maxiter, use_threads, cv_type, nnls_umethod;
   uint64_t cv_draws;
   double *cov_fixed, *cov_fixed_decomp;
 
@@ -1229,6 +1421,7 @@ ok_new (int kind, int kernel, double nu, int d)
   sd->cov_type       = OK_COV_SAMPLE;
   sd->nearPD_maxiter = 200;   /* ncm_stats_dist_kde.c:290-296 */
   sd->use_threads    = 0;
+  sd->nnls_umethod   = OK_NNLS_NORMAL; /* ncm_stats_dist.c:861 */
 
   return sd;
 }
@@ -1268,6 +1461,29 @@ void ok_set_local_frac (ok_sd *sd, double v) { sd->local_frac = v; }
 void ok_set_use_rot_href (ok_sd *sd, int v) { sd->use_rot_href = v; }
 void ok_set_nearPD_maxiter (ok_sd *sd, int v) { sd->nearPD_maxiter = v; }
 void ok_set_use_threads (ok_sd *sd, int v) { sd->use_threads = v; }
+void ok_set_nnls_umethod (ok_sd *sd, int v) { sd->nnls_umethod = v; }
+
+/* OpenMP team size of the batched calls (the CPU-baseline timings run at one
+ * thread and at all cores) */
+void
+ok_set_num_threads (int n)
+{
+#ifdef _OPENMP
+  omp_set_num_threads (n);
+#else
+  (void) n;
+#endif
+}
+
+int
+ok_get_max_threads (void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads ();
+#else
+  return 1;
+#endif
+}
 
 /* ncm_stats_dist_kde.c:731-745 and :770-791 */
 static int
@@ -1992,7 +2208,7 @@ ok_prepare_interp (ok_sd *sd, const double *m2lnp, int len)
           IM[(size_t) i * n_k + j] = IM[(size_t) i * n_k + j] * s;
       }
 
-      sd->rnorm = ok_nnls (n_obs, n_k, IM, f1, sd->weights);
+      sd->rnorm = ok_nnls_m (n_obs, n_k, IM, f1, sd->weights, sd->nnls_umethod);
     }
 
     free (IM);
@@ -3091,7 +3307,7 @@ ok_cv_fit_at (ok_split_ctx *c, double lnos)
       c->IM[(size_t) i * n_k + j] = c->IM[(size_t) i * n_k + j] * s;
   }
 
-  c->rnorm = ok_nnls (n_obs, n_k, c->IM, c->f1, sd->weights);
+  c->rnorm = ok_nnls_m (n_obs, n_k, c->IM, c->f1, sd->weights, sd->nnls_umethod);
 }
 
 /* _ncm_stats_dist_prepare_interp_fit_nnls_f (ncm_stats_dist.c:826-855) */

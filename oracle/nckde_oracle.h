@@ -57,6 +57,11 @@ void   ok_knn (int n, int d, const double *Y, int q, int k, int *out_idx, double
 
 /* ---- active-set NNLS (ncm_nnls.c, ncm_iset.c) ------------------------- */
 double ok_nnls (int nrows, int ncols, const double *A, const double *f, double *x);
+/* umethod as NcmNNLSUMethod (ncm_nnls.h:51-60): NORMAL, NORMAL_LU, QR */
+enum { OK_NNLS_NORMAL = 0, OK_NNLS_NORMAL_LU = 1, OK_NNLS_QR = 2 };
+double ok_nnls_m (int nrows, int ncols, const double *A, const double *f, double *x, int umethod);
+void   ok_nnls_rung_counts (unsigned *out, int reset);
+void   ok_lstsq_qr (int nrows, int ncols, const double *A, const double *f, double *x);
 
 /* ---- NcmStatsDist object ---------------------------------------------- */
 ok_sd *ok_new (int kind, int kernel, double nu, int d);
@@ -67,6 +72,9 @@ void   ok_set_local_frac (ok_sd *sd, double v);
 void   ok_set_use_rot_href (ok_sd *sd, int v);
 void   ok_set_nearPD_maxiter (ok_sd *sd, int v);
 void   ok_set_use_threads (ok_sd *sd, int v);
+void   ok_set_nnls_umethod (ok_sd *sd, int v);
+void   ok_set_num_threads (int n);
+int    ok_get_max_threads (void);
 int    ok_set_cov_type (ok_sd *sd, int cov_type);
 int    ok_set_cov_fixed (ok_sd *sd, const double *cov);
 void   ok_reset (ok_sd *sd);

@@ -17,7 +17,11 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle_nckde.so")
+# the same restatement built -O3 with FMA contraction and AVX2 (x86-64-v3):
+# the CPU-baseline build, timed by bench.py, never the parity checker
+_FAST_PATH = os.path.join(_HERE, "liboracle_nckde_fast.so")
 _lib = None
+_fast = None
 
 KDE, VKDE = 0, 1
 GAUSS, ST = 0, 1
@@ -42,87 +46,107 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
-        sig = {
-            "ok_kernel_rot_bandwidth": (C.c_double, [C.c_int, C.c_double, C.c_int, C.c_double]),
-            "ok_kernel_lnnorm": (C.c_double, [C.c_int, C.c_double, C.c_int, _dp]),
-            "ok_kernel_eval_unnorm": (C.c_double, [C.c_int, C.c_double, C.c_int, C.c_double]),
-            "ok_kernel_sum0": (None, [C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
-                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]),
-            "ok_kernel_sum1": (None, [C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, C.c_double,
-                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]),
-            "ok_cholesky_upper": (C.c_int, [C.c_int, _dp, C.c_int]),
-            "ok_cholesky_lndet": (C.c_double, [C.c_int, _dp, C.c_int]),
-            "ok_nearPD": (C.c_int, [C.c_int, _dp, C.c_int]),
-            "ok_cholesky_decomp_fallback": (None, [C.c_int, _dp, _dp, C.c_int]),
-            "ok_jacobi_eigen": (C.c_int, [C.c_int, _dp, _dp, _dp]),
-            "ok_cov_welford": (None, [C.c_int, C.c_int, _dp, C.c_void_p, _dp]),
-            "ok_knn": (None, [C.c_int, C.c_int, _dp, C.c_int, C.c_int, _ip, C.c_void_p]),
-            "ok_nnls": (C.c_double, [C.c_int, C.c_int, _dp, _dp, _dp]),
-            "ok_new": (C.c_void_p, [C.c_int, C.c_int, C.c_double, C.c_int]),
-            "ok_free": (None, [C.c_void_p]),
-            "ok_set_over_smooth": (None, [C.c_void_p, C.c_double]),
-            "ok_set_shrink": (None, [C.c_void_p, C.c_double]),
-            "ok_set_local_frac": (None, [C.c_void_p, C.c_double]),
-            "ok_set_use_rot_href": (None, [C.c_void_p, C.c_int]),
-            "ok_set_nearPD_maxiter": (None, [C.c_void_p, C.c_int]),
-            "ok_set_use_threads": (None, [C.c_void_p, C.c_int]),
-            "ok_set_cov_type": (C.c_int, [C.c_void_p, C.c_int]),
-            "ok_set_cov_fixed": (C.c_int, [C.c_void_p, _dp]),
-            "ok_reset": (None, [C.c_void_p]),
-            "ok_add_obs": (None, [C.c_void_p, _dp]),
-            "ok_prepare": (C.c_int, [C.c_void_p]),
-            "ok_prepare_interp": (C.c_int, [C.c_void_p, _dp, C.c_int]),
-            "ok_compute_IM": (C.c_int, [C.c_void_p, _dp]),
-            "ok_eval_m2lnp": (C.c_double, [C.c_void_p, _dp]),
-            "ok_eval": (C.c_double, [C.c_void_p, _dp]),
-            "ok_eval_m2lnp_batch": (None, [C.c_void_p, _dp, C.c_int, _dp]),
-            "ok_eval_batch": (None, [C.c_void_p, _dp, C.c_int, _dp]),
-            "ok_kernel_choose": (C.c_int, [C.c_void_p, C.c_double]),
-            "ok_sample": (C.c_int, [C.c_void_p, C.c_double, _dp, C.c_double, _dp]),
-            "ok_set_weights": (None, [C.c_void_p, _dp]),
-            "ok_set_prepared_vkde": (None, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_double]),
-            "ok_get_dim": (C.c_int, [C.c_void_p]),
-            "ok_get_sample_size": (C.c_int, [C.c_void_p]),
-            "ok_get_n_kernels": (C.c_int, [C.c_void_p]),
-            "ok_get_n_samples": (C.c_int, [C.c_void_p]),
-            "ok_get_k": (C.c_int, [C.c_void_p]),
-            "ok_get_href": (C.c_double, [C.c_void_p]),
-            "ok_get_rnorm": (C.c_double, [C.c_void_p]),
-            "ok_get_lnnorm": (C.c_double, [C.c_void_p, C.c_int]),
-            "ok_get_weights": (None, [C.c_void_p, _dp]),
-            "ok_get_cov_decomp": (None, [C.c_void_p, C.c_int, _dp]),
-            "ok_get_full_cov": (None, [C.c_void_p, _dp]),
-            "ok_get_full_cov_decomp": (None, [C.c_void_p, _dp]),
-            "ok_get_sample": (None, [C.c_void_p, _dp]),
-            "ok_get_invUsample": (None, [C.c_void_p, _dp]),
-            "ok_get_knn": (None, [C.c_void_p, _ip]),
-            "ok_last_error": (C.c_char_p, []),
-            "ok_qn": (C.c_double, [C.c_int, _dp]),
-            "ok_cov_robust_diag": (C.c_int, [C.c_int, C.c_int, _dp, C.c_void_p, _dp]),
-            "ok_cov_robust_ogk": (C.c_int, [C.c_int, C.c_int, _dp, C.c_void_p, _dp]),
-            "ok_set_cv_type": (None, [C.c_void_p, C.c_int]),
-            "ok_set_split_frac": (None, [C.c_void_p, C.c_double]),
-            "ok_get_over_smooth": (C.c_double, [C.c_void_p]),
-            "ok_cv_objective": (C.c_double, [C.c_void_p, C.c_double]),
-            "ok_nm_minimize1": (C.c_int, [FN1, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double,
-                                          C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
-            "ok_lm_dif1": (C.c_int, [RESFN, C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_int, _dp,
-                                     C.POINTER(C.c_int)]),
-            "ok_pair_stats_update": (None, [_dp, C.c_double, C.c_double]),
-            "ok_draw": (None, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.POINTER(C.c_double), _dp,
-                               C.POINTER(C.c_double)]),
-            "ok_host_normal": (C.c_double, [C.c_uint64, C.c_uint64]),
-            "ok_philox4x32_10": (None, [np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
-                                        np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]),
-            "ok_sample_at": (None, [C.c_void_p, C.c_int, _dp, C.c_double, _dp]),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = L
+        _lib = _bind(C.CDLL(_LIB_PATH))
     return _lib
+
+
+def fast_lib():
+    """The CPU-baseline build of the restatement (timing only)."""
+    global _fast
+    if _fast is None:
+        if not os.path.exists(_FAST_PATH):
+            build()
+        _fast = _bind(C.CDLL(_FAST_PATH))
+    return _fast
+
+
+def _bind(L):
+    sig = {
+        "ok_kernel_rot_bandwidth": (C.c_double, [C.c_int, C.c_double, C.c_int, C.c_double]),
+        "ok_kernel_lnnorm": (C.c_double, [C.c_int, C.c_double, C.c_int, _dp]),
+        "ok_kernel_eval_unnorm": (C.c_double, [C.c_int, C.c_double, C.c_int, C.c_double]),
+        "ok_kernel_sum0": (None, [C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "ok_kernel_sum1": (None, [C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, C.c_double,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "ok_cholesky_upper": (C.c_int, [C.c_int, _dp, C.c_int]),
+        "ok_cholesky_lndet": (C.c_double, [C.c_int, _dp, C.c_int]),
+        "ok_nearPD": (C.c_int, [C.c_int, _dp, C.c_int]),
+        "ok_cholesky_decomp_fallback": (None, [C.c_int, _dp, _dp, C.c_int]),
+        "ok_jacobi_eigen": (C.c_int, [C.c_int, _dp, _dp, _dp]),
+        "ok_cov_welford": (None, [C.c_int, C.c_int, _dp, C.c_void_p, _dp]),
+        "ok_knn": (None, [C.c_int, C.c_int, _dp, C.c_int, C.c_int, _ip, C.c_void_p]),
+        "ok_nnls": (C.c_double, [C.c_int, C.c_int, _dp, _dp, _dp]),
+        "ok_nnls_m": (C.c_double, [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int]),
+        "ok_nnls_rung_counts": (None, [np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
+                                       C.c_int]),
+        "ok_lstsq_qr": (None, [C.c_int, C.c_int, _dp, _dp, _dp]),
+        "ok_set_nnls_umethod": (None, [C.c_void_p, C.c_int]),
+        "ok_set_num_threads": (None, [C.c_int]),
+        "ok_get_max_threads": (C.c_int, []),
+        "ok_new": (C.c_void_p, [C.c_int, C.c_int, C.c_double, C.c_int]),
+        "ok_free": (None, [C.c_void_p]),
+        "ok_set_over_smooth": (None, [C.c_void_p, C.c_double]),
+        "ok_set_shrink": (None, [C.c_void_p, C.c_double]),
+        "ok_set_local_frac": (None, [C.c_void_p, C.c_double]),
+        "ok_set_use_rot_href": (None, [C.c_void_p, C.c_int]),
+        "ok_set_nearPD_maxiter": (None, [C.c_void_p, C.c_int]),
+        "ok_set_use_threads": (None, [C.c_void_p, C.c_int]),
+        "ok_set_cov_type": (C.c_int, [C.c_void_p, C.c_int]),
+        "ok_set_cov_fixed": (C.c_int, [C.c_void_p, _dp]),
+        "ok_reset": (None, [C.c_void_p]),
+        "ok_add_obs": (None, [C.c_void_p, _dp]),
+        "ok_prepare": (C.c_int, [C.c_void_p]),
+        "ok_prepare_interp": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+        "ok_compute_IM": (C.c_int, [C.c_void_p, _dp]),
+        "ok_eval_m2lnp": (C.c_double, [C.c_void_p, _dp]),
+        "ok_eval": (C.c_double, [C.c_void_p, _dp]),
+        "ok_eval_m2lnp_batch": (None, [C.c_void_p, _dp, C.c_int, _dp]),
+        "ok_eval_batch": (None, [C.c_void_p, _dp, C.c_int, _dp]),
+        "ok_kernel_choose": (C.c_int, [C.c_void_p, C.c_double]),
+        "ok_sample": (C.c_int, [C.c_void_p, C.c_double, _dp, C.c_double, _dp]),
+        "ok_set_weights": (None, [C.c_void_p, _dp]),
+        "ok_set_prepared_vkde": (None, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_double]),
+        "ok_get_dim": (C.c_int, [C.c_void_p]),
+        "ok_get_sample_size": (C.c_int, [C.c_void_p]),
+        "ok_get_n_kernels": (C.c_int, [C.c_void_p]),
+        "ok_get_n_samples": (C.c_int, [C.c_void_p]),
+        "ok_get_k": (C.c_int, [C.c_void_p]),
+        "ok_get_href": (C.c_double, [C.c_void_p]),
+        "ok_get_rnorm": (C.c_double, [C.c_void_p]),
+        "ok_get_lnnorm": (C.c_double, [C.c_void_p, C.c_int]),
+        "ok_get_weights": (None, [C.c_void_p, _dp]),
+        "ok_get_cov_decomp": (None, [C.c_void_p, C.c_int, _dp]),
+        "ok_get_full_cov": (None, [C.c_void_p, _dp]),
+        "ok_get_full_cov_decomp": (None, [C.c_void_p, _dp]),
+        "ok_get_sample": (None, [C.c_void_p, _dp]),
+        "ok_get_invUsample": (None, [C.c_void_p, _dp]),
+        "ok_get_knn": (None, [C.c_void_p, _ip]),
+        "ok_last_error": (C.c_char_p, []),
+        "ok_qn": (C.c_double, [C.c_int, _dp]),
+        "ok_cov_robust_diag": (C.c_int, [C.c_int, C.c_int, _dp, C.c_void_p, _dp]),
+        "ok_cov_robust_ogk": (C.c_int, [C.c_int, C.c_int, _dp, C.c_void_p, _dp]),
+        "ok_set_cv_type": (None, [C.c_void_p, C.c_int]),
+        "ok_set_split_frac": (None, [C.c_void_p, C.c_double]),
+        "ok_get_over_smooth": (C.c_double, [C.c_void_p]),
+        "ok_cv_objective": (C.c_double, [C.c_void_p, C.c_double]),
+        "ok_nm_minimize1": (C.c_int, [FN1, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double,
+                                      C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+        "ok_lm_dif1": (C.c_int, [RESFN, C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_int, _dp,
+                                 C.POINTER(C.c_int)]),
+        "ok_pair_stats_update": (None, [_dp, C.c_double, C.c_double]),
+        "ok_draw": (None, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.POINTER(C.c_double), _dp,
+                           C.POINTER(C.c_double)]),
+        "ok_host_normal": (C.c_double, [C.c_uint64, C.c_uint64]),
+        "ok_philox4x32_10": (None, [np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
+                                    np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]),
+        "ok_sample_at": (None, [C.c_void_p, C.c_int, _dp, C.c_double, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
 
 
 def _f64(a):
@@ -233,11 +257,39 @@ def knn(Y, q, k):
     return idx, d2
 
 
-def nnls(A, f):
+NNLS_NORMAL, NNLS_NORMAL_LU, NNLS_QR = 0, 1, 2   # NcmNNLSUMethod (ncm_nnls.h:51-60)
+
+
+def nnls(A, f, umethod=NNLS_NORMAL):
     A = _f64(A)
     x = np.zeros(A.shape[1])
-    rnorm = lib().ok_nnls(A.shape[0], A.shape[1], A, _f64(f), x)
+    rnorm = lib().ok_nnls_m(A.shape[0], A.shape[1], A, _f64(f), x, int(umethod))
     return x, rnorm
+
+
+def nnls_rung_counts(reset=True):
+    """Unconstrained solves per rung (Cholesky, symmetric solve, QR) since the
+    last reset, in this thread."""
+    c = np.zeros(3, dtype=np.uint32)
+    lib().ok_nnls_rung_counts(c, int(reset))
+    return [int(v) for v in c]
+
+
+def set_num_threads(n):
+    """OpenMP team size of the oracle's batched calls (both builds)."""
+    lib().ok_set_num_threads(int(n))
+
+
+def get_max_threads():
+    return lib().ok_get_max_threads()
+
+
+def lstsq_qr(A, f):
+    """min |A x - f| by the oracle's Householder QR rung (dgels)."""
+    A = _f64(A)
+    x = np.zeros(A.shape[1])
+    lib().ok_lstsq_qr(A.shape[0], A.shape[1], A, _f64(f), x)
+    return x
 
 
 # ------------------------------------------------------ cross-validation --
@@ -298,120 +350,125 @@ def host_normal(key, ctr):
 
 # ------------------------------------------------------------ stats dist --
 class StatsDist:
-    """Oracle mirror of NcmStatsDistKDE / NcmStatsDistVKDE."""
+    """Oracle mirror of NcmStatsDistKDE / NcmStatsDistVKDE.
 
-    def __init__(self, kind, kernel, d, nu=3.0, cv_type=CV_NONE):
-        self._h = lib().ok_new(kind, kernel, float(nu), int(d))
+    fast=True binds the CPU-baseline build (liboracle_nckde_fast.so, -O3 with
+    FMA contraction): for timing, never for parity."""
+
+    def __init__(self, kind, kernel, d, nu=3.0, cv_type=CV_NONE, fast=False):
+        self._L = fast_lib() if fast else lib()
+        self._h = self._L.ok_new(kind, kernel, float(nu), int(d))
         self.kind, self.kernel, self.d, self.nu = kind, kernel, int(d), float(nu)
-        lib().ok_set_cv_type(self._h, int(cv_type))
+        self._L.ok_set_cv_type(self._h, int(cv_type))
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().ok_free(self._h)
+            self._L.ok_free(self._h)
             self._h = None
 
     def _check(self, ret):
         if ret != 0:
-            raise OracleError(lib().ok_last_error().decode())
+            raise OracleError(self._L.ok_last_error().decode())
 
     # configuration
-    def set_over_smooth(self, v): lib().ok_set_over_smooth(self._h, v)
-    def set_shrink(self, v): lib().ok_set_shrink(self._h, v)
-    def set_local_frac(self, v): lib().ok_set_local_frac(self._h, v)
-    def set_use_rot_href(self, v): lib().ok_set_use_rot_href(self._h, int(v))
-    def set_nearPD_maxiter(self, v): lib().ok_set_nearPD_maxiter(self._h, int(v))
-    def set_use_threads(self, v): lib().ok_set_use_threads(self._h, int(v))
-    def set_cov_type(self, v): self._check(lib().ok_set_cov_type(self._h, int(v)))
-    def set_cov_fixed(self, cov): self._check(lib().ok_set_cov_fixed(self._h, _f64(cov)))
-    def set_cv_type(self, v): lib().ok_set_cv_type(self._h, int(v))
-    def set_split_frac(self, v): lib().ok_set_split_frac(self._h, float(v))
-    def get_over_smooth(self): return lib().ok_get_over_smooth(self._h)
+    def set_over_smooth(self, v): self._L.ok_set_over_smooth(self._h, v)
+    def set_shrink(self, v): self._L.ok_set_shrink(self._h, v)
+    def set_local_frac(self, v): self._L.ok_set_local_frac(self._h, v)
+    def set_use_rot_href(self, v): self._L.ok_set_use_rot_href(self._h, int(v))
+    def set_nearPD_maxiter(self, v): self._L.ok_set_nearPD_maxiter(self._h, int(v))
+    def set_use_threads(self, v): self._L.ok_set_use_threads(self._h, int(v))
+    def set_nnls_umethod(self, v): self._L.ok_set_nnls_umethod(self._h, int(v))
+    def set_cov_type(self, v): self._check(self._L.ok_set_cov_type(self._h, int(v)))
+    def set_cov_fixed(self, cov): self._check(self._L.ok_set_cov_fixed(self._h, _f64(cov)))
+    def set_cv_type(self, v): self._L.ok_set_cv_type(self._h, int(v))
+    def set_split_frac(self, v): self._L.ok_set_split_frac(self._h, float(v))
+    def get_over_smooth(self): return self._L.ok_get_over_smooth(self._h)
 
     def cv_objective(self, lnos):
         """Objective of the CV type at ln(over_smooth) on the prepared object."""
-        return lib().ok_cv_objective(self._h, float(lnos))
+        return self._L.ok_cv_objective(self._h, float(lnos))
 
     def sample_at(self, i, z, chi2=1.0):
         x = np.empty(self.d)
-        lib().ok_sample_at(self._h, int(i), _f64(z), float(chi2), x)
+        self._L.ok_sample_at(self._h, int(i), _f64(z), float(chi2), x)
         return x
 
-    def reset(self): lib().ok_reset(self._h)
+    def reset(self): self._L.ok_reset(self._h)
 
     def add_obs(self, x):
-        lib().ok_add_obs(self._h, _f64(x))
+        self._L.ok_add_obs(self._h, _f64(x))
 
     def add_obs_array(self, X):
         for row in _f64(X):
-            lib().ok_add_obs(self._h, row)
+            self._L.ok_add_obs(self._h, row)
 
     def prepare(self):
-        self._check(lib().ok_prepare(self._h))
+        self._check(self._L.ok_prepare(self._h))
 
     def prepare_interp(self, m2lnp):
         m2lnp = _f64(m2lnp)
-        self._check(lib().ok_prepare_interp(self._h, m2lnp, len(m2lnp)))
+        self._check(self._L.ok_prepare_interp(self._h, m2lnp, len(m2lnp)))
 
     def compute_IM(self):
         n = self.get_sample_size()
         IM = np.empty((n, self.get_n_kernels()))
-        self._check(lib().ok_compute_IM(self._h, IM))
+        self._check(self._L.ok_compute_IM(self._h, IM))
         return IM
 
     def eval_m2lnp(self, x):
-        return lib().ok_eval_m2lnp(self._h, _f64(x))
+        return self._L.ok_eval_m2lnp(self._h, _f64(x))
 
     def eval(self, x):
-        return lib().ok_eval(self._h, _f64(x))
+        return self._L.ok_eval(self._h, _f64(x))
 
     def eval_m2lnp_batch(self, X):
         X = _f64(X)
         out = np.empty(X.shape[0])
-        lib().ok_eval_m2lnp_batch(self._h, X, X.shape[0], out)
+        self._L.ok_eval_m2lnp_batch(self._h, X, X.shape[0], out)
         return out
 
     def eval_batch(self, X):
         X = _f64(X)
         out = np.empty(X.shape[0])
-        lib().ok_eval_batch(self._h, X, X.shape[0], out)
+        self._L.ok_eval_batch(self._h, X, X.shape[0], out)
         return out
 
     def kernel_choose(self, u):
-        return lib().ok_kernel_choose(self._h, float(u))
+        return self._L.ok_kernel_choose(self._h, float(u))
 
     def sample(self, u, z, chi2=1.0):
         x = np.empty(self.d)
-        i = lib().ok_sample(self._h, float(u), _f64(z), float(chi2), x)
+        i = self._L.ok_sample(self._h, float(u), _f64(z), float(chi2), x)
         return i, x
 
     def set_weights(self, w):
-        lib().ok_set_weights(self._h, _f64(w))
+        self._L.ok_set_weights(self._h, _f64(w))
 
     def set_prepared_vkde(self, X, U, w, href):
         """Checker hook: a prepared VKDE holding the given centres X (n x d),
         upper factors U (n x d x d), weights and bandwidth; the oracle
         recomputes the lnnorms from U."""
         X = _f64(X)
-        lib().ok_set_prepared_vkde(self._h, X.shape[0], X, _f64(U), _f64(w), float(href))
+        self._L.ok_set_prepared_vkde(self._h, X.shape[0], X, _f64(U), _f64(w), float(href))
 
     # peeks
-    def get_dim(self): return lib().ok_get_dim(self._h)
-    def get_sample_size(self): return lib().ok_get_sample_size(self._h)
-    def get_n_kernels(self): return lib().ok_get_n_kernels(self._h)
-    def get_n_samples(self): return lib().ok_get_n_samples(self._h)
-    def get_k(self): return lib().ok_get_k(self._h)
-    def get_href(self): return lib().ok_get_href(self._h)
-    def get_rnorm(self): return lib().ok_get_rnorm(self._h)
-    def get_lnnorm(self, i): return lib().ok_get_lnnorm(self._h, int(i))
+    def get_dim(self): return self._L.ok_get_dim(self._h)
+    def get_sample_size(self): return self._L.ok_get_sample_size(self._h)
+    def get_n_kernels(self): return self._L.ok_get_n_kernels(self._h)
+    def get_n_samples(self): return self._L.ok_get_n_samples(self._h)
+    def get_k(self): return self._L.ok_get_k(self._h)
+    def get_href(self): return self._L.ok_get_href(self._h)
+    def get_rnorm(self): return self._L.ok_get_rnorm(self._h)
+    def get_lnnorm(self, i): return self._L.ok_get_lnnorm(self._h, int(i))
 
     def get_weights(self):
         w = np.empty(self.get_n_kernels())
-        lib().ok_get_weights(self._h, w)
+        self._L.ok_get_weights(self._h, w)
         return w
 
     def get_cov_decomp(self, i):
         U = np.empty((self.d, self.d))
-        lib().ok_get_cov_decomp(self._h, int(i), U)
+        self._L.ok_get_cov_decomp(self._h, int(i), U)
         return U
 
     def get_lnnorms_raw(self):
@@ -421,28 +478,28 @@ class StatsDist:
 
     def get_full_cov(self):
         c = np.empty((self.d, self.d))
-        lib().ok_get_full_cov(self._h, c)
+        self._L.ok_get_full_cov(self._h, c)
         return c
 
     def get_full_cov_decomp(self):
         c = np.empty((self.d, self.d))
-        lib().ok_get_full_cov_decomp(self._h, c)
+        self._L.ok_get_full_cov_decomp(self._h, c)
         return c
 
     def get_sample(self):
         X = np.empty((self.get_n_samples(), self.d))
-        lib().ok_get_sample(self._h, X)
+        self._L.ok_get_sample(self._h, X)
         return X
 
     def get_invUsample(self):
         Y = np.empty((self.get_sample_size(), self.d))
-        lib().ok_get_invUsample(self._h, Y)
+        self._L.ok_get_invUsample(self._h, Y)
         return Y
 
     def get_knn(self):
         k = self.get_k()
         idx = np.empty((self.get_n_kernels(), k), dtype=np.int32)
-        lib().ok_get_knn(self._h, idx)
+        self._L.ok_get_knn(self._h, idx)
         return idx
 
 

@@ -254,6 +254,77 @@ def test_nnls_kkt_interpolation_like(oracle):
     _kkt(A, f, x, 1e-7)
 
 
+@pytest.mark.parametrize("m,n", [(50, 12), (12, 12), (8, 20)])
+def test_lstsq_qr_matches_numpy(oracle, m, n):
+    """The QR rung (dgels 'N'): least squares for m >= n, the minimum-norm
+    solution for m < n; numpy's SVD least squares gives both."""
+    g = _rng(700 + m + n)
+    A = g.normal(size=(m, n))
+    f = g.normal(size=m)
+    x = oracle.lstsq_qr(A, f)
+    xs = np.linalg.lstsq(A, f, rcond=None)[0]
+    np.testing.assert_allclose(x, xs, rtol=1e-11, atol=1e-12)
+    if m <= n:
+        np.testing.assert_allclose(A @ x, f, rtol=1e-11, atol=1e-12)
+
+
+def test_lstsq_qr_zero_column(oracle):
+    """A zero column gets zero weight; the others solve the reduced system."""
+    g = _rng(710)
+    A = g.normal(size=(30, 6))
+    A[:, 2] = 0.0
+    f = g.normal(size=30)
+    x = oracle.lstsq_qr(A, f)
+    keep = [0, 1, 3, 4, 5]
+    xs = np.linalg.lstsq(A[:, keep], f, rcond=None)[0]
+    assert x[2] == 0.0
+    np.testing.assert_allclose(x[keep], xs, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("umethod", [0, 1, 2])
+def test_nnls_umethods_match_scipy(oracle, umethod):
+    """NORMAL (Cholesky), NORMAL_LU (symmetric solve) and QR each reach
+    Lawson-Hanson's solution on well-conditioned problems, and each solve
+    ends on the rung the method names."""
+    for seed in range(4):
+        g = _rng(720 + seed)
+        m, n = 40, 25
+        A = g.uniform(0.0, 1.0, size=(m, n))
+        f = g.normal(size=m) + A @ np.maximum(g.normal(size=n), 0)
+        oracle.nnls_rung_counts(reset=True)
+        x, rnorm = oracle.nnls(A, f, umethod=umethod)
+        rungs = oracle.nnls_rung_counts(reset=True)
+        xs, rs = scipy.optimize.nnls(A, f)
+        assert rnorm == pytest.approx(rs, rel=1e-10)
+        assert np.allclose(x, xs, rtol=1e-8, atol=1e-10)
+        assert rungs[umethod] > 0 and sum(rungs) == rungs[umethod]
+
+
+def test_nnls_duplicate_columns_take_the_qr_rung(oracle):
+    """Two identical columns (3, 4, 0, ...): the Cholesky pivot of the second
+    is exactly 25 - (25/5)^2 = 0, elimination with partial pivoting zeroes the
+    duplicated row exactly, so the solve ends on the QR rung, where the
+    duplicate's reflected column is exactly zero and gets zero weight.  The
+    NNLS optimum (residual norm, fitted values) is unique and matches
+    Lawson-Hanson's; the weight is shared by the pair."""
+    g = _rng(730)
+    m, n = 30, 8
+    A = g.uniform(0.0, 1.0, size=(m, n))
+    A[:, 0] = 0.0
+    A[0, 0], A[1, 0] = 3.0, 4.0
+    A[:, 1] = A[:, 0]
+    f = A @ np.array([0.7, 0.0, 0.2, 0.0, 0.5, 0.1, 0.0, 0.3]) + 0.01 * g.normal(size=m)
+    oracle.nnls_rung_counts(reset=True)
+    x, rnorm = oracle.nnls(A, f)
+    rungs = oracle.nnls_rung_counts(reset=True)
+    assert rungs[2] >= 1, rungs
+    xs, rs = scipy.optimize.nnls(A, f)
+    assert rnorm == pytest.approx(rs, rel=1e-10)
+    np.testing.assert_allclose(A @ x, A @ xs, rtol=1e-9, atol=1e-12)
+    assert x[0] + x[1] == pytest.approx(xs[0] + xs[1], rel=1e-9)
+    _kkt(A, f, x, 1e-9 * np.abs(A).sum() * np.abs(f).max())
+
+
 # ------------------------------------------------- full KDE / VKDE path ---
 def _build(oracle, kind, kernel, d, X, nu=3.0, **kw):
     sd = oracle.StatsDist(kind, kernel, d, nu)
