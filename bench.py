@@ -1,23 +1,28 @@
 #!/usr/bin/env python
 """Kernel-pair evaluations per second of VKDE eval_m2lnp on one MI355X.
 
-Workload (one step = one eval_m2lnp pass over the evaluation points):
-  VKDE, Student-t kernel nu=3, d=8, banana target (x2 += 0.5 (x1^2 - 1)),
-  n=4000 kernels, local_frac 0.05, over_smooth 1, prepare_interp (NNLS
-  weights) on the exact m2lnp, then eval_m2lnp at m=100000 points that are
-  already resident in HBM.  Pairs per step = n * m = 4e8.
+Headline workload (one step = one eval_m2lnp pass over the evaluation points):
+  VKDE, Student-t kernel nu=3, d=16, n=20000 kernels with NNLS weights
+  (prepare_interp on the exact m2lnp of a seeded correlated Gaussian target),
+  eval_m2lnp at m=200000 points already resident in HBM: 4e9 pairs per step.
+  This is where the mixed-precision path is required to beat the plain FP64
+  path by 4x at relative m2lnp error <= 1e-6; both are timed, the mixed one
+  is `value`.  The mixed result is checked against the FP64 oracle on 1e4
+  points and against the device FP64 result on all 2e5.
 
-Also measured: the same eval at d=16, n=20000, m=200000 (Gaussian target)
-in both precisions, which is where the mixed-precision path is required to
-beat the plain FP64 path by 4x at relative m2lnp error <= 1e-6; and a served
-stream (512 requests, Poisson arrivals, eval/sample/prepare_interp 60/30/10
-over VKDE sessions with n log-uniform on [500, 20000], d in {4, 8, 16, 32},
-m = 10 n) whose p50/p99 request latency goes into "latency".
+Also measured: the banana VKDE at d=8, n=4000, m=1e5; the large VKDE
+(d=64, n=50000, nu=5, condition number 1e4, m=1e6) checked against the
+oracle on 1e4 points; 200 APES ensemble steps (d=16, 2x3200 walkers); the
+cross-validated bandwidth fits; and a served stream (512 requests, Poisson
+arrivals, eval/sample/prepare_interp 60/30/10 over VKDE sessions with n
+log-uniform on [500, 20000], d in {4, 8, 16, 32}, m = 10 n) whose p50/p99
+request latency goes into "latency".
 
 Timing: W warm-up steps, then K steps bracketed by torch.cuda.synchronize();
 the dominant kernel's duration comes from HIP events recorded on the stream
-the engine launches on.  The CPU baseline is the FP64 oracle (a restatement
-of the reference algorithm) on a bounded sample of the same workload.
+the engine launches on.  The CPU baseline is the oracle (a restatement of
+the reference algorithm) built -O3 for AVX2/FMA, timed on a bounded sample
+of the same workload at all cores and at one core.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -103,6 +108,45 @@ def supports(sd, prec, Xd, outd):
     return r == 0
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(oracle_mod, d, nu, X, w, Xe, n, threads, target_s):
+    """FP64 eval_m2lnp of the CPU-baseline build of the restatement (-O3,
+    FMA, AVX2) on `threads` OpenMP threads, over a bounded sample of the
+    points sized to about target_s seconds; returns (pairs/s, sample text)."""
+    od = oracle_mod.StatsDist(oracle_mod.VKDE, oracle_mod.ST, d, nu, fast=True)
+    od.set_use_threads(1)
+    od.add_obs_array(X)
+    od.prepare()
+    od.set_weights(w)
+    oracle_mod.set_num_threads(threads)
+    try:
+        pts = max(threads, 8)
+        while True:
+            t0 = time.perf_counter()
+            od.eval_m2lnp_batch(Xe[:pts])
+            dt = time.perf_counter() - t0
+            if dt >= 0.5 or pts >= len(Xe):
+                break
+            pts = min(len(Xe), pts * 4)
+        pts = int(min(len(Xe), max(pts, pts * target_s / max(dt, 1e-3))))
+        t0 = time.perf_counter()
+        od.eval_m2lnp_batch(Xe[:pts])
+        dt = time.perf_counter() - t0
+    finally:
+        oracle_mod.set_num_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    return pts * n / dt, f"{pts} points x {n} kernels, {dt:.1f} s"
+
+
 def run_case(name, kind_target, d, n, m, nu, steps, warmup, fit, check_pts, config_id, oracle_mod, cores):
     import torch
 
@@ -125,43 +169,53 @@ def run_case(name, kind_target, d, n, m, nu, steps, warmup, fit, check_pts, conf
     outd = torch.empty(m, dtype=torch.float64, device="cuda")
     stream = torch.cuda.Stream()
     res = {"prepare_s": prep_s, "n": n, "m": m, "d": d}
+    w = sd.peek_weights()
 
-    # FP64 oracle on a subset of points, with the device-fitted weights
+    # FP64 oracle (the parity build, all threads) on the first check_pts
+    # points, with the device-fitted weights
     od = oracle_mod.StatsDist(oracle_mod.VKDE, oracle_mod.ST, d, nu)
     od.set_use_threads(1)
     od.add_obs_array(X)
     t0 = time.perf_counter()
     od.prepare()
     log(f"[{name}] oracle prepare {time.perf_counter() - t0:.1f} s")
-    od.set_weights(sd.peek_weights())
+    od.set_weights(w)
     sub = Xe[:check_pts]
     t0 = time.perf_counter()
     ref = od.eval_m2lnp_batch(sub)
-    cpu_dt = time.perf_counter() - t0
-    # CPU baseline on a bounded sample sized to ~10 s of oracle work
-    cpu_pts = int(min(m, max(check_pts, check_pts * 10.0 / max(cpu_dt, 1e-3))))
-    t0 = time.perf_counter()
-    od.eval_m2lnp_batch(Xe[:cpu_pts])
-    cpu_dt = time.perf_counter() - t0
-    res["cpu_pairs_per_s"] = cpu_pts * n / cpu_dt
-    res["cpu_sample"] = f"{cpu_pts} points x {n} kernels, {cpu_dt:.1f} s"
-    res["cpu_cores"] = cores
-    log(f"[{name}] cpu baseline {cpu_pts * n / cpu_dt:.3e} pairs/s on {cores} threads")
+    log(f"[{name}] oracle eval of {check_pts} points {time.perf_counter() - t0:.1f} s")
+    del od
 
+    # CPU baseline: the -O3 build of the same restatement, all cores and one
+    cpu_all, sample_all = cpu_baseline(oracle_mod, d, nu, X, w, Xe, n, cores, 10.0)
+    cpu_one, sample_one = cpu_baseline(oracle_mod, d, nu, X, w, Xe, n, 1, 10.0)
+    res["cpu"] = {"pairs_per_s": cpu_all, "cores": cores, "sample": sample_all,
+                  "pairs_per_s_1core": cpu_one, "sample_1core": sample_one, "cpu_model": cpu_model(),
+                  "build": "oracle restatement, gcc -O3 -march=x86-64-v3 -ffp-contract=fast, OpenMP over points"}
+    log(f"[{name}] cpu baseline {cpu_all:.3e} pairs/s on {cores} threads, {cpu_one:.3e} on 1 ({cpu_model()})")
+
+    full = {}
     for prec, key in ((na.Precision.FP64, "fp64"), (na.Precision.MIXED, "mixed")):
         if not supports(sd, prec, Xd, outd):
             log(f"[{name}] {key}: not available")
             continue
         ms, wall, kms = timed_eval(sd, Xd, outd, prec, steps, warmup, stream)
-        got = outd[:check_pts].cpu().numpy()
-        emax, e99 = rel_err(got, ref)
+        out_all = outd.cpu().numpy()
+        full[key] = out_all
+        emax, e99 = rel_err(out_all[:check_pts], ref)
         pairs = n * m
         res[key] = {"ms_per_step": ms, "wall_ms_per_step": wall, "pairs_per_s": pairs / (ms * 1e-3),
-                    "rel_err_max": emax, "rel_err_p99": e99, "kernel_ms": kms,
+                    "rel_err_max": emax, "rel_err_p99": e99, "checked_points": check_pts, "kernel_ms": kms,
                     "tflops": pairs * flop_per_pair(d) / (ms * 1e-3) / 1e12,
                     "kernel_tflops": pairs * flop_per_pair(d) / (kms * 1e-3) / 1e12}
         log(f"[{name}] {key}: {ms:.3f} ms/step (main kernel {kms:.3f} ms), {pairs / (ms * 1e-3):.3e} pairs/s, "
-            f"rel err max {emax:.2e} p99 {e99:.2e}")
+            f"rel err vs oracle on {check_pts} points: max {emax:.2e} p99 {e99:.2e}")
+    if "fp64" in full and "mixed" in full:
+        emax, e99 = rel_err(full["mixed"], full["fp64"])
+        res["mixed"]["vs_fp64_all_points"] = {"points": m, "rel_err_max": emax, "rel_err_p99": e99}
+        res["mixed_over_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
+        log(f"[{name}] mixed vs GPU FP64 on all {m} points: max {emax:.2e}; mixed is "
+            f"{res['mixed_over_fp64']:.1f}x FP64")
     return res
 
 
@@ -293,11 +347,13 @@ def apes_case(steps, precision):
     return res
 
 
-def c5_case(m, m_fp64):
+def c5_case(m, m_fp64, check_pts, oracle_mod):
     """Large VKDE: d=64, n=50000, Student-t nu=5, Gaussian target with
     condition number 1e4; prepare_interp (50000^2 interpolation matrix, NNLS
     weights) then eval_m2lnp at m points in mixed precision and at the first
-    m_fp64 of them in FP64 (the mixed results are checked against those)."""
+    m_fp64 of them in FP64.  Both are checked against the FP64 oracle
+    evaluating the device's factors and weights on the first check_pts
+    points, and the mixed result against the device FP64 one on m_fp64."""
     import torch
 
     import numcosmo_amd as na
@@ -311,7 +367,7 @@ def c5_case(m, m_fp64):
     t0 = time.perf_counter()
     sd.prepare_interp(m2lnp)
     prep = time.perf_counter() - t0
-    log(f"[C5] prepare_interp {prep:.1f} s, {sd.get_n_kernels()} kernels")
+    log(f"[C5] prepare_interp {prep:.1f} s, {sd.get_n_kernels()} kernels, NNLS rungs {sd.get_nnls_rungs()}")
     Xd = torch.tensor(Xe, dtype=torch.float64, device="cuda")
     outm = torch.empty(m, dtype=torch.float64, device="cuda")
     out64 = torch.empty(m_fp64, dtype=torch.float64, device="cuda")
@@ -319,13 +375,29 @@ def c5_case(m, m_fp64):
     nk = sd.get_n_kernels()
     ms_m, _, k_m = timed_eval(sd, Xd, outm, na.Precision.MIXED, 3, 1, stream)
     ms_f, _, k_f = timed_eval(sd, Xd[:m_fp64].contiguous(), out64, na.Precision.FP64, 1, 1, stream)
-    emax, e99 = rel_err(outm[:m_fp64].cpu().numpy(), out64.cpu().numpy())
-    res = {"d": d, "n": n, "m": m, "prepare_interp_s": prep,
-           "mixed": {"ms_per_step": ms_m, "pairs_per_s": nk * m / (ms_m * 1e-3)},
-           "fp64": {"m": m_fp64, "ms_per_step": ms_f, "pairs_per_s": nk * m_fp64 / (ms_f * 1e-3)},
+    got_m, got_f = outm.cpu().numpy(), out64.cpu().numpy()
+    emax, e99 = rel_err(got_m[:m_fp64], got_f)
+
+    # the oracle holding the device's factors and weights (lnnorms recomputed)
+    U = np.stack([sd.peek_cov_decomp(i) for i in range(nk)])
+    od = oracle_mod.StatsDist(oracle_mod.VKDE, oracle_mod.ST, d, 5.0)
+    od.set_prepared_vkde(X, U, sd.peek_weights(), sd.get_href())
+    del U
+    t0 = time.perf_counter()
+    ref = od.eval_m2lnp_batch(Xe[:check_pts])
+    t_or = time.perf_counter() - t0
+    om_max, om_99 = rel_err(got_m[:check_pts], ref)
+    of_max, of_99 = rel_err(got_f[:check_pts], ref)
+    res = {"d": d, "n": n, "m": m, "prepare_interp_s": prep, "nnls_rungs": list(sd.get_nnls_rungs()),
+           "mixed": {"ms_per_step": ms_m, "pairs_per_s": nk * m / (ms_m * 1e-3), "kernel_ms": k_m,
+                     "rel_err_vs_oracle_max": om_max, "rel_err_vs_oracle_p99": om_99},
+           "fp64": {"m": m_fp64, "ms_per_step": ms_f, "pairs_per_s": nk * m_fp64 / (ms_f * 1e-3), "kernel_ms": k_f,
+                    "rel_err_vs_oracle_max": of_max, "rel_err_vs_oracle_p99": of_99},
+           "oracle_checked_points": check_pts, "oracle_s": t_or,
            "mixed_vs_fp64_rel_err_max": emax, "mixed_vs_fp64_rel_err_p99": e99}
     log(f"[C5] mixed {ms_m:.1f} ms ({res['mixed']['pairs_per_s']:.3e} pairs/s), fp64 {ms_f:.1f} ms for "
-        f"{m_fp64} points ({res['fp64']['pairs_per_s']:.3e} pairs/s), mixed vs fp64 rel err max {emax:.2e}")
+        f"{m_fp64} points ({res['fp64']['pairs_per_s']:.3e} pairs/s); vs oracle on {check_pts} points "
+        f"({t_or:.0f} s): mixed {om_max:.2e}, fp64 {of_max:.2e}; mixed vs fp64 {emax:.2e}")
     return res
 
 
@@ -371,24 +443,30 @@ def cv_case(oracle_mod):
     return res
 
 
-def pmc_probe(steps):
-    """Short run for hardware-counter passes: the workload's mixed and FP64
-    evals, then a plain device copy of 256 MiB (the calibration kernel of the
-    FETCH_SIZE / WRITE_SIZE counters)."""
+def pmc_probe(case, steps):
+    """Short run for hardware-counter passes: one workload's mixed and FP64
+    evals (case "d16": d=16, n=20000, m=2e5, the headline; "c2": d=8,
+    n=4000, m=1e5), then a plain device copy of 256 MiB (the calibration
+    kernel of the FETCH_SIZE / WRITE_SIZE counters)."""
     import torch
 
     import numcosmo_amd as na
     from numcosmo_amd import synth
 
-    tgt, X, m2lnp, Xe = synth.workload("banana", 4000, 100000, 8, 2)
-    sd = na.StatsDistVKDE.new(na.StatsDistKernelST.new(8, 3.0), na.StatsDistCV.NONE)
+    if case == "d16":
+        d, n, m, target, cid = 16, 20000, 200000, "gaussian", 4
+    else:
+        d, n, m, target, cid = 8, 4000, 100000, "banana", 2
+    tgt, X, m2lnp, Xe = synth.workload(target, n, m, d, cid)
+    sd = na.StatsDistVKDE.new(na.StatsDistKernelST.new(d, 3.0), na.StatsDistCV.NONE)
     sd.add_obs_array(X)
     sd.prepare_interp(m2lnp)
     Xd = torch.tensor(Xe, dtype=torch.float64, device="cuda")
     outd = torch.empty(Xe.shape[0], dtype=torch.float64, device="cuda")
     stream = torch.cuda.Stream()
     for prec in (na.Precision.MIXED, na.Precision.FP64):
-        timed_eval(sd, Xd, outd, prec, steps, 1, stream)
+        ms, _, kms = timed_eval(sd, Xd, outd, prec, steps, 1, stream)
+        log(f"pmc probe {case}: precision {int(prec)} {ms:.3f} ms/step, main kernel {kms:.3f} ms")
     src = torch.empty(32 << 20, dtype=torch.float64, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
     for _ in range(steps):
@@ -402,11 +480,13 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-d16", action="store_true", help="skip the d=16, n=20000 VKDE eval")
+    ap.add_argument("--no-c2", action="store_true", help="skip the d=8, n=4000 banana VKDE eval")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc passes")
-    ap.add_argument("--apes-steps", type=int, default=40, help="APES ensemble steps (0: skip)")
-    ap.add_argument("--c5", action="store_true", help="also run the d=64, n=50000 VKDE (minutes)")
+    ap.add_argument("--pmc-probe", choices=["d16", "c2"], default=None,
+                    help="short run for rocprofv3 --pmc passes")
+    ap.add_argument("--apes-steps", type=int, default=200, help="APES ensemble steps (0: skip)")
+    ap.add_argument("--no-c5", action="store_true", help="skip the d=64, n=50000 VKDE")
+    ap.add_argument("--c5-check", type=int, default=10000, help="C5 points checked against the oracle")
     ap.add_argument("--no-cv", action="store_true", help="skip the cross-validated bandwidth fits")
     ap.add_argument("--only-c5", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512)
@@ -424,24 +504,24 @@ def main(argv=None):
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.pmc_probe:
-        pmc_probe(max(2, min(args.steps, 5)))
-        return 0
-
-    if args.only_c5:
-        print(json.dumps({"c5": c5_case(1000000, 100000)}))
+        pmc_probe(args.pmc_probe, max(2, min(args.steps, 5)))
         return 0
 
     from oracle import oracle as O
 
+    if args.only_c5:
+        print(json.dumps({"c5": c5_case(1000000, 100000, args.c5_check, O)}))
+        return 0
+
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
-    main_case = run_case("C2", "banana", 8, 4000, 100000, 3.0, args.steps, args.warmup, True, 4000, 2, O, cores)
-    ns = None
-    if not args.no_d16:
-        ns = run_case("d16", "gaussian", 16, 20000, 200000, 3.0, max(3, args.steps // 4), 1, False,
-                      200, 4, O, cores)
+    # the headline: VKDE eval_m2lnp at d=16, n=20000, m=2e5, mixed vs FP64
+    head = run_case("d16", "gaussian", 16, 20000, 200000, 3.0, args.steps, args.warmup, True, 10000, 4, O, cores)
+    c2 = None
+    if not args.no_c2:
+        c2 = run_case("C2", "banana", 8, 4000, 100000, 3.0, args.steps, args.warmup, True, 4000, 2, O, cores)
 
-    c5 = c5_case(1000000, 100000) if args.c5 else None
+    c5 = None if args.no_c5 else c5_case(1000000, 100000, args.c5_check, O)
 
     apes = None
     if args.apes_steps > 0:
@@ -455,19 +535,18 @@ def main(argv=None):
         latency = serve_case(args.serve_requests, args.serve_rate, args.serve_sessions, args.serve_slots)
 
     # HBM bytes per launch of the dominant kernel, from the rocprofv3 --pmc
-    # FETCH_SIZE / WRITE_SIZE passes (scaled by a 256 MiB copy) recorded in
-    # profiles/eval_m2lnp_vkde_c2_traffic.json
+    # FETCH_SIZE / WRITE_SIZE passes (scaled by a 256 MiB copy)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "eval_m2lnp_vkde_c2_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "eval_m2lnp_vkde_d16_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
-    best_key = "mixed" if ("mixed" in main_case and main_case["mixed"]["rel_err_max"] <= 1e-6) else "fp64"
-    best = main_case[best_key]
-    d = main_case["d"]
+    best_key = "mixed" if ("mixed" in head and head["mixed"]["rel_err_max"] <= 1e-6) else "fp64"
+    best = head[best_key]
     tflops = best["kernel_tflops"]
     peak = PEAK["f16"] if best_key == "mixed" else PEAK["f64"]
+    cpu = head["cpu"]
     line = {
         "metric": "kernel-pair evals/s, mixed vs FP64, at rel err<=1e-6; p50/p99 request latency",
         "value": best["pairs_per_s"],
@@ -481,17 +560,20 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": "f16split+f32" if best_key == "mixed" else "f64",
         "data": "synthetic",
-        "config": {"workload": "VKDE Student-t nu=3, d=8, banana, n=4000 kernels (NNLS-fitted weights), "
-                               "m=100000 eval points in HBM", "n_kernels": 4000, "d": d, "m": 100000,
-                   "precision": best_key},
+        "config": {"workload": "VKDE eval_m2lnp, Student-t nu=3, d=16, n=20000 kernels with NNLS-fitted weights "
+                               "(seeded correlated Gaussian target), m=200000 eval points resident in HBM",
+                   "n_kernels": 20000, "d": 16, "m": 200000, "precision": best_key},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": peak, "unit": "TFLOP/s",
                      "frac": tflops / peak, "traffic": traffic if best_key == "mixed" else None},
-        "cpu_baseline": {"value": main_case["cpu_pairs_per_s"], "unit": "pairs/s", "cores": cores,
-                         "kind": "port", "sample": main_case["cpu_sample"] + " (FP64 oracle eval_m2lnp)"},
-        "fp64": main_case.get("fp64"),
-        "mixed": main_case.get("mixed"),
-        "prepare_interp_s": main_case["prepare_s"],
-        "vkde_d16_n20000": ns,
+        "cpu_baseline": {"value": cpu["pairs_per_s"], "unit": "pairs/s", "cores": cpu["cores"], "kind": "port",
+                         "sample": cpu["sample"] + " (FP64 eval_m2lnp of the -O3 restatement)",
+                         "value_1core": cpu["pairs_per_s_1core"], "sample_1core": cpu["sample_1core"],
+                         "cpu_model": cpu["cpu_model"]},
+        "mixed_over_fp64": head.get("mixed_over_fp64"),
+        "fp64": head.get("fp64"),
+        "mixed": head.get("mixed"),
+        "prepare_interp_s": head["prepare_s"],
+        "c2_vkde_d8_n4000": c2,
         "latency": latency,
         "apes": apes,
         "c5": c5,

@@ -46,6 +46,10 @@ enum { NCKDE_CV_NONE = 0, NCKDE_CV_SPLIT = 1, NCKDE_CV_SPLIT_NOFIT = 2, NCKDE_CV
 enum { NCKDE_COV_SAMPLE = 0, NCKDE_COV_FIXED = 1, NCKDE_COV_ROBUST_DIAG = 2, NCKDE_COV_ROBUST = 3 };
 /* arithmetic of the pairwise distance and kernel sums in eval */
 enum { NCKDE_PREC_FP64 = 0, NCKDE_PREC_MIXED = 1 };
+/* NcmNNLSUMethod (ncm_nnls.h:51-60): the unconstrained solve of the active-set
+ * NNLS.  NORMAL: normal equations by Cholesky, else the symmetric solve, else
+ * QR; NORMAL_LU: symmetric solve, else QR; QR: least squares on A_P. */
+enum { NCKDE_NNLS_NORMAL = 0, NCKDE_NNLS_NORMAL_LU = 1, NCKDE_NNLS_QR = 2 };
 
 enum
 {
@@ -123,6 +127,22 @@ int nckde_get_cv_type (nckde *h, int *cv_type);
 /* No reference counterpart (measurement hook): objective evaluations (SPLIT:
  * IM + NNLS fits) and wall seconds of the last cross-validation fit */
 int nckde_get_cv_stats (nckde *h, unsigned *n_objective, double *seconds);
+/* ncm_nnls_set_umethod / get (ncm_nnls.h:66-67) of the object's weight fit,
+ * which the reference fixes to NCM_NNLS_UMETHOD_NORMAL (ncm_stats_dist.c:861);
+ * DGELSD and GSL give NCKDE_EUNSUPPORTED */
+int nckde_set_nnls_umethod (nckde *h, int umethod);
+int nckde_get_nnls_umethod (nckde *h, int *umethod);
+/* No reference counterpart: at most max_steps refinement steps after a
+ * Cholesky solve of the normal equations whose pivots spread by more than
+ * 1e6 (squared ratio of the extreme diagonal entries of the factor):
+ * delta = (A_P^T A_P)^{-1} A_P^T (f - A_P x), x += delta, a step kept only if
+ * the residual norm does not grow.  0 disables; the default is 2. */
+int nckde_set_nnls_refine (nckde *h, unsigned max_steps);
+int nckde_get_nnls_refine (nckde *h, unsigned *max_steps);
+/* No reference counterpart (diagnostic): unconstrained solves of the last
+ * prepare_interp by rung -- counts[0] Cholesky, [1] symmetric solve,
+ * [2] QR -- and counts[3] refinement steps taken */
+int nckde_get_nnls_rungs (nckde *h, unsigned counts[4]);
 
 /* ---- sample ------------------------------------------------------------ */
 

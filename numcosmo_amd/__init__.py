@@ -7,6 +7,6 @@ The compute library is ``libnckde.so`` (C-ABI in include/nckde.h), built by
 the library every call raises.
 """
 from .ncm_stats_dist import (  # noqa: F401
-    NckdeError, Precision, RNG, StatsDistCV, StatsDistKDE, StatsDistKDECovType,
+    NckdeError, NNLSUMethod, Precision, RNG, StatsDistCV, StatsDistKDE, StatsDistKDECovType,
     StatsDistKernelGauss, StatsDistKernelST, StatsDistVKDE,
 )

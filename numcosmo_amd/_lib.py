@@ -49,6 +49,11 @@ SIGNATURES = {
     "nckde_set_cv_type": (C.c_int, [_h, C.c_int]),
     "nckde_get_cv_type": (C.c_int, [_h, C.POINTER(C.c_int)]),
     "nckde_get_cv_stats": (C.c_int, [_h, C.POINTER(C.c_uint), C.POINTER(C.c_double)]),
+    "nckde_set_nnls_umethod": (C.c_int, [_h, C.c_int]),
+    "nckde_get_nnls_umethod": (C.c_int, [_h, C.POINTER(C.c_int)]),
+    "nckde_set_nnls_refine": (C.c_int, [_h, C.c_uint]),
+    "nckde_get_nnls_refine": (C.c_int, [_h, C.POINTER(C.c_uint)]),
+    "nckde_get_nnls_rungs": (C.c_int, [_h, C.POINTER(C.c_uint)]),
     "nckde_add_obs": (C.c_int, [_h, _dp]),
     "nckde_add_obs_batch": (C.c_int, [_h, _dp, _sz, _sz]),
     "nckde_reset": (C.c_int, [_h]),

@@ -109,6 +109,13 @@ struct nckde
   double href = 0.0, kernel_lnnorm = 0.0, rnorm = 0.0;
   // last NNLS fit: solution (before shrink), gradient A^T r and residual f - A x
   std::vector<double> nnls_x, nnls_g, nnls_r;
+  // unconstrained-solve method (NcmNNLSUMethod, ncm_nnls.h:51-60; NORMAL in
+  // ncm_stats_dist.c:861), refinement steps after a Cholesky solve, and the
+  // solves of the last prepare_interp per rung: Cholesky, symmetric solve,
+  // QR, refinement steps
+  int nnls_umethod      = NCKDE_NNLS_NORMAL;
+  unsigned nnls_refine  = 2;
+  unsigned nnls_rungs[4] = {0, 0, 0, 0};
   bool wcum_ready = false;
 
   hipStream_t stream      = nullptr;
@@ -129,6 +136,9 @@ struct nckde
   // work buffers
   DBuf dXe, dout, dpartM, dpartS, du, dz, dchi2, didx;
   DBuf dAT, dM, dS, dbv, dbs, dx, dxt, dr, drt, dg, dP, dpart, dscal, dm2lnp, dlu, dtv, drw, drobust, dpd;
+  // NNLS refinement vectors, the symmetric solve's pivot column and the QR
+  // rung's work (matrix, row map)
+  DBuf dref, dcolv, dqr, dqri;
   // cross-validation work buffers
   DBuf dcv1, dcv2, dorder;
 };
@@ -359,12 +369,16 @@ vkde_build_cov_array (nckde *h)
   HIPCHK (h->dstatus.ensure (sizeof (int) * 4));
   HIPCHK (hipMemsetAsync (h->dstatus.p, 0, sizeof (int) * 4, s));
 
-  // queries in batches so that the distance keys stay under ~512 MiB
+  // queries in batches so that the distance keys stay under ~512 MiB (none
+  // when each query's keys fit in LDS)
   const size_t budget = (size_t) 512 << 20;
-  size_t nq           = budget / (sizeof (uint64_t) * h->n_obs);
+  const bool in_lds   = nck::knn_keys_in_lds ((int) h->n_obs, (int) h->k);
+  size_t nq           = in_lds ? h->n_k : budget / (sizeof (uint64_t) * h->n_obs);
 
   nq = std::max<size_t> (1, std::min (nq, h->n_k));
-  HIPCHK (h->dkeys.ensure (sizeof (uint64_t) * h->n_obs * nq));
+
+  if (!in_lds)
+    HIPCHK (h->dkeys.ensure (sizeof (uint64_t) * h->n_obs * nq));
 
   for (size_t q0 = 0; q0 < h->n_k; q0 += nq)
   {
@@ -565,12 +579,72 @@ progress_enabled ()
   return on;
 }
 
+// A Cholesky solve of the normal equations is refined when its pivots span
+// more than this ratio (squared extreme diagonal entries of U): the normal
+// equations then lose more than six digits.
+constexpr double REFINE_PIVOT_RATIO = 1.0e6;
+
+// Corrected semi-normal refinement of x_P (in dbs) against A itself:
+// r = 1 - A_P x, delta = M_PP^{-1} A_P^T r with the factor in dS, x += delta.
+// Each step is kept only if the residual norm it leads to is not larger.
+static int
+nnls_refine (nckde *h, int n, int np, double *dxo)
+{
+  hipStream_t s  = h->stream;
+  const int nobs = (int) h->n_obs;
+  const int nb   = (nobs + 255) / 256;
+  double *c      = h->dref.as<double> ();
+  double *tmp    = c + n;
+  double *xprev  = tmp + n;
+  double *stat   = h->dpart.as<double> () + nb + 2;
+  double last    = -1.0;
+
+  for (unsigned step = 0;; ++step)
+  {
+    double hs[3];
+
+    HIPCHK (nck::launch_scatter (h->dbs.as<double> (), h->dP.as<int> (), np, n, dxo, s));
+    HIPCHK (nck::launch_residual (h->dAT.as<double> (), (int) h->n_k, nobs, dxo, nullptr, h->drt.as<double> (),
+                                  h->dpart.as<double> (), stat + 2, h->drw.as<double> (), s));
+    HIPCHK (hipMemcpyAsync (hs, stat, sizeof (double) * 3, hipMemcpyDeviceToHost, s));
+    HIPCHK (hipStreamSynchronize (s));
+
+    if (step > 0 && hs[2] > last)
+    {
+      // the step made the fit worse: keep the previous solution
+      HIPCHK (hipMemcpyAsync (h->dbs.p, xprev, sizeof (double) * np, hipMemcpyDeviceToDevice, s));
+      break;
+    }
+
+    if (step > 0 && hs[0] <= 1.0e-16 * hs[1])
+      break;
+
+    if (step == h->nnls_refine)
+      break;
+
+    last = hs[2];
+    HIPCHK (nck::launch_row_dot_idx (h->dAT.as<double> (), h->dP.as<int> (), np, nobs, h->drt.as<double> (), c, s));
+    HIPCHK (nck::launch_trsv_pair (h->dS.as<double> (), np, c, tmp, s));
+    HIPCHK (hipMemcpyAsync (xprev, h->dbs.p, sizeof (double) * np, hipMemcpyDeviceToDevice, s));
+    HIPCHK (nck::launch_add_update (h->dbs.as<double> (), c, np, stat, s));
+    h->nnls_rungs[3]++;
+  }
+
+  return NCKDE_OK;
+}
+
+// Unconstrained solve on the sorted index list P (ncm_nnls.c:655-706):
+// NORMAL: Cholesky of the normal matrix (refined against A when its pivots
+// are badly spread), else the symmetric solve, else QR on A_P; NORMAL_LU
+// starts at the symmetric solve; QR solves min |1 - A_P x| directly.
+// The solution is scattered into dxo.
 static int
 nnls_unconstrained (nckde *h, int n, const std::vector<int> &P, double *dxo)
 {
-  hipStream_t s = h->stream;
-  const int np  = (int) P.size ();
-  int info      = 0;
+  hipStream_t s  = h->stream;
+  const int np   = (int) P.size ();
+  const int nobs = (int) h->n_obs;
+  bool solved    = false;
 
   if (progress_enabled ())
   {
@@ -585,31 +659,65 @@ nnls_unconstrained (nckde *h, int n, const std::vector<int> &P, double *dxo)
   }
 
   HIPCHK (hipMemcpyAsync (h->dP.p, P.data (), sizeof (int) * np, hipMemcpyHostToDevice, s));
-  HIPCHK (nck::launch_gather_sym (h->dM.as<double> (), n, h->dP.as<int> (), np, h->dS.as<double> (),
-                                  h->dbv.as<double> (), h->dbs.as<double> (), s));
-  HIPCHK (hipMemsetAsync (h->dscal.p, 0, sizeof (int) * 2, s));
-  HIPCHK (h->dpd.ensure (sizeof (double) * nck::potrf_diag_doubles (np)));
-  HIPCHK (nck::launch_potrf (h->dS.as<double> (), np, h->dscal.as<int> (), h->dpd.as<double> (), s));
-  HIPCHK (hipMemcpyAsync (&info, h->dscal.p, sizeof (int), hipMemcpyDeviceToHost, s));
-  HIPCHK (hipStreamSynchronize (s));
 
-  if (info == 0)
+  if (h->nnls_umethod == NCKDE_NNLS_NORMAL)
   {
-    HIPCHK (nck::launch_trsv_pair (h->dS.as<double> (), np, h->dbs.as<double> (), h->dtv.as<double> (), s));
-  }
-  else
-  {
+    struct
+    {
+      int info, pad;
+      double dmax, dmin;
+    } st = {0, 0, 0.0, 0.0};
+
     HIPCHK (nck::launch_gather_sym (h->dM.as<double> (), n, h->dP.as<int> (), np, h->dS.as<double> (),
                                     h->dbv.as<double> (), h->dbs.as<double> (), s));
-    HIPCHK (nck::launch_lu_solve (h->dS.as<double> (), np, h->dbs.as<double> (), h->dscal.as<int> (), s));
+    HIPCHK (hipMemsetAsync (h->dscal.p, 0, sizeof (int) * 2, s));
+    HIPCHK (h->dpd.ensure (sizeof (double) * nck::potrf_diag_doubles (np)));
+    HIPCHK (nck::launch_potrf (h->dS.as<double> (), np, h->dscal.as<int> (), h->dpd.as<double> (), s));
+    HIPCHK (nck::launch_diag_range (h->dS.as<double> (), np, h->dscal.as<double> () + 1, s));
+    HIPCHK (hipMemcpyAsync (&st, h->dscal.p, sizeof (st), hipMemcpyDeviceToHost, s));
+    HIPCHK (hipStreamSynchronize (s));
+
+    if (st.info == 0)
+    {
+      HIPCHK (nck::launch_trsv_pair (h->dS.as<double> (), np, h->dbs.as<double> (), h->dtv.as<double> (), s));
+      h->nnls_rungs[0]++;
+
+      if (h->nnls_refine > 0 && st.dmax * st.dmax >= REFINE_PIVOT_RATIO * st.dmin * st.dmin)
+        TRY (nnls_refine (h, n, np, dxo));
+
+      solved = true;
+    }
+  }
+
+  if (!solved && h->nnls_umethod != NCKDE_NNLS_QR)
+  {
+    int info = 0;
+
+    HIPCHK (nck::launch_gather_sym (h->dM.as<double> (), n, h->dP.as<int> (), np, h->dS.as<double> (),
+                                    h->dbv.as<double> (), h->dbs.as<double> (), s));
+    HIPCHK (hipMemsetAsync (h->dscal.p, 0, sizeof (int) * 2, s));
+    HIPCHK (nck::launch_lu_factor (h->dS.as<double> (), np, h->dbs.as<double> (), h->dcolv.as<double> (),
+                                   h->dscal.as<int> (), s));
     HIPCHK (hipMemcpyAsync (&info, h->dscal.p, sizeof (int), hipMemcpyDeviceToHost, s));
     HIPCHK (hipStreamSynchronize (s));
 
-    if (info != 0)
+    if (info == 0)
     {
-      set_error ("ncm_nnls_solve: singular normal equations (column set of size %d)", np);
-      return NCKDE_ENUMERIC;
+      HIPCHK (nck::launch_mirror_lower (h->dS.as<double> (), np, s));
+      HIPCHK (nck::launch_trsv_backward (h->dS.as<double> (), np, h->dbs.as<double> (), h->dtv.as<double> (), s));
+      HIPCHK (hipMemcpyAsync (h->dbs.p, h->dtv.p, sizeof (double) * np, hipMemcpyDeviceToDevice, s));
+      h->nnls_rungs[1]++;
+      solved = true;
     }
+  }
+
+  if (!solved)
+  {
+    HIPCHK (h->dqr.ensure (sizeof (double) * nck::qr_work_doubles (nobs, np)));
+    HIPCHK (h->dqri.ensure (sizeof (int) * (np + 1)));
+    HIPCHK (nck::launch_qr_lstsq (h->dAT.as<double> (), nobs, h->dP.as<int> (), np, nullptr, h->dqr.as<double> (),
+                                  h->dqri.as<int> (), h->dbs.as<double> (), s));
+    h->nnls_rungs[2]++;
   }
 
   HIPCHK (nck::launch_scatter (h->dbs.as<double> (), h->dP.as<int> (), np, n, dxo, s));
@@ -697,8 +805,11 @@ nnls_solve (nckde *h)
   const double reltol = 2.220446049250313e-16;
   hipStream_t s    = h->stream;
 
-  HIPCHK (h->dM.ensure (sizeof (double) * n * n));
-  HIPCHK (h->dS.ensure (sizeof (double) * n * n));
+  if (h->nnls_umethod != NCKDE_NNLS_QR)
+  {
+    HIPCHK (h->dM.ensure (sizeof (double) * n * n));
+    HIPCHK (h->dS.ensure (sizeof (double) * n * n));
+  }
   HIPCHK (h->dbv.ensure (sizeof (double) * n));
   HIPCHK (h->dbs.ensure (sizeof (double) * n));
   HIPCHK (h->dtv.ensure (sizeof (double) * n));
@@ -709,12 +820,17 @@ nnls_solve (nckde *h)
   HIPCHK (h->drt.ensure (sizeof (double) * nobs));
   HIPCHK (h->dg.ensure (sizeof (double) * n));
   HIPCHK (h->dP.ensure (sizeof (int) * n));
-  HIPCHK (h->dpart.ensure (sizeof (double) * ((nobs + 255) / 256 + 4)));
+  HIPCHK (h->dpart.ensure (sizeof (double) * ((nobs + 255) / 256 + 8)));
   HIPCHK (h->dscal.ensure (64));
+  HIPCHK (h->dref.ensure (sizeof (double) * 3 * n));
+  HIPCHK (h->dcolv.ensure (sizeof (double) * n));
 
-  // M = A^T A, b = A^T f
-  HIPCHK (nck::launch_gram_nt (h->dAT.as<double> (), n, nobs, h->dM.as<double> (), s));
-  HIPCHK (nck::launch_row_dot (h->dAT.as<double> (), n, nobs, nullptr, h->dbv.as<double> (), s));
+  // M = A^T A, b = A^T f (the QR method works on A alone, ncm_nnls.c:795-797)
+  if (h->nnls_umethod != NCKDE_NNLS_QR)
+  {
+    HIPCHK (nck::launch_gram_nt (h->dAT.as<double> (), n, nobs, h->dM.as<double> (), s));
+    HIPCHK (nck::launch_row_dot (h->dAT.as<double> (), n, nobs, nullptr, h->dbv.as<double> (), s));
+  }
 
   std::vector<char> P (n, 1), Ptry (n);
   std::vector<double> x (n), xt (n), g (n);
@@ -1739,6 +1855,95 @@ nckde_get_cv_stats (nckde *h, unsigned *n_objective, double *seconds)
 }
 
 int
+nckde_set_nnls_umethod (nckde *h, int v)
+{
+  TRY (check_handle (h));
+
+  if (v == 3 || v == 4)
+  {
+    set_error ("nckde_set_nnls_umethod: the DGELSD and GSL methods are not implemented");
+    return NCKDE_EUNSUPPORTED;
+  }
+
+  if (v < NCKDE_NNLS_NORMAL || v > NCKDE_NNLS_QR)
+  {
+    set_error ("nckde_set_nnls_umethod: unknown method %d", v);
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+  h->nnls_umethod = v;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_nnls_umethod (nckde *h, int *v)
+{
+  TRY (check_handle (h));
+
+  if (v == nullptr)
+  {
+    set_error ("nckde_get_nnls_umethod: NULL output");
+    return NCKDE_EINVAL;
+  }
+
+  *v = h->nnls_umethod;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_nnls_refine (nckde *h, unsigned v)
+{
+  TRY (check_handle (h));
+
+  if (v > 16)
+  {
+    set_error ("nckde_set_nnls_refine: at most 16 steps");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+  h->nnls_refine = v;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_nnls_refine (nckde *h, unsigned *v)
+{
+  TRY (check_handle (h));
+
+  if (v == nullptr)
+  {
+    set_error ("nckde_get_nnls_refine: NULL output");
+    return NCKDE_EINVAL;
+  }
+
+  *v = h->nnls_refine;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_nnls_rungs (nckde *h, unsigned counts[4])
+{
+  TRY (check_handle (h));
+
+  if (counts == nullptr)
+  {
+    set_error ("nckde_get_nnls_rungs: NULL output");
+    return NCKDE_EINVAL;
+  }
+
+  for (int i = 0; i < 4; ++i)
+    counts[i] = h->nnls_rungs[i];
+
+  return NCKDE_OK;
+}
+
+int
 nckde_set_split_frac (nckde *h, double v)
 {
   TRY (check_handle (h));
@@ -2139,6 +2344,9 @@ nckde_prepare_interp (nckde *h, const double *m2lnp, size_t n)
   }
 
   std::lock_guard<std::mutex> lk (h->mtx);
+
+  for (unsigned &c : h->nnls_rungs)
+    c = 0;
 
   return do_prepare_interp (h, m2lnp, n);
 }

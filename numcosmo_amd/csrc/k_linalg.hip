@@ -15,16 +15,25 @@ namespace nck
 
 // --------------------------------------------------------------------------
 // A[j][i] = K(chi2_ij) / (u_i h^d) / f_j,  f_j = exp(-(m2lnp_j - min)/2)
+// One thread per observation j, a block per (256 observations, kernel
+// chunk).  The per-kernel normalisation 1 / exp(lnnorm_i + d ln h) is formed
+// once per block for 256 kernels at a time in LDS, and the Student-t kernel
+// is exp(kappa log1p(chi2 / nu)) (equal to the reference's pow form to a few
+// ulp).  Threads past the last observation take part in the LDS staging only.
 // --------------------------------------------------------------------------
+constexpr int IM_KB = 256;
+
 template <int D, int KERNEL, int KIND>
 __global__ void __launch_bounds__ (256)
 k_build_IM (IMArgs a)
 {
+  __shared__ double kscale[IM_KB];
   const int j     = blockIdx.x * 256 + threadIdx.x;
   const int chunk = (a.n_k + gridDim.y - 1) / gridDim.y;
   const int i0    = blockIdx.y * chunk;
   const int i1    = min (a.n_k, i0 + chunk);
-  const int jj    = (j < a.n_obs) ? j : (a.n_obs - 1);
+  const bool live = (j < a.n_obs);
+  const int jj    = live ? j : (a.n_obs - 1);
   const int d     = a.d;
   double xj[D];
 
@@ -46,61 +55,79 @@ k_build_IM (IMArgs a)
   if (a.m2lnp)
     rowscale = 1.0 / exp (-0.5 * (a.m2lnp[jj] - a.min_m2lnp));
 
-  const double kappa = -0.5 * (a.nu + (double) (unsigned) d);
+  const double kappa  = -0.5 * (a.nu + (double) (unsigned) d);
+  const double inv_nu = 1.0 / a.nu;
 
-  for (int i = i0; i < i1; ++i)
+  for (int ib = i0; ib < i1; ib += IM_KB)
   {
-    const double *th = a.Th + (size_t) i * D;
-    double chi2      = 0.0;
-    double scale;
+    const int ie = min (i1, ib + IM_KB);
 
     if (KIND == NCKDE_VKDE)
     {
-      const double *W = a.Wp + (size_t) i * (D * (D + 1) / 2);
-      double delta[D];
+      __syncthreads ();
 
-#pragma unroll
-      for (int k = 0; k < D; ++k)
-        delta[k] = xj[k] - th[k];
+      if (ib + (int) threadIdx.x < ie)
+        kscale[threadIdx.x] = 1.0 / exp (a.lnnorm[ib + threadIdx.x] + a.lnnorm_href);
 
-#pragma unroll
-      for (int r = 0; r < D; ++r)
-      {
-        double z = 0.0;
-
-#pragma unroll
-        for (int c = 0; c <= r; ++c)
-          z = fma (W[r * (r + 1) / 2 + c], delta[c], z);
-
-        chi2 = fma (z, z, chi2);
-      }
-
-      chi2  = chi2 * a.inv_h2;
-      scale = 1.0 / exp (a.lnnorm[i] + a.lnnorm_href);
+      __syncthreads ();
     }
-    else
+
+    if (!live)
+      continue;
+
+    for (int i = ib; i < ie; ++i)
     {
-#pragma unroll
-      for (int k = 0; k < D; ++k)
-      {
-        const double t = th[k] - xj[k];
+      const double *th = a.Th + (size_t) i * D;
+      double chi2      = 0.0;
+      double scale;
 
-        chi2 = fma (t, t, chi2);
+      if (KIND == NCKDE_VKDE)
+      {
+        const double *W = a.Wp + (size_t) i * (D * (D + 1) / 2);
+        double delta[D];
+
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+          delta[k] = xj[k] - th[k];
+
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+        {
+          double z = 0.0;
+
+#pragma unroll
+          for (int c = 0; c <= r; ++c)
+            z = fma (W[r * (r + 1) / 2 + c], delta[c], z);
+
+          chi2 = fma (z, z, chi2);
+        }
+
+        chi2  = chi2 * a.inv_h2;
+        scale = kscale[i - ib];
+      }
+      else
+      {
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+        {
+          const double t = th[k] - xj[k];
+
+          chi2 = fma (t, t, chi2);
+        }
+
+        chi2  = chi2 * a.inv_h2;
+        scale = a.kde_scale;
       }
 
-      chi2  = chi2 / a.h2;
-      scale = a.kde_scale;
-    }
+      double K;
 
-    double K;
+      if (KERNEL == NCKDE_KERNEL_GAUSS)
+        K = exp (-0.5 * chi2);
+      else
+        K = exp (kappa * log1p (chi2 * inv_nu));
 
-    if (KERNEL == NCKDE_KERNEL_GAUSS)
-      K = exp (-0.5 * chi2);
-    else
-      K = pow (1.0 + chi2 / a.nu, kappa);
-
-    if (j < a.n_obs)
       a.AT[(size_t) i * a.n_obs + j] = (K * scale) * rowscale;
+    }
   }
 }
 
@@ -1250,6 +1277,32 @@ launch_trsv_pair (const double *S, int np, double *b, double *tmp, hipStream_t s
 
     hipLaunchKernelGGL (k_trsv_panel<0>, dim3 (grid), dim3 (256), 0, s, S, np, p0, pb, tmp, b);
   }
+
+  return hipGetLastError ();
+}
+
+// U x = y alone, U in the upper triangle and U^T mirrored into the lower one
+// (the factor of launch_lu_factor after launch_mirror_lower); y is consumed
+hipError_t
+launch_trsv_backward (const double *S, int np, double *y, double *x, hipStream_t s)
+{
+  const int last = ((np - 1) / TS_NB) * TS_NB;
+
+  for (int p0 = last; p0 >= 0; p0 -= TS_NB)
+  {
+    const int pb   = (np - p0 < TS_NB) ? np - p0 : TS_NB;
+    const int grid = (p0 + 255) / 256 > 0 ? (p0 + 255) / 256 : 1;
+
+    hipLaunchKernelGGL (k_trsv_panel<0>, dim3 (grid), dim3 (256), 0, s, S, np, p0, pb, y, x);
+  }
+
+  return hipGetLastError ();
+}
+
+hipError_t
+launch_mirror_lower (double *S, int np, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_mirror_lower, dim3 ((np + 255) / 256, np), dim3 (256), 0, s, S, np);
 
   return hipGetLastError ();
 }

@@ -845,11 +845,320 @@ k_knn (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, uint64
     o[i] = sidx[i];
 }
 
+// The same selection with the query's keys in LDS when they fit (small
+// samples, e.g. an APES half-ensemble): no key array in HBM, the radix passes
+// read LDS, the histogram is kept per wave (4 x 256 bins) so that lanes of
+// different waves never contend, and the leading bytes that every key shares
+// (those of the smallest and largest key) are taken without a pass.
+// Dynamic LDS: keys[n], then the sort buffers skey[npow], sidx[npow].
+constexpr int KNN_LDS_BYTES = 56 * 1024;
+
+static int
+knn_npow (int k)
+{
+  int p = 1;
+
+  while (p < k)
+    p <<= 1;
+
+  return p;
+}
+
+static size_t
+knn_lds_bytes (int n, int k)
+{
+  return (size_t) n * sizeof (uint64_t) + (size_t) knn_npow (k) * (sizeof (uint64_t) + sizeof (int));
+}
+
+bool
+knn_keys_in_lds (int n, int k)
+{
+  return knn_lds_bytes (n, k) <= (size_t) KNN_LDS_BYTES;
+}
+
+__global__ void __launch_bounds__ (KNN_THREADS)
+k_knn_lds (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, int npow, int *__restrict__ out,
+           int *__restrict__ status)
+{
+  extern __shared__ uint64_t knn_smem[];
+  uint64_t *keys = knn_smem;
+  uint64_t *skey = knn_smem + n;
+  int *sidx      = (int *) (skey + npow);
+  __shared__ unsigned hist[KNN_THREADS / 64][256];
+  __shared__ uint64_t s_prefix, s_wmin[KNN_THREADS / 64], s_wmax[KNN_THREADS / 64];
+  __shared__ int s_kk, s_nlt, s_eq_taken, s_pass0;
+  __shared__ int s_scan[KNN_THREADS];
+  __shared__ unsigned s_wsum[KNN_THREADS / 64];
+
+  const int qb   = blockIdx.x;
+  const int tid  = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+
+  if (qb >= nq)
+    return;
+
+  const int q      = q0 + qb;
+  const double *yq = Y + (size_t) q * d;
+
+  // 1. distances (coordinates summed in order, as the global-memory kernel)
+  uint64_t kmin = ~(uint64_t) 0, kmax = 0;
+
+  for (int j = tid; j < n; j += KNN_THREADS)
+  {
+    const double *yj = Y + (size_t) j * d;
+    double dist      = 0.0;
+
+    for (int l = 0; l < d; l++)
+    {
+      const double t = yj[l] - yq[l];
+
+      dist = dist + t * t;
+    }
+
+    const uint64_t key = (uint64_t) __double_as_longlong (dist);
+
+    keys[j] = key;
+    kmin    = (key < kmin) ? key : kmin;
+    kmax    = (key > kmax) ? key : kmax;
+  }
+
+  for (int o = 32; o > 0; o >>= 1)
+  {
+    const uint64_t a = __shfl_xor (kmin, o, 64);
+    const uint64_t b = __shfl_xor (kmax, o, 64);
+
+    kmin = (a < kmin) ? a : kmin;
+    kmax = (b > kmax) ? b : kmax;
+  }
+
+  if (lane == 0)
+  {
+    s_wmin[wave] = kmin;
+    s_wmax[wave] = kmax;
+  }
+
+  __syncthreads ();
+
+  if (tid == 0)
+  {
+    uint64_t mn = s_wmin[0], mx = s_wmax[0];
+
+    for (int w = 1; w < KNN_THREADS / 64; ++w)
+    {
+      mn = (s_wmin[w] < mn) ? s_wmin[w] : mn;
+      mx = (s_wmax[w] > mx) ? s_wmax[w] : mx;
+    }
+
+    // whole leading bytes shared by every key
+    const uint64_t x = mn ^ mx;
+    const int lead   = (x == 0) ? 8 : __clzll ((long long) x) / 8;
+
+    s_pass0  = lead;
+    s_prefix = (lead == 0) ? 0 : (lead == 8 ? mn : (mn & (~(uint64_t) 0 << (64 - 8 * lead))));
+    s_kk     = k;
+  }
+
+  __syncthreads ();
+
+  // 2. radix select of the k-th smallest key over the remaining bytes
+  const int pass0 = s_pass0;
+  uint64_t mask   = (pass0 == 0) ? 0 : (pass0 == 8 ? ~(uint64_t) 0 : (~(uint64_t) 0 << (64 - 8 * pass0)));
+
+  for (int pass = pass0; pass < 8; ++pass)
+  {
+    const int shift = 56 - 8 * pass;
+
+    for (int b = tid; b < 256 * (KNN_THREADS / 64); b += KNN_THREADS)
+      (&hist[0][0])[b] = 0;
+
+    __syncthreads ();
+
+    const uint64_t prefix = s_prefix;
+
+    for (int j = tid; j < n; j += KNN_THREADS)
+    {
+      const uint64_t key = keys[j];
+
+      if ((key & mask) == prefix)
+        atomicAdd (&hist[wave][(key >> shift) & 255u], 1u);
+    }
+
+    __syncthreads ();
+
+    {
+      unsigned h = 0;
+
+      for (int w = 0; w < KNN_THREADS / 64; ++w)
+        h += hist[w][tid];
+
+      unsigned v = h;
+
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1)
+      {
+        const unsigned t = __shfl_up (v, o, 64);
+
+        if (lane >= o)
+          v += t;
+      }
+
+      if (lane == 63)
+        s_wsum[wave] = v;
+
+      const int kk = s_kk;
+
+      __syncthreads ();
+
+      unsigned base = 0;
+
+      for (int w = 0; w < wave; ++w)
+        base += s_wsum[w];
+
+      const unsigned incl = base + v;
+      const unsigned excl = incl - h;
+
+      if (excl < (unsigned) kk && incl >= (unsigned) kk)
+      {
+        s_kk     = kk - (int) excl;
+        s_prefix = prefix | ((uint64_t) tid << shift);
+      }
+    }
+
+    mask |= (uint64_t) 255 << shift;
+    __syncthreads ();
+  }
+
+  const uint64_t T  = s_prefix;
+  const int need_eq = s_kk;
+
+  // 3. collect: keys < T, then the need_eq smallest indices with key == T
+  if (tid == 0)
+  {
+    s_nlt      = 0;
+    s_eq_taken = 0;
+  }
+
+  __syncthreads ();
+
+  for (int j = tid; j < n; j += KNN_THREADS)
+  {
+    const uint64_t key = keys[j];
+
+    if (key < T)
+    {
+      const int slot = atomicAdd (&s_nlt, 1);
+
+      if (slot < npow)
+      {
+        skey[slot] = key;
+        sidx[slot] = j;
+      }
+    }
+  }
+
+  __syncthreads ();
+
+  const int nlt = s_nlt;
+
+  for (int base = 0; base < n; base += KNN_THREADS)
+  {
+    if (s_eq_taken >= need_eq)
+      break;
+
+    const int j    = base + tid;
+    const int flag = (j < n && keys[j] == T) ? 1 : 0;
+
+    s_scan[tid] = flag;
+    __syncthreads ();
+
+    for (int off = 1; off < KNN_THREADS; off <<= 1)
+    {
+      const int v = (tid >= off) ? s_scan[tid - off] : 0;
+
+      __syncthreads ();
+      s_scan[tid] += v;
+      __syncthreads ();
+    }
+
+    const int taken = s_eq_taken;
+    const int rank  = s_scan[tid] - flag;
+
+    if (flag && taken + rank < need_eq)
+    {
+      const int slot = nlt + taken + rank;
+
+      if (slot < npow)
+      {
+        skey[slot] = T;
+        sidx[slot] = j;
+      }
+    }
+
+    __syncthreads ();
+
+    if (tid == KNN_THREADS - 1)
+      s_eq_taken = taken + s_scan[KNN_THREADS - 1];
+
+    __syncthreads ();
+  }
+
+  if (nlt + need_eq != k && tid == 0)
+    status[0] = 2;
+
+  // 4. bitonic sort of (key, idx) over npow slots
+  for (int i = k + tid; i < npow; i += KNN_THREADS)
+  {
+    skey[i] = ~(uint64_t) 0;
+    sidx[i] = 0x7fffffff;
+  }
+
+  __syncthreads ();
+
+  for (int size = 2; size <= npow; size <<= 1)
+  {
+    for (int stride = size >> 1; stride > 0; stride >>= 1)
+    {
+      for (int i = tid; i < npow; i += KNN_THREADS)
+      {
+        const int pj = i ^ stride;
+
+        if (pj > i)
+        {
+          const bool up     = ((i & size) == 0);
+          const uint64_t ka = skey[i], kb = skey[pj];
+          const int ia      = sidx[i], ib = sidx[pj];
+          const bool a_gt_b = (ka > kb) || (ka == kb && ia > ib);
+
+          if (a_gt_b == up)
+          {
+            skey[i]  = kb;
+            skey[pj] = ka;
+            sidx[i]  = ib;
+            sidx[pj] = ia;
+          }
+        }
+      }
+
+      __syncthreads ();
+    }
+  }
+
+  int *o = out + (size_t) q * k;
+
+  for (int i = tid; i < k; i += KNN_THREADS)
+    o[i] = sidx[i];
+}
+
 hipError_t
 launch_knn (const double *Y, int n, int d, int q0, int nq, int k, uint64_t *keys, int *out, int *status,
             hipStream_t s)
 {
-  hipLaunchKernelGGL (k_knn, dim3 (nq), dim3 (KNN_THREADS), 0, s, Y, n, d, q0, nq, k, keys, out, status);
+  if (knn_keys_in_lds (n, k))
+    hipLaunchKernelGGL (k_knn_lds, dim3 (nq), dim3 (KNN_THREADS), knn_lds_bytes (n, k), s, Y, n, d, q0, nq, k,
+                        knn_npow (k), out, status);
+  else
+    hipLaunchKernelGGL (k_knn, dim3 (nq), dim3 (KNN_THREADS), 0, s, Y, n, d, q0, nq, k, keys, out, status);
 
   return hipGetLastError ();
 }

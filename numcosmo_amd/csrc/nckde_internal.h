@@ -101,6 +101,8 @@ hipError_t launch_nearpd_fix (const int *which, int nfix, const double *cov, dou
                               double *scratch, int d, int maxiter, int kernel, double lnnorm_const,
                               int *status, hipStream_t s);
 hipError_t launch_whiten (const double *X, const double *Ug, double *Y, int n, int d, hipStream_t s);
+// true when the kNN selection keeps a query's keys in LDS (no key buffer)
+bool knn_keys_in_lds (int n, int k);
 hipError_t launch_knn (const double *Y, int n, int d, int q0, int nq, int k, uint64_t *keys,
                        int *out, int *status, hipStream_t s);
 size_t knn_max_k ();
@@ -191,6 +193,17 @@ size_t potrf_diag_doubles (int np);
 hipError_t launch_potrf (double *S, int np, int *info, double *diagbuf, hipStream_t s);
 hipError_t launch_trsv_pair (const double *S, int np, double *b, double *tmp, hipStream_t s);
 hipError_t launch_lu_solve (double *S, int np, double *b, int *info, hipStream_t s);
+hipError_t launch_trsv_backward (const double *S, int np, double *y, double *x, hipStream_t s);
+hipError_t launch_mirror_lower (double *S, int np, hipStream_t s);
+// k_nnls.hip: the symmetric-solve and QR rungs, refinement helpers
+hipError_t launch_lu_factor (double *S, int np, double *b, double *colv, int *info, hipStream_t s);
+size_t qr_work_doubles (int nobs, int np);
+hipError_t launch_qr_lstsq (const double *AT, int nobs, const int *P, int np, const double *f, double *work,
+                            int *iwork, double *x, hipStream_t s);
+hipError_t launch_row_dot_idx (const double *AT, const int *P, int np, int K, const double *v, double *out,
+                               hipStream_t s);
+hipError_t launch_add_update (double *x, const double *dx, int np, double *stat, hipStream_t s);
+hipError_t launch_diag_range (const double *S, int np, double *out, hipStream_t s);
 hipError_t launch_scatter (const double *sol, const int *P, int np, int n, double *x, hipStream_t s);
 hipError_t launch_shrink (double *w, int n, double shrink, int *status, hipStream_t s);
 

@@ -29,7 +29,7 @@ import numpy as np
 from ._lib import NckdeError, check, lib
 
 __all__ = [
-    "StatsDistCV", "StatsDistKDECovType", "Precision", "RNG", "NckdeError",
+    "StatsDistCV", "StatsDistKDECovType", "Precision", "NNLSUMethod", "RNG", "NckdeError",
     "StatsDistKernelGauss", "StatsDistKernelST", "StatsDistKDE", "StatsDistVKDE", "deserialize",
 ]
 
@@ -53,6 +53,14 @@ class StatsDistKDECovType(enum.IntEnum):
 class Precision(enum.IntEnum):
     FP64 = 0
     MIXED = 1
+
+
+class NNLSUMethod(enum.IntEnum):
+    """NcmNNLSUMethod (ncm_nnls.h:51-60) of the weight fit's unconstrained
+    solve; DGELSD and GSL are not implemented."""
+    NORMAL = 0
+    NORMAL_LU = 1
+    QR = 2
 
 
 class RNG:
@@ -176,6 +184,33 @@ class _StatsDist:
         n, t = C.c_uint(), C.c_double()
         check(lib().nckde_get_cv_stats(self._h, C.byref(n), C.byref(t)))
         return n.value, t.value
+
+    def set_nnls_umethod(self, umethod):
+        """ncm_nnls_set_umethod (ncm_nnls.h:66) of the weight fit (NORMAL in
+        the reference, ncm_stats_dist.c:861)."""
+        check(lib().nckde_set_nnls_umethod(self._h, int(umethod)))
+
+    def get_nnls_umethod(self):
+        v = C.c_int()
+        check(lib().nckde_get_nnls_umethod(self._h, C.byref(v)))
+        return NNLSUMethod(v.value)
+
+    def set_nnls_refine(self, max_steps):
+        """Refinement steps after a badly spread Cholesky solve (0 disables);
+        no reference counterpart."""
+        check(lib().nckde_set_nnls_refine(self._h, int(max_steps)))
+
+    def get_nnls_refine(self):
+        v = C.c_uint()
+        check(lib().nckde_get_nnls_refine(self._h, C.byref(v)))
+        return v.value
+
+    def get_nnls_rungs(self):
+        """Unconstrained solves of the last prepare_interp per rung:
+        (Cholesky, symmetric solve, QR, refinement steps)."""
+        c = (C.c_uint * 4)()
+        check(lib().nckde_get_nnls_rungs(self._h, c))
+        return tuple(int(v) for v in c)
 
     def set_shrink(self, v):
         check(lib().nckde_set_shrink(self._h, float(v)))

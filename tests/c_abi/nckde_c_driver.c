@@ -292,7 +292,12 @@ main (void)
     buf = malloc (len);
     CALL (nckde_serialize (h, 1, buf, len, &len));
     CALL (nckde_deserialize (&h2, buf, len));
-    CHECK (nckde_deserialize (&h2, buf, len / 2) == NCKDE_EINVAL, "truncated serialisation accepted");
+    {
+      nckde *h3 = NULL;
+
+      CHECK (nckde_deserialize (&h3, buf, len / 2) == NCKDE_EINVAL && h3 == NULL,
+             "truncated serialisation accepted");
+    }
     CALL (nckde_eval_m2lnp (h2, Xe, M, D, got2, NCKDE_PREC_FP64));
     CALL (nckde_eval_m2lnp (h, Xe, M, D, got, NCKDE_PREC_FP64));
     CHECK (memcmp (got, got2, sizeof (double) * M) == 0, "restored object evaluates differently");

@@ -193,16 +193,24 @@ def test_vkde_st5_d64_cond_n50000(oracle):
     od.set_weights(x)
     r_rows = 1.0 - od.eval_batch(X[rows]) / f[rows]
     assert np.allclose(r[rows], r_rows, rtol=1e-9, atol=1e-12)
-    # gradient g_j = sum_i A_ij r_i on sampled kernels, A_ij = IM_ij / f_i:
-    # column j of IM is the linear density of kernel j alone at every theta_i
-    for j in np.sort(g.choice(n, size=16, replace=False)):
+    # gradient g_j = sum_i A_ij r_i, A_ij = IM_ij / f_i, on 16 random kernels
+    # and on the 16 worst KKT offenders among the active and the inactive
+    # ones: column j of IM is the linear density of kernel j alone at every
+    # theta_i.  KKT per column, relative to the column's size sum_i A_ij
+    # (a solve's backward error bounds |g_j| by a multiple of 2^-52 of it):
+    # |g_j| for active j, max(g_j, 0) for inactive j.
+    act = x > 0
+    viol = np.where(act, np.abs(gr), np.maximum(gr, 0.0))
+    cols = set(int(j) for j in g.choice(n, size=16, replace=False))
+    cols |= set(int(j) for j in np.argsort(np.where(act, viol, -1.0))[-16:])
+    cols |= set(int(j) for j in np.argsort(np.where(act, -1.0, viol))[-16:])
+    worst = 0.0
+    for j in sorted(cols):
         oj = oracle.StatsDist(oracle.VKDE, oracle.ST, d, nu)
         oj.set_prepared_vkde(X[j:j + 1], U[j:j + 1], np.ones(1), sd.get_href())
         Aj = oj.eval_batch(X) / f
         gj = Aj @ r
         assert gj == pytest.approx(gr[j], rel=1e-8, abs=1e-10 * np.abs(Aj).sum())
-    scale = np.abs(gr).max()
-    act = x > 0
-    kkt = max(np.abs(gr[act]).max(initial=0.0), gr[~act].max(initial=0.0)) / max(scale, 1e-300)
-    _say(f"   KKT residual {kkt:.2e}, {int(act.sum())} active weights")
-    assert kkt < 1e-6
+        worst = max(worst, viol[j] / Aj.sum())
+    _say(f"   KKT residual {worst:.2e} over {len(cols)} columns, {int(act.sum())} active weights")
+    assert worst < 1e-8

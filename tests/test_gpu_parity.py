@@ -129,16 +129,31 @@ def test_prepare_interp_weights(gpu, oracle, kind, kernel, d, n, nu):
     _say(f"   weights rel L1 {rel_l1:.3e}, rnorm {sd.get_rnorm():.6e} vs {od.get_rnorm():.6e}")
     assert rel_l1 <= 1e-7
     assert sd.get_rnorm() == pytest.approx(od.get_rnorm(), rel=1e-7, abs=1e-14)
-    # the device interpolation matrix equals the oracle's
+    # the device interpolation matrix equals the oracle's: entries K e^{-lnnorm}
+    # to 1e-12 in their logarithm relative to its size (a kernel value deep
+    # in the tail, K = e^{-chi2/2} with chi2 ~ 1e3, carries the rounding of
+    # chi2 times chi2/2 in its relative error)
     IM, IMo = sd.compute_IM(), od.compute_IM()
-    assert np.allclose(IM, IMo, rtol=1e-12, atol=1e-300)
-    # KKT of the device solution
+    live = IMo > 1e-290
+    assert np.array_equal(IM > 1e-290, live)
+    lg, lgo = np.log(IM[live]), np.log(IMo[live])
+    assert np.all(np.abs(lg - lgo) <= 1e-12 * np.maximum(1.0, np.abs(lgo)))
+    big = IMo > 1e-3 * IMo.max()
+    assert np.allclose(IM[big], IMo[big], rtol=1e-12, atol=0)
+    # KKT of the device solution, held to the oracle's own (the active-set
+    # loop stops when a single added index no longer lowers the residual by
+    # a relative 2^-52, not at an exact KKT point)
     fvec = np.exp(-0.5 * (m2lnp - m2lnp.min()))
     A = IMo / fvec[:, None]
-    v = w - 0.01 / n
-    Av = A @ v
-    x = v * (Av.sum() / (Av @ Av))
-    assert _kkt_rel(A, x) < 1e-8
+
+    def unshrunk(wv):
+        v = wv - 0.01 / n
+        Av = A @ v
+        return v * (Av.sum() / (Av @ Av))
+
+    kd, ko = _kkt_rel(A, unshrunk(w)), _kkt_rel(A, unshrunk(wo))
+    _say(f"   KKT residual device {kd:.2e}, oracle {ko:.2e}")
+    assert kd < max(1e-8, 10.0 * ko)
     # evaluation after the fit
     Xe = tgt.draw(g, 300)
     ref = od.eval_m2lnp_batch(Xe)
