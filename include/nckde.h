@@ -127,6 +127,16 @@ int nckde_get_cv_type (nckde *h, int *cv_type);
 /* No reference counterpart (measurement hook): objective evaluations (SPLIT:
  * IM + NNLS fits) and wall seconds of the last cross-validation fit */
 int nckde_get_cv_stats (nckde *h, unsigned *n_objective, double *seconds);
+/* No reference counterpart: the check of the mixed-precision factors made
+ * when they are built (at the first mixed eval after a prepare).  Two probe
+ * points 1.5 kernel widths from each of the 64 kernels most exposed to the
+ * rounding of the split products (narrow kernels far from the sample mean)
+ * are evaluated in both precisions; *max_rel_err is the largest
+ * |m2lnp_mixed - m2lnp_fp64| / max(|m2lnp_fp64|, 1) found, and *fallback is
+ * 1 when it exceeded 5e-7, in which case NCKDE_PREC_MIXED evals of this
+ * handle run the FP64 path until the next prepare.  Builds the factors if
+ * needed; either output may be NULL. */
+int nckde_get_mixed_check (nckde *h, double *max_rel_err, int *fallback);
 /* ncm_nnls_set_umethod / get (ncm_nnls.h:66-67) of the object's weight fit,
  * which the reference fixes to NCM_NNLS_UMETHOD_NORMAL (ncm_stats_dist.c:861);
  * DGELSD and GSL give NCKDE_EUNSUPPORTED */

@@ -132,6 +132,12 @@ struct nckde
   // mixed-precision factors (built at the first mixed eval after a prepare)
   DBuf dmu, dYc, dymax, dA16, dc32, dcoef, dYw, dpartMf, dflag;
   bool mixed_ready = false, mixed_b_ready = false;
+  // probe of the mixed factors (built with them): risk score per kernel,
+  // probe points and their results; the largest relative m2lnp error the
+  // probes found, and whether mixed evals run in FP64 for that reason
+  DBuf drisk, dprobe;
+  double mixed_probe_err = 0.0;
+  bool mixed_fallback    = false;
   double ysc = 1.0, bref = 0.0;
   // work buffers
   DBuf dXe, dout, dpartM, dpartS, du, dz, dchi2, didx;
@@ -1057,6 +1063,95 @@ max_b (nckde *h, double &bref)
   return NCKDE_OK;
 }
 
+static int eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s);
+static int eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, int linear,
+                        hipStream_t s);
+
+// Probe of freshly built mixed factors.  The split products carry an absolute
+// error in z that grows with |B_i| |y| (kernel narrowness times distance from
+// the sample mean), so narrow kernels far out can exceed the 1e-6 bar in
+// m2lnp.  The PROBE_KERNELS kernels with the largest score |B_i| (|y_i| + 1)
+// each get two points 1.5 kernel widths from their centre (the sample
+// transform with z = 1.5 (1, +-1, 1, ...)/sqrt(d)); both precisions evaluate
+// them and a relative m2lnp error above MIXED_PROBE_TOL sends the handle's
+// mixed evals to the FP64 path until the next prepare.
+constexpr int PROBE_KERNELS       = 64;
+constexpr double MIXED_PROBE_TOL  = 5.0e-7;
+
+static int
+mixed_probe (nckde *h)
+{
+  const int d = h->d;
+  const int n = (int) h->n_k;
+  const int np = std::min (PROBE_KERNELS, n);
+  const int m  = 2 * np;
+  hipStream_t s = h->stream;
+  std::vector<float> risk (n);
+
+  HIPCHK (hipMemcpyAsync (risk.data (), h->drisk.p, sizeof (float) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK (hipStreamSynchronize (s));
+
+  std::vector<uint32_t> order (n);
+
+  for (int i = 0; i < n; ++i)
+    order[i] = (uint32_t) i;
+
+  std::partial_sort (order.begin (), order.begin () + np, order.end (), [&] (uint32_t a, uint32_t b) {
+    return (risk[a] > risk[b]) || (risk[a] == risk[b] && a < b);
+  });
+
+  // device layout: x[m*d] | out64[m] | outmx[m] | z[m*d] | chi2[m] | idx[m]
+  const size_t nd = (size_t) m * d;
+  HIPCHK (h->dprobe.ensure (sizeof (double) * (2 * nd + 3 * (size_t) m) + sizeof (uint32_t) * m));
+
+  double *dx    = h->dprobe.as<double> ();
+  double *do64  = dx + nd;
+  double *domx  = do64 + m;
+  double *dz    = domx + m;
+  double *dc2   = dz + nd;
+  uint32_t *did = (uint32_t *) (dc2 + m);
+  std::vector<double> z (nd), c2 (m, (h->kernel == NCKDE_KERNEL_ST) ? h->nu : 1.0);
+  std::vector<uint32_t> idx (m);
+  const double c = 1.5 / std::sqrt ((double) d);
+
+  for (int p = 0; p < m; ++p)
+  {
+    idx[p] = order[p / 2];
+
+    for (int l = 0; l < d; ++l)
+      z[(size_t) p * d + l] = ((p & 1) && (l & 1)) ? -c : c;
+  }
+
+  HIPCHK (hipMemcpyAsync (dz, z.data (), sizeof (double) * nd, hipMemcpyHostToDevice, s));
+  HIPCHK (hipMemcpyAsync (dc2, c2.data (), sizeof (double) * m, hipMemcpyHostToDevice, s));
+  HIPCHK (hipMemcpyAsync (did, idx.data (), sizeof (uint32_t) * m, hipMemcpyHostToDevice, s));
+  HIPCHK (nck::launch_sample_at (did, dz, dc2, m, d, h->kind, h->kernel, h->nu, h->href,
+                                 (h->kind == NCKDE_VKDE) ? h->dU.as<double> () : h->dUg.as<double> (),
+                                 h->dX.as<double> (), dx, s));
+  TRY (eval_device (h, dx, m, do64, NCKDE_PREC_FP64, 0, s));
+  TRY (eval_mixed (h, dx, m, domx, 0, s));
+
+  std::vector<double> r64 (m), rmx (m);
+
+  HIPCHK (hipMemcpyAsync (r64.data (), do64, sizeof (double) * m, hipMemcpyDeviceToHost, s));
+  HIPCHK (hipMemcpyAsync (rmx.data (), domx, sizeof (double) * m, hipMemcpyDeviceToHost, s));
+  HIPCHK (hipStreamSynchronize (s));
+
+  double err = 0.0;
+
+  for (int p = 0; p < m; ++p)
+  {
+    const double e = std::fabs (rmx[p] - r64[p]) / std::max (std::fabs (r64[p]), 1.0);
+
+    err = std::isnan (e) ? INFINITY : std::max (err, e);
+  }
+
+  h->mixed_probe_err = err;
+  h->mixed_fallback  = !(err <= MIXED_PROBE_TOL);
+
+  return NCKDE_OK;
+}
+
 static int
 ensure_mixed (nckde *h)
 {
@@ -1074,14 +1169,16 @@ ensure_mixed (nckde *h)
     HIPCHK (h->dA16.ensure (sizeof (_Float16) * n_pad * 2 * DP * DP));
     HIPCHK (h->dc32.ensure (sizeof (float) * n_pad * DP));
     HIPCHK (h->dcoef.ensure (sizeof (float) * 4 * n_pad));
+    HIPCHK (h->drisk.ensure (sizeof (float) * n_pad));
     HIPCHK (nck::mixed_build (h->dX.as<double> (), (int) h->n_k, d, h->D, DP,
                               (h->kind == NCKDE_VKDE) ? h->dWp.as<double> () : nullptr,
                               h->dUg.as<double> (), h->db.as<double> (), h->bref, 1.0 / h->href, h->kernel, h->nu,
                               h->dmu.as<double> (), h->dYc.as<double> (),
                               h->dymax.as<unsigned long long> (), &h->ysc, h->dA16.as<_Float16> (),
-                              h->dc32.as<float> (), h->dcoef.as<float> (), h->stream));
+                              h->dc32.as<float> (), h->dcoef.as<float> (), h->drisk.as<float> (), h->stream));
     h->mixed_ready   = true;
     h->mixed_b_ready = true;
+    TRY (mixed_probe (h));
   }
   else if (!h->mixed_b_ready)
   {
@@ -1169,9 +1266,15 @@ eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, 
   }
 
   // the mixed factors are built (or their weights refreshed) on the handle's
-  // stream; an eval on another stream must not start before that work ends
+  // stream; an eval on another stream must not start before that work ends.
+  // Factors whose probe exceeded the error bar evaluate in FP64.
   if (precision == NCKDE_PREC_MIXED)
+  {
     TRY (ensure_mixed (h));
+
+    if (h->mixed_fallback)
+      precision = NCKDE_PREC_FP64;
+  }
 
   if (s != h->stream)
   {
@@ -1850,6 +1953,24 @@ nckde_get_cv_stats (nckde *h, unsigned *n_objective, double *seconds)
 
   if (seconds)
     *seconds = h->cv_seconds;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_get_mixed_check (nckde *h, double *max_rel_err, int *fallback)
+{
+  TRY (check_handle (h));
+  TRY (check_prepared (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  TRY (ensure_mixed (h));
+
+  if (max_rel_err)
+    *max_rel_err = h->mixed_probe_err;
+
+  if (fallback)
+    *fallback = h->mixed_fallback ? 1 : 0;
 
   return NCKDE_OK;
 }

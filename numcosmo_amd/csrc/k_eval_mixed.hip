@@ -191,7 +191,7 @@ __global__ void __launch_bounds__ (256)
 k_mixed_factors (const double *__restrict__ Wp, int D, const double *__restrict__ Yc, const double *__restrict__ Ug,
                  const double *__restrict__ b, double bref, int n, int n_pad, int d, int DP, double inv_h,
                  double ysc, int kernel, double nu, _Float16 *__restrict__ A, float *__restrict__ c,
-                 float4 *__restrict__ coef)
+                 float4 *__restrict__ coef, float *__restrict__ risk)
 {
   const long i   = blockIdx.x * 4L + (threadIdx.x >> 6);   // wave-uniform
   const int lane = threadIdx.x & 63;
@@ -243,6 +243,21 @@ k_mixed_factors (const double *__restrict__ Wp, int D, const double *__restrict_
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1)
     bmax = fmax (bmax, __shfl_xor (bmax, o, 64));
+
+  // risk score of the split products: their absolute error in z grows with
+  // |B_i| |y| while what matters near the kernel is |B_i (y - y_i)|, so
+  // narrow kernels far from the sample mean are the ones to probe
+  double yabs = 0.0;
+
+  for (int q = lane; q < d; q += 64)
+    yabs = fmax (yabs, fabs (yi[q]));
+
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1)
+    yabs = fmax (yabs, __shfl_xor (yabs, o, 64));
+
+  if (lane == 0)
+    risk[i] = (float) (bmax * (yabs + 1.0));
 
   // s_i / ysc brings max |A_i| into [2^13, 2^14)
   const int e      = (bmax > 0.0) ? ilogb (bmax / ysc) : 0;
@@ -1223,7 +1238,8 @@ launch_mixed_merge (const MixedArgs &a, double bref, int second, hipStream_t s)
 hipError_t
 mixed_build (const double *X, int n, int d, int D, int DP, const double *Wp, const double *Ug, const double *b,
              double bref, double inv_h, int kernel, double nu, double *mu, double *Yc,
-             unsigned long long *ymax_bits, double *ysc_out, _Float16 *A, float *c, float *coef, hipStream_t s)
+             unsigned long long *ymax_bits, double *ysc_out, _Float16 *A, float *c, float *coef, float *risk,
+             hipStream_t s)
 {
   hipError_t e;
 
@@ -1259,7 +1275,7 @@ mixed_build (const double *X, int n, int d, int D, int DP, const double *Wp, con
 
   *ysc_out = ysc;
   hipLaunchKernelGGL (k_mixed_factors, dim3 ((n_pad + 3) / 4), dim3 (256), 0, s, Wp, D, Yc, Ug, b, bref, n, n_pad,
-                      d, DP, inv_h, ysc, kernel, nu, A, c, (float4 *) coef);
+                      d, DP, inv_h, ysc, kernel, nu, A, c, (float4 *) coef, risk);
 
   return hipGetLastError ();
 }

@@ -162,6 +162,7 @@ hipError_t launch_mixed_merge (const MixedArgs &a, double bref, int second, hipS
 hipError_t mixed_build (const double *X, int n, int d, int D, int DP, const double *Wp, const double *Ug,
                         const double *b, double bref, double inv_h, int kernel, double nu, double *mu, double *Yc,
                         unsigned long long *ymax_bits, double *ysc_out, _Float16 *A, float *c, float *coef,
+                        float *risk,
                         hipStream_t s);
 hipError_t mixed_refresh_b (const double *b, double bref, int n, float *coef, hipStream_t s);
 

@@ -185,6 +185,13 @@ class _StatsDist:
         check(lib().nckde_get_cv_stats(self._h, C.byref(n), C.byref(t)))
         return n.value, t.value
 
+    def get_mixed_check(self):
+        """(max relative m2lnp error of the mixed-precision probes, whether
+        mixed evals fall back to FP64); no reference counterpart."""
+        e, f = C.c_double(), C.c_int()
+        check(lib().nckde_get_mixed_check(self._h, C.byref(e), C.byref(f)))
+        return e.value, bool(f.value)
+
     def set_nnls_umethod(self, umethod):
         """ncm_nnls_set_umethod (ncm_nnls.h:66) of the weight fit (NORMAL in
         the reference, ncm_stats_dist.c:861)."""
