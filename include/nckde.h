@@ -255,6 +255,26 @@ int nckde_get_knn (nckde *h, size_t *k, int32_t *idx);
  * may be NULL.  NCKDE_ESTATE when the last prepare fitted no NNLS. */
 int nckde_get_nnls_state (nckde *h, double *x, double *g, double *r);
 
+/* ---- allocation-free eval ---------------------------------------------- */
+
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): nckde_eval_m2lnp_device with every scratch buffer (point work
+ * and split partials) taken from a fixed device array of
+ * nckde_noalloc_scratch_bytes() bytes (64 MiB) that each GPU holds for the
+ * life of the process; the call allocates nothing.  The caller passes device
+ * points and output as for nckde_eval_m2lnp_device, on the handle's device.
+ * Limits: NCKDE_EINVAL when m points need more scratch than the array holds
+ * (about 6e5 points in mixed precision at d <= 16, 4e6 in FP64; evaluate in
+ * chunks); NCKDE_ESTATE for NCKDE_PREC_MIXED until the mixed factors are
+ * built (an allocating mixed eval or nckde_get_mixed_check after the
+ * prepare builds them).  All calls on one GPU share the array, whatever the
+ * handle: calls that overlap in time on one GPU race, so order them (one
+ * stream, or a synchronisation between them). */
+int nckde_eval_m2lnp_device_noalloc (nckde *h, const double *dX, size_t m, double *dout, int precision,
+                                     void *stream);
+size_t nckde_noalloc_scratch_bytes (void);
+
 /* ---- serialisation ------------------------------------------------------
  * The reference serialises an object's configuration with NcmSerialize
  * (ncm_serialize.h; exercised by test_ncm_stats_dist.c:955-957: a

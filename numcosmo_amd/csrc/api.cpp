@@ -118,6 +118,7 @@ struct nckde
   unsigned nnls_rungs[4] = {0, 0, 0, 0};
   bool wcum_ready = false;
 
+  int device              = 0;        // device current at nckde_create
   hipStream_t stream      = nullptr;
   hipStream_t last_stream = nullptr;  // stream of the last device-form eval
   hipEvent_t ready_ev     = nullptr;  // marks the handle stream's work for evals on other streams
@@ -1190,9 +1191,11 @@ ensure_mixed (nckde *h)
   return NCKDE_OK;
 }
 
-// ensure_mixed must have run (eval_device does it before ordering the streams)
+// Split count and scratch layout of the two eval forms, shared by the
+// allocating entry points (handle buffers) and the allocation-free one (the
+// device array `scratch` below).
 static int
-eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s)
+mixed_nsplit (const nckde *h, size_t m)
 {
   const int DP      = nck::mfma_dim (h->d);
   const int kpt     = (DP >= 32) ? 1 : 32 / DP;
@@ -1200,17 +1203,84 @@ eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipS
   const int bx      = (int) ((m + nck::mixed_points_per_block (DP) - 1) / nck::mixed_points_per_block (DP));
   int nsplit        = (2048 + bx - 1) / bx;
 
-  nsplit = std::max (1, std::min (nsplit, std::max (1, ngroups / 8)));
+  return std::max (1, std::min (nsplit, std::max (1, ngroups / 8)));
+}
 
-  if (h->last_stream != nullptr && h->last_stream != s)
-    HIPCHK (hipStreamSynchronize (h->last_stream));
+static int
+fp64_nsplit (const nckde *h, size_t m)
+{
+  const int nblk = (int) ((m + 255) / 256);
+  int nsplit     = (2048 + nblk - 1) / nblk;
+  const int maxs = (int) ((h->n_k + 63) / 64);
 
-  h->last_stream = s;
+  return std::max (1, std::min (nsplit, maxs));
+}
 
-  HIPCHK (h->dYw.ensure (m * (2 * sizeof (_Float16) * DP + sizeof (float) * 2)));
-  HIPCHK (h->dflag.ensure (m * (sizeof (int) + sizeof (float))));
-  HIPCHK (h->dpartMf.ensure (sizeof (float) * m * nsplit));
-  HIPCHK (h->dpartS.ensure (sizeof (double) * m * nsplit));
+static size_t
+align256 (size_t b)
+{
+  return (b + 255) & ~(size_t) 255;
+}
+
+// byte sizes of the mixed eval's work: points (hi, lo, scales), flags and
+// row maxima, split maxima, split sums
+static void
+mixed_work_bytes (const nckde *h, size_t m, size_t sz[4])
+{
+  const int DP     = nck::mfma_dim (h->d);
+  const int nsplit = mixed_nsplit (h, m);
+
+  sz[0] = m * (2 * sizeof (_Float16) * DP + sizeof (float) * 2);
+  sz[1] = m * (sizeof (int) + sizeof (float));
+  sz[2] = sizeof (float) * m * nsplit;
+  sz[3] = sizeof (double) * m * nsplit;
+}
+
+// ensure_mixed must have run (eval_device does it before ordering the streams).
+// ws != NULL: the work arrays are carved from it (laid out as
+// mixed_work_bytes, each start aligned to 256 bytes) instead of the handle's
+static int
+eval_mixed_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s, char *ws);
+
+static int
+eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s)
+{
+  return eval_mixed_ws (h, dX, m, dout, linear, s, nullptr);
+}
+
+static int
+eval_mixed_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s, char *ws)
+{
+  const int DP     = nck::mfma_dim (h->d);
+  const int nsplit = mixed_nsplit (h, m);
+  size_t sz[4];
+  void *Yw, *flag, *partMf, *partS;
+
+  mixed_work_bytes (h, m, sz);
+
+  if (ws == nullptr)
+  {
+    if (h->last_stream != nullptr && h->last_stream != s)
+      HIPCHK (hipStreamSynchronize (h->last_stream));
+
+    h->last_stream = s;
+
+    HIPCHK (h->dYw.ensure (sz[0]));
+    HIPCHK (h->dflag.ensure (sz[1]));
+    HIPCHK (h->dpartMf.ensure (sz[2]));
+    HIPCHK (h->dpartS.ensure (sz[3]));
+    Yw     = h->dYw.p;
+    flag   = h->dflag.p;
+    partMf = h->dpartMf.p;
+    partS  = h->dpartS.p;
+  }
+  else
+  {
+    Yw     = ws;
+    flag   = ws + align256 (sz[0]);
+    partMf = (char *) flag + align256 (sz[1]);
+    partS  = (char *) partMf + align256 (sz[2]);
+  }
 
   nck::MixedArgs a;
 
@@ -1231,11 +1301,11 @@ eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipS
   a.mu     = h->dmu.as<double> ();
   a.G      = h->dUg.as<double> ();
   a.ysc    = h->ysc;
-  a.Yh     = h->dYw.p;
-  a.partM  = h->dpartMf.as<float> ();
-  a.partS  = h->dpartS.as<double> ();
-  a.flag   = h->dflag.as<int> ();
-  a.M0     = (float *) (h->dflag.as<int> () + m);
+  a.Yh     = Yw;
+  a.partM  = (float *) partMf;
+  a.partS  = (double *) partS;
+  a.flag   = (int *) flag;
+  a.M0     = (float *) ((int *) flag + m);
   a.bref   = h->bref;
   a.out    = dout;
   a.linear = linear;
@@ -1252,6 +1322,11 @@ eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipS
 
   return NCKDE_OK;
 }
+
+// FP64 eval; ws != NULL: the split partials (two arrays of m x nsplit
+// doubles, the second at an offset aligned to 256 bytes) come from it
+static int
+eval_fp64_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s, char *ws);
 
 static int
 eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, int linear, hipStream_t s)
@@ -1285,21 +1360,34 @@ eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, 
   if (precision == NCKDE_PREC_MIXED)
     return eval_mixed (h, dX, m, dout, linear, s);
 
-  const int nblk = (int) ((m + 255) / 256);
-  int nsplit     = (2048 + nblk - 1) / nblk;
-  const int maxs = (int) ((h->n_k + 63) / 64);
+  return eval_fp64_ws (h, dX, m, dout, linear, s, nullptr);
+}
 
-  nsplit = std::max (1, std::min (nsplit, maxs));
+static int
+eval_fp64_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s, char *ws)
+{
+  const int nsplit = fp64_nsplit (h, m);
+  double *partM, *partS;
 
-  // the partial buffers belong to the handle: work queued on another stream
-  // must finish before this call reuses them
-  if (h->last_stream != nullptr && h->last_stream != s)
-    HIPCHK (hipStreamSynchronize (h->last_stream));
+  if (ws == nullptr)
+  {
+    // the partial buffers belong to the handle: work queued on another
+    // stream must finish before this call reuses them
+    if (h->last_stream != nullptr && h->last_stream != s)
+      HIPCHK (hipStreamSynchronize (h->last_stream));
 
-  h->last_stream = s;
+    h->last_stream = s;
 
-  HIPCHK (h->dpartM.ensure (sizeof (double) * m * nsplit));
-  HIPCHK (h->dpartS.ensure (sizeof (double) * m * nsplit));
+    HIPCHK (h->dpartM.ensure (sizeof (double) * m * nsplit));
+    HIPCHK (h->dpartS.ensure (sizeof (double) * m * nsplit));
+    partM = h->dpartM.as<double> ();
+    partS = h->dpartS.as<double> ();
+  }
+  else
+  {
+    partM = (double *) ws;
+    partS = (double *) (ws + align256 (sizeof (double) * m * nsplit));
+  }
 
   nck::EvalArgs a;
 
@@ -1319,8 +1407,8 @@ eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, 
   a.b             = h->db.as<double> ();
   a.Ug            = h->dUg.as<double> ();
   a.X             = dX;
-  a.partM         = h->dpartM.as<double> ();
-  a.partS         = h->dpartS.as<double> ();
+  a.partM         = partM;
+  a.partS         = partS;
   a.nsplit        = nsplit;
   a.out           = dout;
   a.linear        = linear;
@@ -1850,6 +1938,13 @@ nckde_create (nckde **out, int kind, int kernel, double nu, unsigned dim, int cv
   h->cv     = cv_type;
   h->nu     = (kernel == NCKDE_KERNEL_ST) ? nu : 0.0;
   h->d      = (int) dim;
+
+  if (hipGetDevice (&h->device) != hipSuccess)
+  {
+    delete h;
+    set_error ("nckde_create: hipGetDevice failed");
+    return NCKDE_EHIP;
+  }
 
   if (hipStreamCreateWithFlags (&h->stream, hipStreamNonBlocking) != hipSuccess)
   {
@@ -2513,6 +2608,142 @@ nckde_eval_m2lnp_device (nckde *h, const double *dX, size_t m, double *dout, int
   std::lock_guard<std::mutex> lk (h->mtx);
 
   return eval_device (h, dX, m, dout, precision, 0, stream ? (hipStream_t) stream : h->stream);
+}
+
+// ---- allocation-free eval (no reference counterpart) ---------------------
+// For callers that cannot allocate device memory (code under graph capture,
+// tight serving loops): the eval's scratch -- point work and split partials
+// -- lives in this file-scope device array, of which each GPU holds one copy
+// for the life of the process.  Its device address is taken once per device
+// with hipGetSymbolAddress (the array's name is only a host-side stand-in)
+// and every call on that device shares it.
+constexpr size_t SCRATCH_BYTES = (size_t) 64 << 20;
+constexpr int SCRATCH_MAX_DEVICES = 64;
+
+static __device__ __align__ (256) unsigned char scratch[SCRATCH_BYTES];
+static std::mutex scratch_mtx;
+static void *scratch_ptr[SCRATCH_MAX_DEVICES];
+
+static int
+scratch_on_device (int dev, char **out)
+{
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+  {
+    set_error ("nckde_eval_m2lnp_device_noalloc: device %d out of range", dev);
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (scratch_mtx);
+
+  if (scratch_ptr[dev] == nullptr)
+  {
+    void *p        = nullptr;
+    const hipError_t e = hipGetSymbolAddress (&p, HIP_SYMBOL (scratch));
+
+    if (e != hipSuccess || p == nullptr)
+    {
+      set_error ("nckde_eval_m2lnp_device_noalloc: hipGetSymbolAddress failed: %s", hipGetErrorString (e));
+      return NCKDE_EHIP;
+    }
+
+    scratch_ptr[dev] = p;
+  }
+
+  *out = (char *) scratch_ptr[dev];
+
+  return NCKDE_OK;
+}
+
+size_t
+nckde_noalloc_scratch_bytes (void)
+{
+  return SCRATCH_BYTES;
+}
+
+int
+nckde_eval_m2lnp_device_noalloc (nckde *h, const double *dX, size_t m, double *dout, int precision, void *stream)
+{
+  TRY (check_handle (h));
+  TRY (check_prepared (h));
+
+  if (m == 0)
+    return NCKDE_OK;
+
+  if (dX == nullptr || dout == nullptr)
+  {
+    set_error ("nckde_eval_m2lnp_device_noalloc: NULL points or output");
+    return NCKDE_EINVAL;
+  }
+
+  if (precision != NCKDE_PREC_FP64 && precision != NCKDE_PREC_MIXED)
+  {
+    set_error ("nckde: unknown precision %d", precision);
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+  hipStream_t s = stream ? (hipStream_t) stream : h->stream;
+  int dev       = -1;
+
+  HIPCHK (hipGetDevice (&dev));
+
+  if (dev != h->device)
+  {
+    set_error ("nckde_eval_m2lnp_device_noalloc: the handle lives on device %d, device %d is current", h->device,
+               dev);
+    return NCKDE_EINVAL;
+  }
+
+  if (precision == NCKDE_PREC_MIXED)
+  {
+    if (!h->mixed_ready || !h->mixed_b_ready)
+    {
+      set_error ("nckde_eval_m2lnp_device_noalloc: the mixed-precision factors are not built; an allocating "
+                 "mixed eval or nckde_get_mixed_check after the prepare builds them");
+      return NCKDE_ESTATE;
+    }
+
+    if (h->mixed_fallback)
+      precision = NCKDE_PREC_FP64;
+  }
+
+  size_t need;
+
+  if (precision == NCKDE_PREC_MIXED)
+  {
+    size_t sz[4];
+
+    mixed_work_bytes (h, m, sz);
+    need = align256 (sz[0]) + align256 (sz[1]) + align256 (sz[2]) + sz[3];
+  }
+  else
+  {
+    const size_t part = sizeof (double) * m * fp64_nsplit (h, m);
+
+    need = align256 (part) + part;
+  }
+
+  if (need > SCRATCH_BYTES)
+  {
+    set_error ("nckde_eval_m2lnp_device_noalloc: %zu points need %zu bytes of scratch, the maximum is %zu; "
+               "evaluate fewer points per call", m, need, SCRATCH_BYTES);
+    return NCKDE_EINVAL;
+  }
+
+  char *ws = nullptr;
+
+  TRY (scratch_on_device (dev, &ws));
+
+  if (s != h->stream)
+  {
+    HIPCHK (hipEventRecord (h->ready_ev, h->stream));
+    HIPCHK (hipStreamWaitEvent (s, h->ready_ev, 0));
+  }
+
+  if (precision == NCKDE_PREC_MIXED)
+    return eval_mixed_ws (h, dX, m, dout, 0, s, ws);
+
+  return eval_fp64_ws (h, dX, m, dout, 0, s, ws);
 }
 
 int
