@@ -133,10 +133,16 @@ int nckde_get_cv_stats (nckde *h, unsigned *n_objective, double *seconds);
  * rounding of the split products (narrow kernels far from the sample mean)
  * are evaluated in both precisions; *max_rel_err is the largest
  * |m2lnp_mixed - m2lnp_fp64| / max(|m2lnp_fp64|, 1) found, and *fallback is
- * 1 when it exceeded 5e-7, in which case NCKDE_PREC_MIXED evals of this
+ * 1 when it exceeded the handle's tolerance (5e-7 unless set with
+ * nckde_set_mixed_tolerance), in which case NCKDE_PREC_MIXED evals of this
  * handle run the FP64 path until the next prepare.  Builds the factors if
  * needed; either output may be NULL. */
 int nckde_get_mixed_check (nckde *h, double *max_rel_err, int *fallback);
+/* No reference counterpart: the probe error above which this handle's mixed
+ * evals run in FP64 (default 5e-7, half the 1e-6 bar).  A caller with a
+ * looser bar (the APES walker holds its transition densities to 3e-6) may
+ * raise it; an existing probe result is judged again. */
+int nckde_set_mixed_tolerance (nckde *h, double tol);
 /* ncm_nnls_set_umethod / get (ncm_nnls.h:66-67) of the object's weight fit,
  * which the reference fixes to NCM_NNLS_UMETHOD_NORMAL (ncm_stats_dist.c:861);
  * DGELSD and GSL give NCKDE_EUNSUPPORTED */
@@ -144,7 +150,7 @@ int nckde_set_nnls_umethod (nckde *h, int umethod);
 int nckde_get_nnls_umethod (nckde *h, int *umethod);
 /* No reference counterpart: at most max_steps refinement steps after a
  * Cholesky solve of the normal equations whose pivots spread by more than
- * 1e6 (squared ratio of the extreme diagonal entries of the factor):
+ * 1e10 (squared ratio of the extreme diagonal entries of the factor):
  * delta = (A_P^T A_P)^{-1} A_P^T (f - A_P x), x += delta, a step kept only if
  * the residual norm does not grow.  0 disables; the default is 2. */
 int nckde_set_nnls_refine (nckde *h, unsigned max_steps);

@@ -138,6 +138,7 @@ struct nckde
   // probes found, and whether mixed evals run in FP64 for that reason
   DBuf drisk, dprobe;
   double mixed_probe_err = 0.0;
+  double mixed_tol       = 5.0e-7;   // probe error above which mixed evals run in FP64
   bool mixed_fallback    = false;
   double ysc = 1.0, bref = 0.0;
   // work buffers
@@ -587,9 +588,10 @@ progress_enabled ()
 }
 
 // A Cholesky solve of the normal equations is refined when its pivots span
-// more than this ratio (squared extreme diagonal entries of U): the normal
-// equations then lose more than six digits.
-constexpr double REFINE_PIVOT_RATIO = 1.0e6;
+// more than this ratio (squared extreme diagonal entries of U, a lower
+// bound on the condition number of the normal matrix): the normal-equation
+// solution then carries relative errors of 1e-6 and more.
+constexpr double REFINE_PIVOT_RATIO = 1.0e10;
 
 // Corrected semi-normal refinement of x_P (in dbs) against A itself:
 // r = 1 - A_P x, delta = M_PP^{-1} A_P^T r with the factor in dS, x += delta.
@@ -1074,10 +1076,9 @@ static int eval_device (nckde *h, const double *dX, size_t m, double *dout, int 
 // m2lnp.  The PROBE_KERNELS kernels with the largest score |B_i| (|y_i| + 1)
 // each get two points 1.5 kernel widths from their centre (the sample
 // transform with z = 1.5 (1, +-1, 1, ...)/sqrt(d)); both precisions evaluate
-// them and a relative m2lnp error above MIXED_PROBE_TOL sends the handle's
-// mixed evals to the FP64 path until the next prepare.
-constexpr int PROBE_KERNELS       = 64;
-constexpr double MIXED_PROBE_TOL  = 5.0e-7;
+// them and a relative m2lnp error above the handle's tolerance (5e-7 unless
+// set) sends its mixed evals to the FP64 path until the next prepare.
+constexpr int PROBE_KERNELS = 64;
 
 static int
 mixed_probe (nckde *h)
@@ -1148,7 +1149,7 @@ mixed_probe (nckde *h)
   }
 
   h->mixed_probe_err = err;
-  h->mixed_fallback  = !(err <= MIXED_PROBE_TOL);
+  h->mixed_fallback  = !(err <= h->mixed_tol);
 
   return NCKDE_OK;
 }
@@ -2048,6 +2049,27 @@ nckde_get_cv_stats (nckde *h, unsigned *n_objective, double *seconds)
 
   if (seconds)
     *seconds = h->cv_seconds;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_mixed_tolerance (nckde *h, double tol)
+{
+  TRY (check_handle (h));
+
+  if (!(tol > 0.0))
+  {
+    set_error ("nckde_set_mixed_tolerance: the tolerance must be positive");
+    return NCKDE_EINVAL;
+  }
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+  h->mixed_tol = tol;
+
+  // a probe already made is judged again against the new tolerance
+  if (h->mixed_ready)
+    h->mixed_fallback = !(h->mixed_probe_err <= tol);
 
   return NCKDE_OK;
 }

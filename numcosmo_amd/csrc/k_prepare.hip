@@ -876,6 +876,27 @@ knn_keys_in_lds (int n, int k)
   return knn_lds_bytes (n, k) <= (size_t) KNN_LDS_BYTES;
 }
 
+// Histogram increment with the lanes of a wave that share a digit combined
+// into one LDS atomic (the leading digits of squared distances of similar
+// size are the same for most keys, so plain atomics would serialise).
+__device__ inline void
+hist_add_wave (unsigned *hist, bool active, unsigned digit)
+{
+  uint64_t pending = __ballot (active);
+
+  while (pending)
+  {
+    const int leader   = __ffsll ((long long) pending) - 1;
+    const unsigned dl  = __shfl ((int) digit, leader, 64);
+    const uint64_t grp = __ballot (active && digit == dl) & pending;
+
+    if ((int) (threadIdx.x & 63) == leader)
+      atomicAdd (&hist[dl], (unsigned) __popcll (grp));
+
+    pending &= ~grp;
+  }
+}
+
 __global__ void __launch_bounds__ (KNN_THREADS)
 k_knn_lds (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, int npow, int *__restrict__ out,
            int *__restrict__ status)
@@ -886,6 +907,7 @@ k_knn_lds (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, in
   int *sidx      = (int *) (skey + npow);
   __shared__ unsigned hist[KNN_THREADS / 64][256];
   __shared__ uint64_t s_prefix, s_wmin[KNN_THREADS / 64], s_wmax[KNN_THREADS / 64];
+  __shared__ unsigned s_wzero[KNN_THREADS / 64];
   __shared__ int s_kk, s_nlt, s_eq_taken, s_pass0;
   __shared__ int s_scan[KNN_THREADS];
   __shared__ unsigned s_wsum[KNN_THREADS / 64];
@@ -901,8 +923,11 @@ k_knn_lds (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, in
   const int q      = q0 + qb;
   const double *yq = Y + (size_t) q * d;
 
-  // 1. distances (coordinates summed in order, as the global-memory kernel)
+  // 1. distances (coordinates summed in order, as the global-memory kernel);
+  //    the extremes of the positive keys and the number of zero keys (the
+  //    query itself and its duplicates)
   uint64_t kmin = ~(uint64_t) 0, kmax = 0;
+  unsigned nzero = 0;
 
   for (int j = tid; j < n; j += KNN_THREADS)
   {
@@ -919,8 +944,14 @@ k_knn_lds (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, in
     const uint64_t key = (uint64_t) __double_as_longlong (dist);
 
     keys[j] = key;
-    kmin    = (key < kmin) ? key : kmin;
-    kmax    = (key > kmax) ? key : kmax;
+
+    if (key == 0)
+      nzero++;
+    else
+    {
+      kmin = (key < kmin) ? key : kmin;
+      kmax = (key > kmax) ? key : kmax;
+    }
   }
 
   for (int o = 32; o > 0; o >>= 1)
@@ -930,12 +961,14 @@ k_knn_lds (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, in
 
     kmin = (a < kmin) ? a : kmin;
     kmax = (b > kmax) ? b : kmax;
+    nzero += __shfl_xor (nzero, o, 64);
   }
 
   if (lane == 0)
   {
-    s_wmin[wave] = kmin;
-    s_wmax[wave] = kmax;
+    s_wmin[wave]  = kmin;
+    s_wmax[wave]  = kmax;
+    s_wzero[wave] = nzero;
   }
 
   __syncthreads ();
@@ -943,25 +976,38 @@ k_knn_lds (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, in
   if (tid == 0)
   {
     uint64_t mn = s_wmin[0], mx = s_wmax[0];
+    unsigned nz = s_wzero[0];
 
     for (int w = 1; w < KNN_THREADS / 64; ++w)
     {
       mn = (s_wmin[w] < mn) ? s_wmin[w] : mn;
       mx = (s_wmax[w] > mx) ? s_wmax[w] : mx;
+      nz += s_wzero[w];
     }
 
-    // whole leading bytes shared by every key
-    const uint64_t x = mn ^ mx;
-    const int lead   = (x == 0) ? 8 : __clzll ((long long) x) / 8;
+    if ((int) nz >= k || mx == 0)
+    {
+      // the k smallest keys are zeros: T = 0, the first k zero keys by index
+      s_pass0  = 8;
+      s_prefix = 0;
+      s_kk     = k;
+    }
+    else
+    {
+      // rank k - nz among the positive keys, which share their leading bytes
+      // with the smallest and largest of them
+      const uint64_t x = mn ^ mx;
+      const int lead   = (x == 0) ? 8 : __clzll ((long long) x) / 8;
 
-    s_pass0  = lead;
-    s_prefix = (lead == 0) ? 0 : (lead == 8 ? mn : (mn & (~(uint64_t) 0 << (64 - 8 * lead))));
-    s_kk     = k;
+      s_pass0  = lead;
+      s_prefix = (lead == 0) ? 0 : (lead == 8 ? mn : (mn & (~(uint64_t) 0 << (64 - 8 * lead))));
+      s_kk     = k - (int) nz;
+    }
   }
 
   __syncthreads ();
 
-  // 2. radix select of the k-th smallest key over the remaining bytes
+  // 2. radix select over the remaining bytes, positive keys only
   const int pass0 = s_pass0;
   uint64_t mask   = (pass0 == 0) ? 0 : (pass0 == 8 ? ~(uint64_t) 0 : (~(uint64_t) 0 << (64 - 8 * pass0)));
 
@@ -976,12 +1022,15 @@ k_knn_lds (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, in
 
     const uint64_t prefix = s_prefix;
 
-    for (int j = tid; j < n; j += KNN_THREADS)
+    // every lane of the wave takes part in each round (the combining needs
+    // whole-wave ballots)
+    for (int j0 = wave * 64; j0 < n; j0 += KNN_THREADS)
     {
-      const uint64_t key = keys[j];
+      const int j         = j0 + lane;
+      const uint64_t key  = (j < n) ? keys[j] : 0;
+      const bool active   = (j < n) && key != 0 && (key & mask) == prefix;
 
-      if ((key & mask) == prefix)
-        atomicAdd (&hist[wave][(key >> shift) & 255u], 1u);
+      hist_add_wave (hist[wave], active, (unsigned) ((key >> shift) & 255u));
     }
 
     __syncthreads ();

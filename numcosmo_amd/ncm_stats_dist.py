@@ -192,6 +192,11 @@ class _StatsDist:
         check(lib().nckde_get_mixed_check(self._h, C.byref(e), C.byref(f)))
         return e.value, bool(f.value)
 
+    def set_mixed_tolerance(self, tol):
+        """Probe error above which mixed evals run in FP64 (default 5e-7);
+        no reference counterpart."""
+        check(lib().nckde_set_mixed_tolerance(self._h, float(tol)))
+
     def set_nnls_umethod(self, umethod):
         """ncm_nnls_set_umethod (ncm_nnls.h:66) of the weight fit (NORMAL in
         the reference, ncm_stats_dist.c:861)."""
