@@ -306,11 +306,22 @@ def serve_case(n_req, rate, n_sessions, slots, seed=None):
            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
            "requests_per_s": n_req / (t_last - t_first), "eval_pairs_per_s": pairs / (t_last - t_first),
            "submit_lateness_p99_ms": float(np.percentile(lateness, 99) * 1e3), "setup_s": setup_s}
+    wait = np.array([i["t_start"] - i["t_submit"] for i in infos]) * 1e3
+    run = np.array([i["t_end"] - i["t_start"] for i in infos]) * 1e3
+    sizes = np.array([stream.sessions[int(stream.session[r])]["n"] for r in range(n_req)])
+    dims = np.array([stream.sessions[int(stream.session[r])]["d"] for r in range(n_req)])
     for op in stream.OPS:
         sel = ops == op
         if sel.any():
             res[op] = {"count": int(sel.sum()), "p50_ms": float(np.percentile(lat[sel], 50)),
-                       "p99_ms": float(np.percentile(lat[sel], 99))}
+                       "p99_ms": float(np.percentile(lat[sel], 99)),
+                       "queue_p99_ms": float(np.percentile(wait[sel], 99)),
+                       "run_p50_ms": float(np.percentile(run[sel], 50)),
+                       "run_p99_ms": float(np.percentile(run[sel], 99))}
+    # the slowest requests: what they were and where their time went
+    worst = np.argsort(lat)[-8:][::-1]
+    res["slowest"] = [{"op": str(ops[r]), "n": int(sizes[r]), "d": int(dims[r]), "latency_ms": float(lat[r]),
+                       "queue_ms": float(wait[r]), "run_ms": float(run[r])} for r in worst]
     log(f"[serve] p50 {res['p50_ms']:.2f} ms p99 {res['p99_ms']:.2f} ms, {res['requests_per_s']:.1f} req/s, "
         f"per op {[(op, res[op]) for op in stream.OPS if op in res]}")
     return res
@@ -363,6 +374,7 @@ def c5_case(m, m_fp64, check_pts, oracle_mod):
     log(f"[C5] generating d={d} n={n} m={m}")
     tgt, X, m2lnp, Xe = synth.workload("cond", n, m, d, 5)
     sd = na.StatsDistVKDE.new(na.StatsDistKernelST.new(d, 5.0), na.StatsDistCV.NONE)
+    sd.set_nnls_refine(2)       # the workload fits its NNLS weights with FP64 refinement
     sd.add_obs_array(X)
     t0 = time.perf_counter()
     sd.prepare_interp(m2lnp)
@@ -374,8 +386,11 @@ def c5_case(m, m_fp64, check_pts, oracle_mod):
     stream = torch.cuda.Stream()
     nk = sd.get_n_kernels()
     ms_m, _, k_m = timed_eval(sd, Xd, outm, na.Precision.MIXED, 3, 1, stream)
-    ms_f, _, k_f = timed_eval(sd, Xd[:m_fp64].contiguous(), out64, na.Precision.FP64, 1, 1, stream)
+    X64 = Xd[:m_fp64].contiguous()
+    ms_f, _, k_f = timed_eval(sd, X64, out64, na.Precision.FP64, 1, 1, stream)
     got_m, got_f = outm.cpu().numpy(), out64.cpu().numpy()
+    ms_32, _, k_32 = timed_eval(sd, X64, out64, na.Precision.FP32, 1, 1, stream)
+    got_32 = out64.cpu().numpy()
     emax, e99 = rel_err(got_m[:m_fp64], got_f)
 
     # the oracle holding the device's factors and weights (lnnorms recomputed)
@@ -388,16 +403,20 @@ def c5_case(m, m_fp64, check_pts, oracle_mod):
     t_or = time.perf_counter() - t0
     om_max, om_99 = rel_err(got_m[:check_pts], ref)
     of_max, of_99 = rel_err(got_f[:check_pts], ref)
+    o32_max, o32_99 = rel_err(got_32[:check_pts], ref)
     res = {"d": d, "n": n, "m": m, "prepare_interp_s": prep, "nnls_rungs": list(sd.get_nnls_rungs()),
            "mixed": {"ms_per_step": ms_m, "pairs_per_s": nk * m / (ms_m * 1e-3), "kernel_ms": k_m,
                      "rel_err_vs_oracle_max": om_max, "rel_err_vs_oracle_p99": om_99},
+           "fp32": {"m": m_fp64, "ms_per_step": ms_32, "pairs_per_s": nk * m_fp64 / (ms_32 * 1e-3),
+                    "kernel_ms": k_32, "rel_err_vs_oracle_max": o32_max, "rel_err_vs_oracle_p99": o32_99},
            "fp64": {"m": m_fp64, "ms_per_step": ms_f, "pairs_per_s": nk * m_fp64 / (ms_f * 1e-3), "kernel_ms": k_f,
                     "rel_err_vs_oracle_max": of_max, "rel_err_vs_oracle_p99": of_99},
            "oracle_checked_points": check_pts, "oracle_s": t_or,
            "mixed_vs_fp64_rel_err_max": emax, "mixed_vs_fp64_rel_err_p99": e99}
-    log(f"[C5] mixed {ms_m:.1f} ms ({res['mixed']['pairs_per_s']:.3e} pairs/s), fp64 {ms_f:.1f} ms for "
-        f"{m_fp64} points ({res['fp64']['pairs_per_s']:.3e} pairs/s); vs oracle on {check_pts} points "
-        f"({t_or:.0f} s): mixed {om_max:.2e}, fp64 {of_max:.2e}; mixed vs fp64 {emax:.2e}")
+    log(f"[C5] mixed {ms_m:.1f} ms ({res['mixed']['pairs_per_s']:.3e} pairs/s), fp32 {ms_32:.1f} ms and fp64 "
+        f"{ms_f:.1f} ms for {m_fp64} points ({res['fp32']['pairs_per_s']:.3e}, {res['fp64']['pairs_per_s']:.3e} "
+        f"pairs/s); vs oracle on {check_pts} points ({t_or:.0f} s): mixed {om_max:.2e}, fp32 {o32_max:.2e}, "
+        f"fp64 {of_max:.2e}; mixed vs fp64 {emax:.2e}")
     return res
 
 

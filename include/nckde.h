@@ -44,8 +44,10 @@ enum { NCKDE_CV_NONE = 0, NCKDE_CV_SPLIT = 1, NCKDE_CV_SPLIT_NOFIT = 2, NCKDE_CV
 /* NcmStatsDistKDECovType (ncm_stats_dist_kde.h:64-72): sample, fixed, robust diagonal (Qn scales) and
  * robust OGK covariances (ncm_stats_vec.c:1834-2064) */
 enum { NCKDE_COV_SAMPLE = 0, NCKDE_COV_FIXED = 1, NCKDE_COV_ROBUST_DIAG = 2, NCKDE_COV_ROBUST = 3 };
-/* arithmetic of the pairwise distance and kernel sums in eval */
-enum { NCKDE_PREC_FP64 = 0, NCKDE_PREC_MIXED = 1 };
+/* arithmetic of the pairwise distance and kernel sums in eval: FP64; MIXED
+ * (FP16 hi/lo split products on the matrix cores, FP32 sums); FP32 (the
+ * FP32 rung of the precision ladder: single-precision distances and sums) */
+enum { NCKDE_PREC_FP64 = 0, NCKDE_PREC_MIXED = 1, NCKDE_PREC_FP32 = 2 };
 /* NcmNNLSUMethod (ncm_nnls.h:51-60): the unconstrained solve of the active-set
  * NNLS.  NORMAL: normal equations by Cholesky, else the symmetric solve, else
  * QR; NORMAL_LU: symmetric solve, else QR; QR: least squares on A_P. */
@@ -138,6 +140,21 @@ int nckde_get_cv_stats (nckde *h, unsigned *n_objective, double *seconds);
  * handle run the FP64 path until the next prepare.  Builds the factors if
  * needed; either output may be NULL. */
 int nckde_get_mixed_check (nckde *h, double *max_rel_err, int *fallback);
+/* No reference counterpart (tuning hook): force the number of kernel-range
+ * splits of this handle's evals (each split is one grid row of blocks whose
+ * partial sums are merged in a fixed order); 0 restores the choice from the
+ * tuned table (numcosmo_amd/tune.py), else the default rule.  Results depend
+ * on the split count only in the last bits. */
+int nckde_set_eval_split (nckde *h, unsigned nsplit);
+/* No reference counterpart: replay of captured evals.  With on != 0, an eval
+ * repeated with the same shape (points, precision, linear/log form), the
+ * same device buffers and the same non-null stream is captured into a HIP
+ * graph at its second call and replayed as one graph launch afterwards
+ * (host-buffer evals reuse the handle's point and output buffers, so
+ * repeated evals of m points qualify).  A prepare, a change of the weights
+ * or of the bandwidth, or a factor rebuild invalidates the graphs.  Results
+ * are bit-identical to the eager launches.  Off by default. */
+int nckde_set_graphs (nckde *h, int on);
 /* No reference counterpart: the probe error above which this handle's mixed
  * evals run in FP64 (default 5e-7, half the 1e-6 bar).  A caller with a
  * looser bar (the APES walker holds its transition densities to 3e-6) may
@@ -152,7 +169,8 @@ int nckde_get_nnls_umethod (nckde *h, int *umethod);
  * Cholesky solve of the normal equations whose pivots spread by more than
  * 1e10 (squared ratio of the extreme diagonal entries of the factor):
  * delta = (A_P^T A_P)^{-1} A_P^T (f - A_P x), x += delta, a step kept only if
- * the residual norm does not grow.  0 disables; the default is 2. */
+ * the residual norm does not grow.  The default is 0 (no refinement, as the
+ * reference's solver); at most 16. */
 int nckde_set_nnls_refine (nckde *h, unsigned max_steps);
 int nckde_get_nnls_refine (nckde *h, unsigned *max_steps);
 /* No reference counterpart (diagnostic): unconstrained solves of the last

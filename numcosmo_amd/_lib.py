@@ -51,6 +51,8 @@ SIGNATURES = {
     "nckde_get_cv_stats": (C.c_int, [_h, C.POINTER(C.c_uint), C.POINTER(C.c_double)]),
     "nckde_get_mixed_check": (C.c_int, [_h, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "nckde_set_mixed_tolerance": (C.c_int, [_h, C.c_double]),
+    "nckde_set_graphs": (C.c_int, [_h, C.c_int]),
+    "nckde_set_eval_split": (C.c_int, [_h, C.c_uint]),
     "nckde_set_nnls_umethod": (C.c_int, [_h, C.c_int]),
     "nckde_get_nnls_umethod": (C.c_int, [_h, C.POINTER(C.c_int)]),
     "nckde_set_nnls_refine": (C.c_int, [_h, C.c_uint]),

@@ -32,7 +32,7 @@ COMMON = [
     "-I", os.path.join(HERE, "..", "include"),
 ]
 
-SOURCES = ["api.cpp", "api_apes.cpp", "api_serve.cpp", "cv_min.cpp", "k_prepare.hip", "k_eval.hip", "k_eval_mixed.hip", "k_linalg.hip", "k_nnls.hip", "k_sample.hip", "k_robust.hip"]
+SOURCES = ["api.cpp", "api_apes.cpp", "api_serve.cpp", "cv_min.cpp", "k_prepare.hip", "k_eval.hip", "k_eval_mixed.hip", "k_linalg.hip", "k_nnls.hip", "k_probe.hip", "k_sample.hip", "k_robust.hip"]
 
 
 def _compile(src: str) -> str:

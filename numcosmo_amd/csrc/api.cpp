@@ -114,7 +114,7 @@ struct nckde
   // solves of the last prepare_interp per rung: Cholesky, symmetric solve,
   // QR, refinement steps
   int nnls_umethod      = NCKDE_NNLS_NORMAL;
-  unsigned nnls_refine  = 2;
+  unsigned nnls_refine  = 0;
   unsigned nnls_rungs[4] = {0, 0, 0, 0};
   bool wcum_ready = false;
 
@@ -136,6 +136,29 @@ struct nckde
   // probe of the mixed factors (built with them): risk score per kernel,
   // probe points and their results; the largest relative m2lnp error the
   // probes found, and whether mixed evals run in FP64 for that reason
+  // captured eval launch sequences: an eval of the same shape on the same
+  // buffers and stream is replayed as one graph launch from its second
+  // repetition on; graph_gen counts the changes (prepare, weights, factor
+  // builds, bandwidth) that invalidate every captured graph
+  struct EvalGraph
+  {
+    const double *dX;
+    double *dout;
+    size_t m;
+    int precision, linear;
+    hipStream_t s;
+    unsigned gen;
+    int seen;
+    hipGraphExec_t exec;
+  };
+  unsigned split_override = 0;   // tuning hook: forced split count of evals
+  bool graphs        = false;
+  unsigned graph_gen = 0;
+  std::vector<EvalGraph> graph_cache;
+  // FP32 rung: FP32 copies of the packed factors and of b_i - max b
+  DBuf dWf, db2f;
+  bool f32_ready = false, f32_b_ready = false;
+  double bref32  = 0.0;
   DBuf drisk, dprobe;
   double mixed_probe_err = 0.0;
   double mixed_tol       = 5.0e-7;   // probe error above which mixed evals run in FP64
@@ -388,11 +411,15 @@ vkde_build_cov_array (nckde *h)
   if (!in_lds)
     HIPCHK (h->dkeys.ensure (sizeof (uint64_t) * h->n_obs * nq));
 
+  // the whitened sample transposed for unit-stride distance reads
+  HIPCHK (h->dscratch.ensure (sizeof (double) * h->n_obs * d));
+  HIPCHK (nck::launch_transpose (h->dY.as<double> (), (int) h->n_obs, d, h->dscratch.as<double> (), s));
+
   for (size_t q0 = 0; q0 < h->n_k; q0 += nq)
   {
     const size_t cnt = std::min (nq, h->n_k - q0);
 
-    HIPCHK (nck::launch_knn (h->dY.as<double> (), (int) h->n_obs, d, (int) q0, (int) cnt, (int) h->k,
+    HIPCHK (nck::launch_knn (h->dscratch.as<double> (), (int) h->n_obs, d, (int) q0, (int) cnt, (int) h->k,
                              h->dkeys.as<uint64_t> (), h->dknn.as<int> (), h->dstatus.as<int> (), s));
   }
 
@@ -429,6 +456,8 @@ refresh_bvec (nckde *h)
                             (int) h->n_k, h->db.as<double> (), h->stream));
   h->wcum_ready    = false;
   h->mixed_b_ready = false;
+  h->f32_b_ready   = false;
+  h->graph_gen++;
 
   return NCKDE_OK;
 }
@@ -521,6 +550,8 @@ do_prepare (nckde *h)
   h->rnorm       = 0.0;
   h->prepared    = true;
   h->mixed_ready = false;
+  h->f32_ready   = false;
+  h->graph_gen++;
 
   // bandwidth by minimisation for CV_SPLIT_NOFIT and CV_LOO
   // (ncm_stats_dist.c:769-788)
@@ -1083,70 +1114,38 @@ constexpr int PROBE_KERNELS = 64;
 static int
 mixed_probe (nckde *h)
 {
-  const int d = h->d;
-  const int n = (int) h->n_k;
-  const int np = std::min (PROBE_KERNELS, n);
-  const int m  = 2 * np;
+  const int d   = h->d;
+  const int n   = (int) h->n_k;
+  const int np  = std::min (PROBE_KERNELS, n);
+  const int m   = 2 * np;
   hipStream_t s = h->stream;
-  std::vector<float> risk (n);
 
-  HIPCHK (hipMemcpyAsync (risk.data (), h->drisk.p, sizeof (float) * n, hipMemcpyDeviceToHost, s));
-  HIPCHK (hipStreamSynchronize (s));
-
-  std::vector<uint32_t> order (n);
-
-  for (int i = 0; i < n; ++i)
-    order[i] = (uint32_t) i;
-
-  std::partial_sort (order.begin (), order.begin () + np, order.end (), [&] (uint32_t a, uint32_t b) {
-    return (risk[a] > risk[b]) || (risk[a] == risk[b] && a < b);
-  });
-
-  // device layout: x[m*d] | out64[m] | outmx[m] | z[m*d] | chi2[m] | idx[m]
+  // device layout: x[m*d] | out64[m] | outmx[m] | z[m*d] | chi2[m] | err | sel[np] | idx[m]
   const size_t nd = (size_t) m * d;
-  HIPCHK (h->dprobe.ensure (sizeof (double) * (2 * nd + 3 * (size_t) m) + sizeof (uint32_t) * m));
+  HIPCHK (h->dprobe.ensure (sizeof (double) * (2 * nd + 3 * (size_t) m + 1) + sizeof (uint32_t) * (np + m)));
 
   double *dx    = h->dprobe.as<double> ();
   double *do64  = dx + nd;
   double *domx  = do64 + m;
   double *dz    = domx + m;
   double *dc2   = dz + nd;
-  uint32_t *did = (uint32_t *) (dc2 + m);
-  std::vector<double> z (nd), c2 (m, (h->kernel == NCKDE_KERNEL_ST) ? h->nu : 1.0);
-  std::vector<uint32_t> idx (m);
-  const double c = 1.5 / std::sqrt ((double) d);
+  double *derr  = dc2 + m;
+  uint32_t *sel = (uint32_t *) (derr + 1);
+  uint32_t *did = sel + np;
 
-  for (int p = 0; p < m; ++p)
-  {
-    idx[p] = order[p / 2];
-
-    for (int l = 0; l < d; ++l)
-      z[(size_t) p * d + l] = ((p & 1) && (l & 1)) ? -c : c;
-  }
-
-  HIPCHK (hipMemcpyAsync (dz, z.data (), sizeof (double) * nd, hipMemcpyHostToDevice, s));
-  HIPCHK (hipMemcpyAsync (dc2, c2.data (), sizeof (double) * m, hipMemcpyHostToDevice, s));
-  HIPCHK (hipMemcpyAsync (did, idx.data (), sizeof (uint32_t) * m, hipMemcpyHostToDevice, s));
+  HIPCHK (nck::launch_probe_points (h->drisk.as<float> (), n, np, d, 1.5 / std::sqrt ((double) d),
+                                    (h->kernel == NCKDE_KERNEL_ST) ? h->nu : 1.0, sel, did, dz, dc2, s));
   HIPCHK (nck::launch_sample_at (did, dz, dc2, m, d, h->kind, h->kernel, h->nu, h->href,
                                  (h->kind == NCKDE_VKDE) ? h->dU.as<double> () : h->dUg.as<double> (),
                                  h->dX.as<double> (), dx, s));
   TRY (eval_device (h, dx, m, do64, NCKDE_PREC_FP64, 0, s));
   TRY (eval_mixed (h, dx, m, domx, 0, s));
-
-  std::vector<double> r64 (m), rmx (m);
-
-  HIPCHK (hipMemcpyAsync (r64.data (), do64, sizeof (double) * m, hipMemcpyDeviceToHost, s));
-  HIPCHK (hipMemcpyAsync (rmx.data (), domx, sizeof (double) * m, hipMemcpyDeviceToHost, s));
-  HIPCHK (hipStreamSynchronize (s));
+  HIPCHK (nck::launch_probe_err (domx, do64, m, derr, s));
 
   double err = 0.0;
 
-  for (int p = 0; p < m; ++p)
-  {
-    const double e = std::fabs (rmx[p] - r64[p]) / std::max (std::fabs (r64[p]), 1.0);
-
-    err = std::isnan (e) ? INFINITY : std::max (err, e);
-  }
+  HIPCHK (hipMemcpyAsync (&err, derr, sizeof (double), hipMemcpyDeviceToHost, s));
+  HIPCHK (hipStreamSynchronize (s));
 
   h->mixed_probe_err = err;
   h->mixed_fallback  = !(err <= h->mixed_tol);
@@ -1180,13 +1179,47 @@ ensure_mixed (nckde *h)
                               h->dc32.as<float> (), h->dcoef.as<float> (), h->drisk.as<float> (), h->stream));
     h->mixed_ready   = true;
     h->mixed_b_ready = true;
+    h->graph_gen++;
     TRY (mixed_probe (h));
   }
   else if (!h->mixed_b_ready)
   {
     TRY (max_b (h, h->bref));
     HIPCHK (nck::mixed_refresh_b (h->db.as<double> (), h->bref, (int) h->n_k, h->dcoef.as<float> (), h->stream));
+    h->graph_gen++;
     h->mixed_b_ready = true;
+  }
+
+  return NCKDE_OK;
+}
+
+// FP32 copies of the factors (at the first FP32 eval after a prepare) and of
+// b_i - max b (after every change of the weights)
+static int
+ensure_f32 (nckde *h)
+{
+  if (!h->f32_ready)
+  {
+    if (h->kind == NCKDE_VKDE)
+    {
+      const size_t cnt = h->n_k * (size_t) (h->D * (h->D + 1) / 2);
+
+      HIPCHK (h->dWf.ensure (sizeof (float) * cnt));
+      HIPCHK (nck::launch_f32_factors (h->dWp.as<double> (), cnt, h->dWf.as<float> (), h->stream));
+    }
+
+    h->f32_ready   = true;
+    h->f32_b_ready = false;
+    h->graph_gen++;
+  }
+
+  if (!h->f32_b_ready)
+  {
+    TRY (max_b (h, h->bref32));
+    HIPCHK (h->db2f.ensure (sizeof (float) * h->n_k));
+    HIPCHK (nck::launch_f32_b2 (h->db.as<double> (), h->bref32, (int) h->n_k, h->db2f.as<float> (), h->stream));
+    h->f32_b_ready = true;
+    h->graph_gen++;
   }
 
   return NCKDE_OK;
@@ -1195,21 +1228,74 @@ ensure_mixed (nckde *h)
 // Split count and scratch layout of the two eval forms, shared by the
 // allocating entry points (handle buffers) and the allocation-free one (the
 // device array `scratch` below).
+// Kernel-range split counts chosen by the project's tuner (numcosmo_amd/tune.py
+// times every candidate on the device and writes eval_tune.inc): precision
+// (0 FP64, 1 mixed, 2 FP32), dimension (the MFMA dimension DP for mixed,
+// the padded dimension D otherwise), log2 n and log2 m of the tuned case,
+// split count.  An eval uses the entry of its precision and dimension
+// nearest in (log2 n, log2 m) within one octave in each, else the default
+// rule (about 2048 blocks in all, each split at least 8 kernel groups
+// (mixed) or 64 kernels (FP64)).
+struct SplitTune
+{
+  int prec, dim, lgn, lgm, nsplit;
+};
+
+static const SplitTune split_tune[] = {
+#include "eval_tune.inc"
+  {-1, 0, 0, 0, 0}
+};
+
+static int
+tuned_nsplit (int prec, int dim, size_t n, size_t m)
+{
+  const double ln = std::log2 ((double) std::max<size_t> (n, 1));
+  const double lm = std::log2 ((double) std::max<size_t> (m, 1));
+  double best     = 1e30;
+  int ns          = 0;
+
+  for (const SplitTune &e : split_tune)
+  {
+    if (e.prec != prec || e.dim != dim)
+      continue;
+
+    const double dn = std::fabs (ln - e.lgn), dm = std::fabs (lm - e.lgm);
+
+    if (dn <= 1.0 && dm <= 1.0 && dn + dm < best)
+    {
+      best = dn + dm;
+      ns   = e.nsplit;
+    }
+  }
+
+  return ns;
+}
+
 static int
 mixed_nsplit (const nckde *h, size_t m)
 {
   const int DP      = nck::mfma_dim (h->d);
   const int kpt     = (DP >= 32) ? 1 : 32 / DP;
   const int ngroups = (int) ((h->n_k + kpt - 1) / kpt);
-  const int bx      = (int) ((m + nck::mixed_points_per_block (DP) - 1) / nck::mixed_points_per_block (DP));
-  int nsplit        = (2048 + bx - 1) / bx;
+  int chosen        = (h->split_override > 0) ? (int) h->split_override : tuned_nsplit (1, DP, h->n_k, m);
+
+  if (chosen > 0)
+    return std::max (1, std::min (chosen, ngroups));
+
+  const int bx = (int) ((m + nck::mixed_points_per_block (DP) - 1) / nck::mixed_points_per_block (DP));
+  int nsplit   = (2048 + bx - 1) / bx;
 
   return std::max (1, std::min (nsplit, std::max (1, ngroups / 8)));
 }
 
 static int
-fp64_nsplit (const nckde *h, size_t m)
+fp64_nsplit (const nckde *h, size_t m, int prec)
 {
+  int chosen = (h->split_override > 0) ? (int) h->split_override : tuned_nsplit (prec, h->D, h->n_k, m);
+
+  if (chosen > 0)
+    return std::max (1, std::min (chosen, (int) h->n_k));
+
   const int nblk = (int) ((m + 255) / 256);
   int nsplit     = (2048 + nblk - 1) / nblk;
   const int maxs = (int) ((h->n_k + 63) / 64);
@@ -1324,10 +1410,16 @@ eval_mixed_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, h
   return NCKDE_OK;
 }
 
-// FP64 eval; ws != NULL: the split partials (two arrays of m x nsplit
-// doubles, the second at an offset aligned to 256 bytes) come from it
+// FP64 eval (FP32 rung when f32 is set; ensure_f32 must have run); ws !=
+// NULL: the split partials (two arrays of m x nsplit doubles, the second at
+// an offset aligned to 256 bytes) come from it
 static int
-eval_fp64_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s, char *ws);
+eval_fp64_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s, char *ws,
+              bool f32 = false);
+
+template <class Run>
+static int eval_graph (nckde *h, const double *dX, size_t m, double *dout, int precision, int linear, hipStream_t s,
+                       const Run &run);
 
 static int
 eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, int linear, hipStream_t s)
@@ -1335,21 +1427,26 @@ eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, 
   if (m == 0)
     return NCKDE_OK;
 
-  if (precision != NCKDE_PREC_FP64 && precision != NCKDE_PREC_MIXED)
+  if (precision != NCKDE_PREC_FP64 && precision != NCKDE_PREC_MIXED && precision != NCKDE_PREC_FP32)
   {
     set_error ("nckde: unknown precision %d", precision);
     return NCKDE_EINVAL;
   }
 
-  // the mixed factors are built (or their weights refreshed) on the handle's
-  // stream; an eval on another stream must not start before that work ends.
-  // Factors whose probe exceeded the error bar evaluate in FP64.
+  // the mixed and FP32 factors are built (or their weights refreshed) on the
+  // handle's stream; an eval on another stream must not start before that
+  // work ends.  Mixed factors whose probe exceeded the error bar evaluate in
+  // FP64.
   if (precision == NCKDE_PREC_MIXED)
   {
     TRY (ensure_mixed (h));
 
     if (h->mixed_fallback)
       precision = NCKDE_PREC_FP64;
+  }
+  else if (precision == NCKDE_PREC_FP32)
+  {
+    TRY (ensure_f32 (h));
   }
 
   if (s != h->stream)
@@ -1358,16 +1455,110 @@ eval_device (nckde *h, const double *dX, size_t m, double *dout, int precision, 
     HIPCHK (hipStreamWaitEvent (s, h->ready_ev, 0));
   }
 
-  if (precision == NCKDE_PREC_MIXED)
-    return eval_mixed (h, dX, m, dout, linear, s);
+  auto run = [&] () -> int {
+    if (precision == NCKDE_PREC_MIXED)
+      return eval_mixed (h, dX, m, dout, linear, s);
 
-  return eval_fp64_ws (h, dX, m, dout, linear, s, nullptr);
+    return eval_fp64_ws (h, dX, m, dout, linear, s, nullptr, precision == NCKDE_PREC_FP32);
+  };
+
+  if (!h->graphs || h->profiling || s == nullptr)
+    return run ();
+
+  return eval_graph (h, dX, m, dout, precision, linear, s, run);
+}
+
+// Replay of a captured eval: the first eval of a shape runs eagerly (it
+// sizes every buffer and settles the stream bookkeeping), the second is
+// captured into a graph (kernel launches only: the eval path has no memset
+// or allocation once its buffers exist) and launched, later ones launch the
+// graph.  Graphs of an older generation are destroyed.
+template <class Run>
+static int
+eval_graph (nckde *h, const double *dX, size_t m, double *dout, int precision, int linear, hipStream_t s,
+            const Run &run)
+{
+  auto &cache = h->graph_cache;
+
+  for (size_t i = 0; i < cache.size ();)
+  {
+    if (cache[i].gen != h->graph_gen)
+    {
+      if (cache[i].exec)
+        (void) hipGraphExecDestroy (cache[i].exec);
+
+      cache.erase (cache.begin () + i);
+    }
+    else
+    {
+      ++i;
+    }
+  }
+
+  nckde::EvalGraph *g = nullptr;
+
+  for (auto &e : cache)
+    if (e.dX == dX && e.dout == dout && e.m == m && e.precision == precision && e.linear == linear && e.s == s)
+      g = &e;
+
+  if (g != nullptr && g->exec != nullptr)
+  {
+    HIPCHK (hipGraphLaunch (g->exec, s));
+    return NCKDE_OK;
+  }
+
+  if (g == nullptr)
+  {
+    if (cache.size () >= 16)
+    {
+      if (cache.front ().exec)
+        (void) hipGraphExecDestroy (cache.front ().exec);
+
+      cache.erase (cache.begin ());
+    }
+
+    cache.push_back ({dX, dout, m, precision, linear, s, h->graph_gen, 1, nullptr});
+    return run ();
+  }
+
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  // nothing inside the capture may wait on another stream
+  if (h->last_stream != nullptr && h->last_stream != s)
+    HIPCHK (hipStreamSynchronize (h->last_stream));
+
+  h->last_stream = s;
+
+  HIPCHK (hipStreamBeginCapture (s, hipStreamCaptureModeThreadLocal));
+
+  const int st = run ();
+  const hipError_t ec = hipStreamEndCapture (s, &graph);
+
+  if (st != NCKDE_OK)
+  {
+    if (graph)
+      (void) hipGraphDestroy (graph);
+
+    return st;
+  }
+
+  HIPCHK (ec);
+
+  const hipError_t ei = hipGraphInstantiate (&exec, graph, nullptr, nullptr, 0);
+
+  (void) hipGraphDestroy (graph);
+  HIPCHK (ei);
+  g->exec = exec;
+  HIPCHK (hipGraphLaunch (exec, s));
+
+  return NCKDE_OK;
 }
 
 static int
-eval_fp64_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s, char *ws)
+eval_fp64_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s, char *ws, bool f32)
 {
-  const int nsplit = fp64_nsplit (h, m);
+  const int nsplit = fp64_nsplit (h, m, f32 ? NCKDE_PREC_FP32 : NCKDE_PREC_FP64);
   double *partM, *partS;
 
   if (ws == nullptr)
@@ -1413,6 +1604,13 @@ eval_fp64_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, hi
   a.nsplit        = nsplit;
   a.out           = dout;
   a.linear        = linear;
+
+  if (f32)
+  {
+    a.Wf   = (h->kind == NCKDE_VKDE) ? h->dWf.as<float> () : nullptr;
+    a.b2f  = h->db2f.as<float> ();
+    a.bref = h->bref32;
+  }
 
   if (h->profiling)
   {
@@ -1486,6 +1684,7 @@ cv_set_lnos (nckde *h, double lnos, double factor)
   h->over_smooth = std::exp (lnos);
   h->href        = factor * get_href (h);
   h->mixed_ready = false;
+  h->graph_gen++;
 }
 
 // n(n-1) in the reference's unsigned arithmetic (guint products)
@@ -1986,6 +2185,10 @@ nckde_destroy (nckde *h)
   if (h->ready_ev)
     (void) hipEventDestroy (h->ready_ev);
 
+  for (auto &g : h->graph_cache)
+    if (g.exec)
+      (void) hipGraphExecDestroy (g.exec);
+
   delete h;
 
   return NCKDE_OK;
@@ -2049,6 +2252,30 @@ nckde_get_cv_stats (nckde *h, unsigned *n_objective, double *seconds)
 
   if (seconds)
     *seconds = h->cv_seconds;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_eval_split (nckde *h, unsigned nsplit)
+{
+  TRY (check_handle (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  h->split_override = nsplit;
+  h->graph_gen++;
+
+  return NCKDE_OK;
+}
+
+int
+nckde_set_graphs (nckde *h, int on)
+{
+  TRY (check_handle (h));
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  h->graphs = (on != 0);
+  h->graph_gen++;
 
   return NCKDE_OK;
 }
@@ -2697,7 +2924,7 @@ nckde_eval_m2lnp_device_noalloc (nckde *h, const double *dX, size_t m, double *d
     return NCKDE_EINVAL;
   }
 
-  if (precision != NCKDE_PREC_FP64 && precision != NCKDE_PREC_MIXED)
+  if (precision != NCKDE_PREC_FP64 && precision != NCKDE_PREC_MIXED && precision != NCKDE_PREC_FP32)
   {
     set_error ("nckde: unknown precision %d", precision);
     return NCKDE_EINVAL;
@@ -2729,6 +2956,13 @@ nckde_eval_m2lnp_device_noalloc (nckde *h, const double *dX, size_t m, double *d
       precision = NCKDE_PREC_FP64;
   }
 
+  if (precision == NCKDE_PREC_FP32 && (!h->f32_ready || !h->f32_b_ready))
+  {
+    set_error ("nckde_eval_m2lnp_device_noalloc: the FP32 factors are not built; an allocating FP32 eval after "
+               "the prepare builds them");
+    return NCKDE_ESTATE;
+  }
+
   size_t need;
 
   if (precision == NCKDE_PREC_MIXED)
@@ -2740,7 +2974,7 @@ nckde_eval_m2lnp_device_noalloc (nckde *h, const double *dX, size_t m, double *d
   }
   else
   {
-    const size_t part = sizeof (double) * m * fp64_nsplit (h, m);
+    const size_t part = sizeof (double) * m * fp64_nsplit (h, m, precision);
 
     need = align256 (part) + part;
   }
@@ -2765,7 +2999,7 @@ nckde_eval_m2lnp_device_noalloc (nckde *h, const double *dX, size_t m, double *d
   if (precision == NCKDE_PREC_MIXED)
     return eval_mixed_ws (h, dX, m, dout, 0, s, ws);
 
-  return eval_fp64_ws (h, dX, m, dout, 0, s, ws);
+  return eval_fp64_ws (h, dX, m, dout, 0, s, ws, precision == NCKDE_PREC_FP32);
 }
 
 int

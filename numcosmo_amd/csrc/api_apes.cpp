@@ -52,6 +52,10 @@ log_normal_interval (double zl, double zu)
 
 } // namespace
 
+// probe tolerance of the walker's mixed-precision densities: half of the
+// 3e-6 relative bar the transition densities are held to
+constexpr double APES_MIXED_TOL = 1.5e-6;
+
 struct nckde_apes
 {
   unsigned n, n2, d;
@@ -502,6 +506,17 @@ nckde_apes_set_precision (nckde_apes *a, int precision)
   }
 
   a->precision = precision;
+
+  // the walker holds its mixed-precision transition densities to 3e-6
+  // relative (half of it for the probe of the factors), not to the 1e-6 bar
+  // of a plain eval
+  for (int w = 0; w < 2; ++w)
+  {
+    const int r = nckde_set_mixed_tolerance (a->sd[w], APES_MIXED_TOL);
+
+    if (r != NCKDE_OK)
+      return r;
+  }
 
   return NCKDE_OK;
 }

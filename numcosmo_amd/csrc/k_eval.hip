@@ -132,6 +132,160 @@ k_eval_f64 (EvalArgs a)
   }
 }
 
+// The FP32 rung of the precision ladder: the same walk in single precision.
+// x - theta_i is formed in FP64 and rounded to FP32; W_i comes from its FP32
+// copy; |z|^2 and the kernel run in FP32 against a fixed reference,
+// lnt = b2_i + ln K with b2_i = b_i - max_j b_j (FP32), with an online FP32
+// maximum and the sum in FP64.  The partial maximum is stored back in
+// natural units (+ max_j b_j) so that k_eval_merge serves both forms.
+template <int D, int KERNEL, int KIND>
+__global__ void __launch_bounds__ (256)
+k_eval_f32 (EvalArgs a)
+{
+  const int p     = blockIdx.x * 256 + threadIdx.x;
+  const int chunk = (a.n + a.nsplit - 1) / a.nsplit;
+  const int i0    = blockIdx.y * chunk;
+  const int i1    = min (a.n, i0 + chunk);
+  const int pp    = (p < a.m) ? p : (a.m - 1);
+  const int d     = a.d;
+  double x[D];
+
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    x[k] = (k < d) ? a.X[(size_t) pp * d + k] : 0.0;
+
+  if (KIND == NCKDE_KDE)
+  {
+    double v[D];
+
+#pragma unroll
+    for (int l = 0; l < D; ++l)
+    {
+      if (l < d)
+      {
+        double t = x[l];
+
+#pragma unroll
+        for (int m = 0; m < l; ++m)
+          t = fma (-v[m], a.Ug[m * d + l], t);
+
+        v[l] = t / a.Ug[l * d + l];
+      }
+      else
+      {
+        v[l] = 0.0;
+      }
+    }
+
+#pragma unroll
+    for (int l = 0; l < D; ++l)
+      x[l] = v[l];
+  }
+
+  const float inv_h2 = (float) a.inv_h2;
+  const float kappa  = (float) a.kappa;
+  const float inv_nu = (float) (1.0 / a.nu);
+  float M            = -INFINITY;
+  double S           = 0.0;
+
+  for (int i = i0; i < i1; ++i)
+  {
+    const double *th = a.Th + (size_t) i * D;
+    float chi2       = 0.0f;
+
+    if (KIND == NCKDE_VKDE)
+    {
+      const float *W = a.Wf + (size_t) i * (D * (D + 1) / 2);
+      float delta[D];
+
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+        delta[k] = (float) (x[k] - th[k]);
+
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+      {
+        float z = 0.0f;
+
+#pragma unroll
+        for (int c = 0; c <= r; ++c)
+          z = fmaf (W[r * (r + 1) / 2 + c], delta[c], z);
+
+        chi2 = fmaf (z, z, chi2);
+      }
+    }
+    else
+    {
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+      {
+        const float t = (float) (th[k] - x[k]);
+
+        chi2 = fmaf (t, t, chi2);
+      }
+    }
+
+    chi2 *= inv_h2;
+
+    float lnt;
+
+    if (KERNEL == NCKDE_KERNEL_GAUSS)
+      lnt = fmaf (-0.5f, chi2, a.b2f[i]);
+    else
+      lnt = fmaf (kappa, log1pf (chi2 * inv_nu), a.b2f[i]);
+
+    if (lnt > M)
+    {
+      S = S * (double) expf (M - lnt);
+      M = lnt;
+    }
+
+    S += (lnt > -INFINITY) ? (double) expf (lnt - M) : 0.0;
+  }
+
+  if (p < a.m)
+  {
+    a.partM[(size_t) blockIdx.y * a.m + p] = (M > -INFINITY) ? (double) M + a.bref : -INFINITY;
+    a.partS[(size_t) blockIdx.y * a.m + p] = S;
+  }
+}
+
+// FP32 copies of the packed factors and the shifted b_i
+__global__ void
+k_f32_factors (const double *__restrict__ Wp, size_t count, float *__restrict__ Wf)
+{
+  const size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+
+  if (e < count)
+    Wf[e] = (float) Wp[e];
+}
+
+__global__ void
+k_f32_b2 (const double *__restrict__ b, double bref, int n, float *__restrict__ b2f)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+
+  if (i < n)
+    b2f[i] = (float) (b[i] - bref);
+}
+
+hipError_t
+launch_f32_factors (const double *Wp, size_t count, float *Wf, hipStream_t s)
+{
+  if (count > 0)
+    hipLaunchKernelGGL (k_f32_factors, dim3 ((unsigned) ((count + 255) / 256)), dim3 (256), 0, s, Wp, count, Wf);
+
+  return hipGetLastError ();
+}
+
+hipError_t
+launch_f32_b2 (const double *b, double bref, int n, float *b2f, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_f32_b2, dim3 ((n + 255) / 256), dim3 (256), 0, s, b, bref, n, b2f);
+
+  return hipGetLastError ();
+}
+
 // gamma = max over splits, 1 + lambda = sum_s S_s exp(M_s - gamma);
 // m2lnp = -2 (gamma - lnnorm_shared + log1p(lambda) - d ln h)
 __global__ void
@@ -168,7 +322,10 @@ launch_t (const EvalArgs &a, hipStream_t s)
 {
   dim3 grid ((a.m + 255) / 256, a.nsplit);
 
-  hipLaunchKernelGGL ((k_eval_f64<D, KERNEL, KIND>), grid, dim3 (256), 0, s, a);
+  if (a.b2f != nullptr)
+    hipLaunchKernelGGL ((k_eval_f32<D, KERNEL, KIND>), grid, dim3 (256), 0, s, a);
+  else
+    hipLaunchKernelGGL ((k_eval_f64<D, KERNEL, KIND>), grid, dim3 (256), 0, s, a);
 
   return hipGetLastError ();
 }

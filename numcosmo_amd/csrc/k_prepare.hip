@@ -591,9 +591,15 @@ launch_whiten (const double *X, const double *Ug, double *Y, int n, int d, hipSt
 // query:
 //   1. squared distances to every point, summed over coordinates in order,
 //      stored as their IEEE bit patterns (monotone for non-negative doubles);
-//   2. radix select (8 passes of 8 bits) of the k-th smallest key;
+//      the zero keys (the query itself, duplicates) are counted apart;
+//   2. radix select (passes of 8 bits) of the k-th smallest key among the
+//      positive ones, starting after the leading bytes that all of them share
+//      (those of the smallest and the largest);
 //   3. collect keys below it, then the smallest-index points equal to it;
 //   4. bitonic sort of the k (key, index) pairs; write the indices.
+// The keys sit in LDS when a query's keys fit (small samples, e.g. an APES
+// half-ensemble), else in HBM.  The histogram is kept per wave (4 x 256
+// bins), and the lanes of a wave that share a digit add to it with one atomic.
 // --------------------------------------------------------------------------
 constexpr int KNN_THREADS = 256;
 constexpr int KNN_KMAX    = 4096;
@@ -604,253 +610,8 @@ knn_max_k ()
   return KNN_KMAX;
 }
 
-__global__ void __launch_bounds__ (KNN_THREADS)
-k_knn (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, uint64_t *__restrict__ keys_all,
-       int *__restrict__ out, int *__restrict__ status)
-{
-  __shared__ uint64_t skey[KNN_KMAX];
-  __shared__ int sidx[KNN_KMAX];
-  __shared__ unsigned hist[256];
-  __shared__ uint64_t s_prefix;
-  __shared__ int s_kk;
-  __shared__ int s_nlt;
-  __shared__ int s_eq_taken;
-  __shared__ int s_scan[KNN_THREADS];
-  __shared__ unsigned s_wsum[KNN_THREADS / 64];
-
-  const int qb  = blockIdx.x;
-  const int tid = threadIdx.x;
-
-  if (qb >= nq)
-    return;
-
-  const int q       = q0 + qb;
-  uint64_t *keys    = keys_all + (size_t) qb * n;
-  const double *yq  = Y + (size_t) q * d;
-
-  // 1. distances
-  for (int j = tid; j < n; j += KNN_THREADS)
-  {
-    const double *yj = Y + (size_t) j * d;
-    double dist      = 0.0;
-
-    for (int l = 0; l < d; l++)
-    {
-      const double t = yj[l] - yq[l];
-
-      dist = dist + t * t;
-    }
-
-    keys[j] = (uint64_t) __double_as_longlong (dist);
-  }
-
-  if (tid == 0)
-  {
-    s_prefix = 0;
-    s_kk     = k;
-  }
-
-  __syncthreads ();
-
-  // 2. radix select of the k-th smallest key (1-based rank s_kk)
-  uint64_t mask = 0;
-
-  for (int pass = 0; pass < 8; ++pass)
-  {
-    const int shift = 56 - 8 * pass;
-
-    hist[tid] = 0;
-    __syncthreads ();
-
-    const uint64_t prefix = s_prefix;
-
-    for (int j = tid; j < n; j += KNN_THREADS)
-    {
-      const uint64_t key = keys[j];
-
-      if ((key & mask) == prefix)
-        atomicAdd (&hist[(key >> shift) & 255u], 1u);
-    }
-
-    __syncthreads ();
-
-    // the digit holding rank s_kk: the one bin whose exclusive prefix count
-    // is below the rank and whose inclusive count reaches it (a block scan
-    // of the 256 bins, one per thread: wave scans, then the wave totals)
-    {
-      const int lane    = tid & 63;
-      const int wave    = tid >> 6;
-      const unsigned h  = hist[tid];
-      unsigned v        = h;
-
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1)
-      {
-        const unsigned t = __shfl_up (v, o, 64);
-
-        if (lane >= o)
-          v += t;
-      }
-
-      if (lane == 63)
-        s_wsum[wave] = v;
-
-      const int kk = s_kk;
-
-      __syncthreads ();
-
-      unsigned base = 0;
-
-      for (int w = 0; w < wave; ++w)
-        base += s_wsum[w];
-
-      const unsigned incl = base + v;
-      const unsigned excl = incl - h;
-
-      if (excl < (unsigned) kk && incl >= (unsigned) kk)
-      {
-        s_kk     = kk - (int) excl;
-        s_prefix = prefix | ((uint64_t) tid << shift);
-      }
-    }
-
-    mask |= (uint64_t) 255 << shift;
-    __syncthreads ();
-  }
-
-  const uint64_t T = s_prefix;
-  const int need_eq = s_kk;   // number of points equal to T to take
-
-  // 3. collect: all keys < T (any order), then the need_eq smallest indices
-  //    with key == T, scanning the points in index order chunk by chunk
-  if (tid == 0)
-  {
-    s_nlt      = 0;
-    s_eq_taken = 0;
-  }
-
-  __syncthreads ();
-
-  for (int j = tid; j < n; j += KNN_THREADS)
-  {
-    const uint64_t key = keys[j];
-
-    if (key < T)
-    {
-      const int slot = atomicAdd (&s_nlt, 1);
-
-      if (slot < KNN_KMAX)
-      {
-        skey[slot] = key;
-        sidx[slot] = j;
-      }
-    }
-  }
-
-  __syncthreads ();
-
-  const int nlt = s_nlt;
-
-  for (int base = 0; base < n; base += KNN_THREADS)
-  {
-    if (s_eq_taken >= need_eq)
-      break;
-
-    const int j    = base + tid;
-    const int flag = (j < n && keys[j] == T) ? 1 : 0;
-
-    s_scan[tid] = flag;
-    __syncthreads ();
-
-    for (int off = 1; off < KNN_THREADS; off <<= 1)
-    {
-      const int v = (tid >= off) ? s_scan[tid - off] : 0;
-
-      __syncthreads ();
-      s_scan[tid] += v;
-      __syncthreads ();
-    }
-
-    const int taken = s_eq_taken;
-    const int rank  = s_scan[tid] - flag;   // exclusive
-
-    if (flag && taken + rank < need_eq)
-    {
-      const int slot = nlt + taken + rank;
-
-      if (slot < KNN_KMAX)
-      {
-        skey[slot] = T;
-        sidx[slot] = j;
-      }
-    }
-
-    __syncthreads ();
-
-    if (tid == KNN_THREADS - 1)
-      s_eq_taken = taken + s_scan[KNN_THREADS - 1];
-
-    __syncthreads ();
-  }
-
-  if (nlt + need_eq != k && tid == 0)
-    status[0] = 2;
-
-  // 4. bitonic sort of (key, idx), padded to a power of two
-  int npow = 1;
-
-  while (npow < k)
-    npow <<= 1;
-
-  for (int i = k + tid; i < npow; i += KNN_THREADS)
-  {
-    skey[i] = ~(uint64_t) 0;
-    sidx[i] = 0x7fffffff;
-  }
-
-  __syncthreads ();
-
-  for (int size = 2; size <= npow; size <<= 1)
-  {
-    for (int stride = size >> 1; stride > 0; stride >>= 1)
-    {
-      for (int i = tid; i < npow; i += KNN_THREADS)
-      {
-        const int pj = i ^ stride;
-
-        if (pj > i)
-        {
-          const bool up      = ((i & size) == 0);
-          const uint64_t ka  = skey[i], kb = skey[pj];
-          const int ia       = sidx[i], ib = sidx[pj];
-          const bool a_gt_b  = (ka > kb) || (ka == kb && ia > ib);
-
-          if (a_gt_b == up)
-          {
-            skey[i]  = kb;
-            skey[pj] = ka;
-            sidx[i]  = ib;
-            sidx[pj] = ia;
-          }
-        }
-      }
-
-      __syncthreads ();
-    }
-  }
-
-  int *o = out + (size_t) q * k;
-
-  for (int i = tid; i < k; i += KNN_THREADS)
-    o[i] = sidx[i];
-}
-
-// The same selection with the query's keys in LDS when they fit (small
-// samples, e.g. an APES half-ensemble): no key array in HBM, the radix passes
-// read LDS, the histogram is kept per wave (4 x 256 bins) so that lanes of
-// different waves never contend, and the leading bytes that every key shares
-// (those of the smallest and largest key) are taken without a pass.
-// Dynamic LDS: keys[n], then the sort buffers skey[npow], sidx[npow].
+// Dynamic LDS: keys[n] (when they fit), then the sort buffers skey[npow],
+// sidx[npow].
 constexpr int KNN_LDS_BYTES = 56 * 1024;
 
 static int
@@ -897,13 +658,14 @@ hist_add_wave (unsigned *hist, bool active, unsigned digit)
   }
 }
 
+template <bool LDS_KEYS>
 __global__ void __launch_bounds__ (KNN_THREADS)
-k_knn_lds (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, int npow, int *__restrict__ out,
-           int *__restrict__ status)
+k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int npow, uint64_t *__restrict__ keys_all,
+       int *__restrict__ out, int *__restrict__ status)
 {
   extern __shared__ uint64_t knn_smem[];
-  uint64_t *keys = knn_smem;
-  uint64_t *skey = knn_smem + n;
+  uint64_t *keys = LDS_KEYS ? knn_smem : keys_all + (size_t) blockIdx.x * n;
+  uint64_t *skey = LDS_KEYS ? knn_smem + n : knn_smem;
   int *sidx      = (int *) (skey + npow);
   __shared__ unsigned hist[KNN_THREADS / 64][256];
   __shared__ uint64_t s_prefix, s_wmin[KNN_THREADS / 64], s_wmax[KNN_THREADS / 64];
@@ -920,8 +682,7 @@ k_knn_lds (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, in
   if (qb >= nq)
     return;
 
-  const int q      = q0 + qb;
-  const double *yq = Y + (size_t) q * d;
+  const int q = q0 + qb;
 
   // 1. distances (coordinates summed in order, as the global-memory kernel);
   //    the extremes of the positive keys and the number of zero keys (the
@@ -931,12 +692,11 @@ k_knn_lds (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, in
 
   for (int j = tid; j < n; j += KNN_THREADS)
   {
-    const double *yj = Y + (size_t) j * d;
-    double dist      = 0.0;
+    double dist = 0.0;
 
     for (int l = 0; l < d; l++)
     {
-      const double t = yj[l] - yq[l];
+      const double t = Yt[(size_t) l * n + j] - Yt[(size_t) l * n + q];
 
       dist = dist + t * t;
     }
@@ -1200,14 +960,17 @@ k_knn_lds (const double *__restrict__ Y, int n, int d, int q0, int nq, int k, in
 }
 
 hipError_t
-launch_knn (const double *Y, int n, int d, int q0, int nq, int k, uint64_t *keys, int *out, int *status,
+launch_knn (const double *Yt, int n, int d, int q0, int nq, int k, uint64_t *keys, int *out, int *status,
             hipStream_t s)
 {
+  const size_t sort_bytes = (size_t) knn_npow (k) * (sizeof (uint64_t) + sizeof (int));
+
   if (knn_keys_in_lds (n, k))
-    hipLaunchKernelGGL (k_knn_lds, dim3 (nq), dim3 (KNN_THREADS), knn_lds_bytes (n, k), s, Y, n, d, q0, nq, k,
-                        knn_npow (k), out, status);
+    hipLaunchKernelGGL (k_knn<true>, dim3 (nq), dim3 (KNN_THREADS), knn_lds_bytes (n, k), s, Yt, n, d, q0, nq, k,
+                        knn_npow (k), keys, out, status);
   else
-    hipLaunchKernelGGL (k_knn, dim3 (nq), dim3 (KNN_THREADS), 0, s, Y, n, d, q0, nq, k, keys, out, status);
+    hipLaunchKernelGGL (k_knn<false>, dim3 (nq), dim3 (KNN_THREADS), sort_bytes, s, Yt, n, d, q0, nq, k,
+                        knn_npow (k), keys, out, status);
 
   return hipGetLastError ();
 }

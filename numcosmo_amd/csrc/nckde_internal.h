@@ -103,7 +103,8 @@ hipError_t launch_nearpd_fix (const int *which, int nfix, const double *cov, dou
 hipError_t launch_whiten (const double *X, const double *Ug, double *Y, int n, int d, hipStream_t s);
 // true when the kNN selection keeps a query's keys in LDS (no key buffer)
 bool knn_keys_in_lds (int n, int k);
-hipError_t launch_knn (const double *Y, int n, int d, int q0, int nq, int k, uint64_t *keys,
+// Yt: the whitened sample transposed (d x n, coordinate l of point j at l n + j)
+hipError_t launch_knn (const double *Yt, int n, int d, int q0, int nq, int k, uint64_t *keys,
                        int *out, int *status, hipStream_t s);
 size_t knn_max_k ();
 hipError_t launch_vkde_factors (const double *U, const double *X, int nk, int d, int D, double *Wp,
@@ -123,8 +124,14 @@ struct EvalArgs
   int nsplit;
   double *out;                     // m
   int linear;                      // 0: m2lnp, 1: density
+  // FP32 rung (b2f != NULL): FP32 factors (VKDE), b_i - bref in FP32
+  const float *Wf = nullptr, *b2f = nullptr;
+  double bref     = 0.0;
 };
+// FP64 eval, or the FP32 rung when a.b2f is set
 hipError_t launch_eval_f64 (const EvalArgs &a, hipStream_t s);
+hipError_t launch_f32_factors (const double *Wp, size_t count, float *Wf, hipStream_t s);
+hipError_t launch_f32_b2 (const double *b, double bref, int n, float *b2f, hipStream_t s);
 hipError_t launch_eval_merge (const EvalArgs &a, hipStream_t s);
 
 // ---- mixed-precision evaluation (k_eval_mixed.hip) ------------------------
@@ -205,6 +212,11 @@ hipError_t launch_row_dot_idx (const double *AT, const int *P, int np, int K, co
                                hipStream_t s);
 hipError_t launch_add_update (double *x, const double *dx, int np, double *stat, hipStream_t s);
 hipError_t launch_diag_range (const double *S, int np, double *out, hipStream_t s);
+// k_probe.hip: the np highest-risk kernels (sel) and two probe variates each
+// (kernel idx, z, chi2); the largest relative difference of two results
+hipError_t launch_probe_points (const float *risk, int n, int np, int d, double c, double chi2v, uint32_t *sel,
+                                uint32_t *idx, double *z, double *chi2, hipStream_t s);
+hipError_t launch_probe_err (const double *a, const double *b, int m, double *out, hipStream_t s);
 hipError_t launch_scatter (const double *sol, const int *P, int np, int n, double *x, hipStream_t s);
 hipError_t launch_shrink (double *w, int n, double shrink, int *status, hipStream_t s);
 

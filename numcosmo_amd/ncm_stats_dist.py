@@ -53,6 +53,7 @@ class StatsDistKDECovType(enum.IntEnum):
 class Precision(enum.IntEnum):
     FP64 = 0
     MIXED = 1
+    FP32 = 2
 
 
 class NNLSUMethod(enum.IntEnum):
@@ -191,6 +192,16 @@ class _StatsDist:
         e, f = C.c_double(), C.c_int()
         check(lib().nckde_get_mixed_check(self._h, C.byref(e), C.byref(f)))
         return e.value, bool(f.value)
+
+    def set_eval_split(self, nsplit):
+        """Force the eval's kernel-range split count (0: tuned table or
+        default rule); tuning hook, no reference counterpart."""
+        check(lib().nckde_set_eval_split(self._h, int(nsplit)))
+
+    def set_graphs(self, on):
+        """Replay repeated evals of one shape as captured HIP graphs; no
+        reference counterpart."""
+        check(lib().nckde_set_graphs(self._h, int(bool(on))))
 
     def set_mixed_tolerance(self, tol):
         """Probe error above which mixed evals run in FP64 (default 5e-7);

@@ -314,6 +314,53 @@ def test_eval_m2lnp_mixed_precision(gpu, oracle, kind, kernel, d, n, nu):
     assert abs(alone[0] - got[10]) <= 1e-6 * max(abs(ref[10]), 1.0)
 
 
+FP32_CASES = [(GAUSS, 3, 700, 0.0), (ST, 8, 1500, 3.0), (ST, 16, 2000, 1.0), (ST, 40, 1300, 5.0), (ST, 64, 1500, 5.0)]
+
+
+@pytest.mark.parametrize("kind", [1, 0])
+@pytest.mark.parametrize("kernel,d,n,nu", FP32_CASES)
+def test_eval_m2lnp_fp32_rung(gpu, oracle, kind, kernel, d, n, nu):
+    """The FP32 rung of the precision ladder (single-precision distances and
+    kernel sums, FP64 differences x - theta_i): within 1e-6 of the FP64
+    oracle, including points on kernel centres and far outliers; it follows
+    a refit's new weights."""
+    _say(f"[eval fp32] kind={kind} kernel={kernel} d={d} n={n}")
+    g = np.random.default_rng(5200 + d)
+    tgt = synth.Gaussian(d, g)
+    X = tgt.draw(g, n)
+    sd, od = _pair(gpu, oracle, kind, kernel, d, X, nu)
+    sd.prepare_interp(tgt.m2lnp(X)) if d <= 16 else sd.prepare()
+    od.prepare()
+    od.set_weights(sd.peek_weights())
+    Xe = tgt.draw(g, 1000)
+    Xe[:5] = X[:5]
+    Xe[5] = 50.0 * Xe[5]
+    ref = od.eval_m2lnp_batch(Xe)
+    got = sd.eval_m2lnp_batch(Xe, gpu.Precision.FP32)
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)
+    _say(f"   fp32 max rel err {err.max():.3e} p99 {np.quantile(err, 0.99):.3e}")
+    assert np.all(np.isfinite(got))
+    assert err.max() <= 1e-6
+    # the linear density carries the same error in its logarithm:
+    # |d ln p| = |d m2lnp| / 2
+    lin, lref = sd.eval_batch(Xe[:50], gpu.Precision.FP32), od.eval_batch(Xe[:50])
+    live = lref > 1e-300
+    assert np.all(np.abs(np.log(lin[live]) - np.log(lref[live])) <= 0.5e-6 * np.maximum(np.abs(ref[:50][live]), 1.0))
+    # new weights after a refit reach the FP32 factors
+    if d <= 16:
+        X2 = tgt.draw(g, n)
+        sd.reset()
+        sd.add_obs_array(X2)
+        sd.prepare_interp(tgt.m2lnp(X2))
+        od2 = oracle.StatsDist(od.kind, od.kernel, d, nu)
+        od2.add_obs_array(X2)
+        od2.prepare()
+        od2.set_weights(sd.peek_weights())
+        ref2 = od2.eval_m2lnp_batch(Xe[:200])
+        got2 = sd.eval_m2lnp_batch(Xe[:200], gpu.Precision.FP32)
+        assert np.max(np.abs(got2 - ref2) / np.maximum(np.abs(ref2), 1.0)) <= 1e-6
+
+
 def test_mixed_after_refit_uses_new_weights(gpu, oracle):
     g = np.random.default_rng(5100)
     d, n = 8, 800
