@@ -279,6 +279,17 @@ int nckde_get_knn (nckde *h, size_t *k, int32_t *idx);
  * may be NULL.  NCKDE_ESTATE when the last prepare fitted no NNLS. */
 int nckde_get_nnls_state (nckde *h, double *x, double *g, double *r);
 
+/* No reference counterpart: the evals of several handles at once, results
+ * bit-identical to one nckde_eval_m2lnp per handle.  Evaluation i takes
+ * m[i] host points X[i] (row stride ld[i] doubles; ld == NULL means d) into
+ * out[i].  In mixed precision the evals of handles with the same MFMA
+ * dimension and kernel type run as one launch per pass over a segment table
+ * (the served stream's many small requests fill the GPU together); other
+ * evals (other precisions, and d > 16 handles of 1024 kernels or more, whose
+ * evals start with a pilot pass) run one by one.  A handle may appear once. */
+int nckde_eval_m2lnp_multi (nckde *const *hs, const double *const *X, const size_t *m, const size_t *ld,
+                            double *const *out, size_t count, int precision);
+
 /* ---- allocation-free eval ---------------------------------------------- */
 
 /* No reference counterpart.  For callers that cannot allocate device memory

@@ -68,6 +68,8 @@ SIGNATURES = {
     "nckde_eval": (C.c_int, [_h, _dp, _sz, _sz, _dp, C.c_int]),
     "nckde_eval_m2lnp_device": (C.c_int, [_h, C.c_void_p, _sz, C.c_void_p, C.c_int, C.c_void_p]),
     "nckde_eval_m2lnp_device_noalloc": (C.c_int, [_h, C.c_void_p, _sz, C.c_void_p, C.c_int, C.c_void_p]),
+    "nckde_eval_m2lnp_multi": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(_sz),
+                                         C.c_void_p, C.POINTER(C.c_void_p), _sz, C.c_int]),
     "nckde_noalloc_scratch_bytes": (_sz, []),
     "nckde_kernel_choose": (C.c_int, [_h, _dp, _sz, _up]),
     "nckde_sample": (C.c_int, [_h, _dp, _dp, C.c_void_p, _sz, _dp, C.c_void_p]),

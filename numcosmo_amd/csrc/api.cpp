@@ -133,9 +133,11 @@ struct nckde
   // mixed-precision factors (built at the first mixed eval after a prepare)
   DBuf dmu, dYc, dymax, dA16, dc32, dcoef, dYw, dpartMf, dflag;
   bool mixed_ready = false, mixed_b_ready = false;
-  // probe of the mixed factors (built with them): risk score per kernel,
-  // probe points and their results; the largest relative m2lnp error the
-  // probes found, and whether mixed evals run in FP64 for that reason
+  // pilot of the mixed eval (d > 16 and at least 1024 kernels): copies of
+  // the factors of every pilot_stride-th kernel, pilot_n of them
+  DBuf dpA16, dpc32, dpcoef;
+  int pilot_n         = 0;
+  size_t pilot_stride = 0;
   // captured eval launch sequences: an eval of the same shape on the same
   // buffers and stream is replayed as one graph launch from its second
   // repetition on; graph_gen counts the changes (prepare, weights, factor
@@ -159,7 +161,11 @@ struct nckde
   DBuf dWf, db2f;
   bool f32_ready = false, f32_b_ready = false;
   double bref32  = 0.0;
+  // probe of the mixed factors (built with them): risk score per kernel,
+  // probe points and their results; the largest relative m2lnp error the
+  // probes found, and whether mixed evals run in FP64 for that reason
   DBuf drisk, dprobe;
+  DBuf dmulti;   // argument table of fused multi-handle evals led by this handle
   double mixed_probe_err = 0.0;
   double mixed_tol       = 5.0e-7;   // probe error above which mixed evals run in FP64
   bool mixed_fallback    = false;
@@ -1153,6 +1159,45 @@ mixed_probe (nckde *h)
   return NCKDE_OK;
 }
 
+// The mixed eval's pilot: PILOT_KERNELS kernels at a fixed stride over the
+// sample.  At d > 16 the terms of most points lie far below b_ref (a
+// Student-t term falls as (nu + d)/2 log(1 + chi2/nu)), so the main pass runs
+// against the pilot's largest term per point instead; without it most points
+// of a d = 64 eval would underflow and run twice.
+constexpr int PILOT_KERNELS   = 256;
+constexpr size_t PILOT_MIN_N  = 4 * PILOT_KERNELS;
+
+// copies the pilot kernels' factors (coef only when coef_only)
+static int
+pilot_gather (nckde *h, bool coef_only)
+{
+  const int DP = nck::mfma_dim (h->d);
+
+  if (DP < 32 || h->n_k < PILOT_MIN_N)
+  {
+    h->pilot_n = 0;
+    return NCKDE_OK;
+  }
+
+  const size_t np = PILOT_KERNELS, st = h->n_k / PILOT_KERNELS;
+  const size_t wA = sizeof (_Float16) * 2 * DP * DP, wc = sizeof (float) * DP, wf = sizeof (float) * 4;
+
+  if (!coef_only)
+  {
+    HIPCHK (h->dpA16.ensure (wA * np));
+    HIPCHK (h->dpc32.ensure (wc * np));
+    HIPCHK (h->dpcoef.ensure (wf * np));
+    HIPCHK (hipMemcpy2DAsync (h->dpA16.p, wA, h->dA16.p, wA * st, wA, np, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK (hipMemcpy2DAsync (h->dpc32.p, wc, h->dc32.p, wc * st, wc, np, hipMemcpyDeviceToDevice, h->stream));
+  }
+
+  HIPCHK (hipMemcpy2DAsync (h->dpcoef.p, wf, h->dcoef.p, wf * st, wf, np, hipMemcpyDeviceToDevice, h->stream));
+  h->pilot_n      = (int) np;
+  h->pilot_stride = st;
+
+  return NCKDE_OK;
+}
+
 static int
 ensure_mixed (nckde *h)
 {
@@ -1177,6 +1222,7 @@ ensure_mixed (nckde *h)
                               h->dmu.as<double> (), h->dYc.as<double> (),
                               h->dymax.as<unsigned long long> (), &h->ysc, h->dA16.as<_Float16> (),
                               h->dc32.as<float> (), h->dcoef.as<float> (), h->drisk.as<float> (), h->stream));
+    TRY (pilot_gather (h, false));
     h->mixed_ready   = true;
     h->mixed_b_ready = true;
     h->graph_gen++;
@@ -1186,6 +1232,7 @@ ensure_mixed (nckde *h)
   {
     TRY (max_b (h, h->bref));
     HIPCHK (nck::mixed_refresh_b (h->db.as<double> (), h->bref, (int) h->n_k, h->dcoef.as<float> (), h->stream));
+    TRY (pilot_gather (h, true));
     h->graph_gen++;
     h->mixed_b_ready = true;
   }
@@ -1234,8 +1281,10 @@ ensure_f32 (nckde *h)
 // the padded dimension D otherwise), log2 n and log2 m of the tuned case,
 // split count.  An eval uses the entry of its precision and dimension
 // nearest in (log2 n, log2 m) within one octave in each, else the default
-// rule (about 2048 blocks in all, each split at least 8 kernel groups
-// (mixed) or 64 kernels (FP64)).
+// rule: about 8192 blocks in all and at least 8 kernel groups per split
+// (mixed), about 16384 blocks and at least 32 kernels per split (FP64 and
+// FP32).  Many short splits keep a split's kernel factors in its XCD's L2
+// and even out the blocks' finishing times.
 struct SplitTune
 {
   int prec, dim, lgn, lgm, nsplit;
@@ -1283,7 +1332,7 @@ mixed_nsplit (const nckde *h, size_t m)
     return std::max (1, std::min (chosen, ngroups));
 
   const int bx = (int) ((m + nck::mixed_points_per_block (DP) - 1) / nck::mixed_points_per_block (DP));
-  int nsplit   = (2048 + bx - 1) / bx;
+  int nsplit   = (8192 + bx - 1) / bx;
 
   return std::max (1, std::min (nsplit, std::max (1, ngroups / 8)));
 }
@@ -1297,8 +1346,8 @@ fp64_nsplit (const nckde *h, size_t m, int prec)
     return std::max (1, std::min (chosen, (int) h->n_k));
 
   const int nblk = (int) ((m + 255) / 256);
-  int nsplit     = (2048 + nblk - 1) / nblk;
-  const int maxs = (int) ((h->n_k + 63) / 64);
+  int nsplit     = (16384 + nblk - 1) / nblk;
+  const int maxs = (int) ((h->n_k + 31) / 32);
 
   return std::max (1, std::min (nsplit, maxs));
 }
@@ -1328,6 +1377,8 @@ mixed_work_bytes (const nckde *h, size_t m, size_t sz[4])
 // mixed_work_bytes, each start aligned to 256 bytes) instead of the handle's
 static int
 eval_mixed_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s, char *ws);
+static int mixed_args (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s, char *ws,
+                       nck::MixedArgs &a);
 
 static int
 eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s)
@@ -1337,6 +1388,29 @@ eval_mixed (nckde *h, const double *dX, size_t m, double *dout, int linear, hipS
 
 static int
 eval_mixed_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s, char *ws)
+{
+  nck::MixedArgs a;
+
+  TRY (mixed_args (h, dX, m, dout, linear, s, ws, a));
+
+  if (h->profiling)
+  {
+    a.ev0           = h->pev[0];
+    a.ev1           = h->pev[1];
+    h->pev_recorded = true;
+  }
+
+  HIPCHK (nck::launch_mixed_points (a, s));
+  HIPCHK (nck::launch_eval_mixed (a, s));
+
+  return NCKDE_OK;
+}
+
+// the arguments of a mixed eval of m device points on stream s, with its
+// work arrays from the handle (ws == NULL, sized here) or carved from ws
+static int
+mixed_args (nckde *h, const double *dX, size_t m, double *dout, int linear, hipStream_t s, char *ws,
+            nck::MixedArgs &a)
 {
   const int DP     = nck::mfma_dim (h->d);
   const int nsplit = mixed_nsplit (h, m);
@@ -1369,8 +1443,6 @@ eval_mixed_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, h
     partS  = (char *) partMf + align256 (sz[2]);
   }
 
-  nck::MixedArgs a;
-
   a.kernel = h->kernel;
   a.d      = h->d;
   a.n      = (int) h->n_k;
@@ -1396,16 +1468,18 @@ eval_mixed_ws (nckde *h, const double *dX, size_t m, double *dout, int linear, h
   a.bref   = h->bref;
   a.out    = dout;
   a.linear = linear;
+  a.pn     = h->pilot_n;
 
-  if (h->profiling)
+  if (a.pn > 0)
   {
-    a.ev0           = h->pev[0];
-    a.ev1           = h->pev[1];
-    h->pev_recorded = true;
-  }
+    const int bx = (int) ((m + nck::mixed_points_per_block (DP) - 1) / nck::mixed_points_per_block (DP));
 
-  HIPCHK (nck::launch_mixed_points (a, s));
-  HIPCHK (nck::launch_eval_mixed (a, s));
+    a.pf.DP    = DP;
+    a.pf.A     = h->dpA16.p;
+    a.pf.c     = h->dpc32.as<float> ();
+    a.pf.coef  = h->dpcoef.as<float> ();
+    a.pnsplit  = std::max (1, std::min ({(1024 + bx - 1) / bx, 8, nsplit}));
+  }
 
   return NCKDE_OK;
 }
@@ -3000,6 +3074,166 @@ nckde_eval_m2lnp_device_noalloc (nckde *h, const double *dX, size_t m, double *d
     return eval_mixed_ws (h, dX, m, dout, 0, s, ws);
 
   return eval_fp64_ws (h, dX, m, dout, 0, s, ws, precision == NCKDE_PREC_FP32);
+}
+
+// ---- several handles' evals in fused launches (no reference counterpart) -
+// Mixed-precision evals of handles that share the MFMA dimension and kernel
+// type run as one launch per pass over a segment table; every segment's
+// blocks compute what the handle's own eval computes, so the results are
+// bit-identical to separate calls.  Other precisions, and handles whose
+// mixed factors fall back to FP64, run one by one.
+int
+nckde_eval_m2lnp_multi (nckde *const *hs, const double *const *X, const size_t *m, const size_t *ld,
+                        double *const *out, size_t count, int precision)
+{
+  if (count == 0)
+    return NCKDE_OK;
+
+  if (hs == nullptr || X == nullptr || m == nullptr || out == nullptr)
+  {
+    set_error ("nckde_eval_m2lnp_multi: NULL argument");
+    return NCKDE_EINVAL;
+  }
+
+  if (precision != NCKDE_PREC_FP64 && precision != NCKDE_PREC_MIXED && precision != NCKDE_PREC_FP32)
+  {
+    set_error ("nckde: unknown precision %d", precision);
+    return NCKDE_EINVAL;
+  }
+
+  std::vector<nckde *> order (hs, hs + count);
+
+  for (size_t i = 0; i < count; ++i)
+  {
+    TRY (check_handle (hs[i]));
+    TRY (check_prepared (hs[i]));
+
+    const size_t ldi = ld ? ld[i] : (size_t) hs[i]->d;
+
+    if (m[i] > 0 && (X[i] == nullptr || out[i] == nullptr || ldi < (size_t) hs[i]->d))
+    {
+      set_error ("nckde_eval_m2lnp_multi: invalid buffers for evaluation %zu", i);
+      return NCKDE_EINVAL;
+    }
+  }
+
+  // the handles' locks in address order (no handle twice)
+  std::sort (order.begin (), order.end ());
+
+  if (std::adjacent_find (order.begin (), order.end ()) != order.end ())
+  {
+    set_error ("nckde_eval_m2lnp_multi: a handle appears twice");
+    return NCKDE_EINVAL;
+  }
+
+  std::vector<std::unique_lock<std::mutex>> locks;
+
+  for (nckde *h : order)
+    locks.emplace_back (h->mtx);
+
+  // groups of fusable evals by (MFMA dimension, kernel type)
+  std::vector<std::vector<size_t>> groups;
+  std::vector<int> keys;
+
+  for (size_t i = 0; i < count; ++i)
+  {
+    nckde *h = hs[i];
+
+    if (m[i] == 0)
+      continue;
+
+    bool fuse = (precision == NCKDE_PREC_MIXED);
+
+    if (fuse)
+    {
+      TRY (ensure_mixed (h));
+      fuse = !h->mixed_fallback && h->pilot_n == 0;
+    }
+
+    if (!fuse)
+    {
+      TRY (eval_host (h, X[i], m[i], ld ? ld[i] : (size_t) h->d, out[i], precision, 0));
+      continue;
+    }
+
+    const int key = nck::mfma_dim (h->d) * 2 + h->kernel;
+    size_t g      = 0;
+
+    while (g < keys.size () && keys[g] != key)
+      ++g;
+
+    if (g == keys.size ())
+    {
+      keys.push_back (key);
+      groups.emplace_back ();
+    }
+
+    groups[g].push_back (i);
+  }
+
+  for (size_t g = 0; g < groups.size (); ++g)
+  {
+    const std::vector<size_t> &mem = groups[g];
+    nckde *h0                      = hs[mem[0]];
+    hipStream_t S                  = h0->stream;
+    const int nseg                 = (int) mem.size ();
+    std::vector<nck::MixedArgs> args (nseg);
+    std::vector<int> block0 (2 * (nseg + 1), 0);
+
+    for (int k = 0; k < nseg; ++k)
+    {
+      nckde *h         = hs[mem[k]];
+      const size_t mi  = m[mem[k]];
+      const size_t ldi = ld ? ld[mem[k]] : (size_t) h->d;
+      const int d      = h->d;
+
+      // the handle's own work (factor builds) ends before S uses its
+      // buffers, and so does an earlier eval on another stream
+      if (h->stream != S)
+      {
+        HIPCHK (hipEventRecord (h->ready_ev, h->stream));
+        HIPCHK (hipStreamWaitEvent (S, h->ready_ev, 0));
+      }
+
+      if (h->last_stream != nullptr && h->last_stream != S)
+        HIPCHK (hipStreamSynchronize (h->last_stream));
+
+      h->last_stream = S;
+
+      HIPCHK (h->dXe.ensure (sizeof (double) * mi * d));
+      HIPCHK (h->dout.ensure (sizeof (double) * mi));
+
+      if (ldi == (size_t) d)
+        HIPCHK (hipMemcpyAsync (h->dXe.p, X[mem[k]], sizeof (double) * mi * d, hipMemcpyHostToDevice, S));
+      else
+        HIPCHK (hipMemcpy2DAsync (h->dXe.p, sizeof (double) * d, X[mem[k]], sizeof (double) * ldi,
+                                  sizeof (double) * d, mi, hipMemcpyHostToDevice, S));
+
+      TRY (mixed_args (h, h->dXe.as<double> (), mi, h->dout.as<double> (), 0, S, nullptr, args[k]));
+      HIPCHK (nck::launch_mixed_points (args[k], S));
+      block0[k + 1]            = block0[k] + nck::mixed_eval_blocks (args[k]);
+      block0[nseg + 1 + k + 1] = block0[nseg + 1 + k] + nck::mixed_merge_blocks (args[k]);
+    }
+
+    HIPCHK (h0->dmulti.ensure (sizeof (nck::MixedArgs) * nseg + sizeof (int) * block0.size ()));
+
+    nck::MixedArgs *dargs = h0->dmulti.as<nck::MixedArgs> ();
+    int *dblock0          = (int *) (dargs + nseg);
+
+    HIPCHK (hipMemcpyAsync (dargs, args.data (), sizeof (nck::MixedArgs) * nseg, hipMemcpyHostToDevice, S));
+    HIPCHK (hipMemcpyAsync (dblock0, block0.data (), sizeof (int) * block0.size (), hipMemcpyHostToDevice, S));
+    HIPCHK (nck::launch_eval_mixed_multi (args[0].f.DP, h0->kernel, dargs, dblock0, nseg, block0[nseg],
+                                          block0[2 * nseg + 1], S));
+
+    for (int k = 0; k < nseg; ++k)
+      HIPCHK (hipMemcpyAsync (out[mem[k]], hs[mem[k]]->dout.p, sizeof (double) * m[mem[k]], hipMemcpyDeviceToHost,
+                              S));
+
+    // the argument table and block offsets stay alive until S is done
+    HIPCHK (hipStreamSynchronize (S));
+  }
+
+  return NCKDE_OK;
 }
 
 int

@@ -212,6 +212,11 @@ k_eval_f32 (EvalArgs a)
           z = fmaf (W[r * (r + 1) / 2 + c], delta[c], z);
 
         chi2 = fmaf (z, z, chi2);
+
+        // one row's factor entries live at a time: without the fence the
+        // scalar loads of every row are hoisted together and spill at D >= 32
+        if (D >= 32)
+          __builtin_amdgcn_sched_barrier (0);
       }
     }
     else

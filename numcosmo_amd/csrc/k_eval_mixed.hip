@@ -24,6 +24,7 @@
 // ncm_stats_dist_kernel_gauss.c:246-290 / ncm_stats_dist_kernel_st.c:295-341.
 #include "nckde_internal.h"
 
+#include <cfloat>
 #include <cstring>
 
 namespace nck
@@ -946,9 +947,11 @@ group_loop (int g0, int g1, const _Float16 *__restrict__ A, const float *__restr
   }
 }
 
+// The evaluation of one block: block (bx, by) takes the point tiles of
+// column bx and the kernel groups of split by.
 template <int DP, int KERNEL, int NPT, bool OFFSET>
-__global__ void __launch_bounds__ (64 * waves_per_block (DP))
-k_eval_mixed (MixedArgs a)
+__device__ inline void
+eval_mixed_block (const MixedArgs &a, int bx, int by)
 {
   using P        = Plan<DP>;
   constexpr int NST = (DP >= 32) ? NPT / 2 : NPT;   // LSE states per lane
@@ -965,10 +968,10 @@ k_eval_mixed (MixedArgs a)
   const float4 *coef = (const float4 *) a.f.coef;
   const float kappa  = (float) a.kappa;
 
-  const int tile0   = (blockIdx.x * waves_per_block (DP) + wave) * NPT;
+  const int tile0   = (bx * waves_per_block (DP) + wave) * NPT;
   const int ngroups = (a.n + P::KPT - 1) / P::KPT;
   const int chunk   = (ngroups + a.nsplit - 1) / a.nsplit;
-  const int g0      = blockIdx.y * chunk;
+  const int g0      = by * chunk;
   const int g1      = min (ngroups, g0 + chunk);
 
   // second pass: only waves holding a flagged point do any work
@@ -1065,8 +1068,8 @@ k_eval_mixed (MixedArgs a)
 
       if (p < a.m)
       {
-        a.partM[(size_t) blockIdx.y * a.m + p] = Mb[q];
-        a.partS[(size_t) blockIdx.y * a.m + p] = S64[q];
+        a.partM[(size_t) by * a.m + p] = Mb[q];
+        a.partS[(size_t) by * a.m + p] = S64[q];
       }
     }
 
@@ -1084,37 +1087,83 @@ k_eval_mixed (MixedArgs a)
 
     if (g == 0 && p < a.m)
     {
-      a.partM[(size_t) blockIdx.y * a.m + p] = fmaxf (Mb[t], Mo);
-      a.partS[(size_t) blockIdx.y * a.m + p] = S64[t] + So;
+      a.partM[(size_t) by * a.m + p] = fmaxf (Mb[t], Mo);
+      a.partS[(size_t) by * a.m + p] = S64[t] + So;
     }
   }
 }
 
-// Below 2^-100 the FP32 terms lose range: such points are recomputed against
-// their own largest term.
-constexpr float MIXED_FLOOR = -100.0f;
+template <int DP, int KERNEL, int NPT, bool OFFSET>
+__global__ void __launch_bounds__ (64 * waves_per_block (DP))
+k_eval_mixed (MixedArgs a)
+{
+  eval_mixed_block<DP, KERNEL, NPT, OFFSET> (a, blockIdx.x, blockIdx.y);
+}
 
-// pass 1: log sum = b_ref + ln2 (M0 + log2 S); flag points that underflowed
+// Several evaluations in one launch (sessions with the same DP and kernel
+// type): segment s owns the blocks [block0[s], block0[s+1]) of the flattened
+// grid, laid out as its own (column, split) grid, so every block computes
+// exactly what it computes in the segment's own launch.
+__device__ inline int
+segment_of (const int *__restrict__ block0, int nseg, int b)
+{
+  int lo = 0, hi = nseg - 1;
+
+  while (lo < hi)
+  {
+    const int mid = (lo + hi + 1) / 2;
+
+    if (block0[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+
+  return lo;
+}
+
+template <int DP, int KERNEL, int NPT, bool OFFSET>
+__global__ void __launch_bounds__ (64 * waves_per_block (DP))
+k_eval_mixed_seg (const MixedArgs *__restrict__ segs, const int *__restrict__ block0, int nseg)
+{
+  const int sg      = segment_of (block0, nseg, blockIdx.x);
+  const MixedArgs a = segs[sg];
+  const int pts     = waves_per_block (DP) * NPT * 32;
+  const int cols    = (a.m + pts - 1) / pts;
+  const int local   = blockIdx.x - block0[sg];
+
+  eval_mixed_block<DP, KERNEL, NPT, OFFSET> (a, local % cols, local / cols);
+}
+
+// Below 2^-100 the FP32 terms lose range: such points are recomputed against
+// their own largest term.  Against a pilot reference the largest term may
+// also lie up to 2^96 above it (the FP32 lane sums stay finite).
+constexpr float MIXED_FLOOR = -100.0f;
+constexpr float MIXED_CEIL  = 96.0f;
+
 // Merge of the nsplit partial (max, sum) pairs of every point: a block takes
 // 32 points with 8 lanes each; lane l sums the splits s = l, l+8, ... in
 // order (reads of consecutive points are coalesced), then the 8 lane
 // partials combine in lane order, so the result depends only on nsplit.
+// log sum = b_ref + ln2 (M0 + log2 S), M0 the terms' reference (0 for
+// MERGE_PLAIN); mode is a MixedMerge.
 constexpr int MG_PTS = 32, MG_LANES = 8;
 
-__global__ void __launch_bounds__ (MG_PTS * MG_LANES)
-k_mixed_merge (MixedArgs a, double bref, int second)
+__device__ inline void
+mixed_merge_block (const MixedArgs &a, double bref, int mode, int bx)
 {
   __shared__ float Ms[MG_LANES][MG_PTS];
   __shared__ double Ss[MG_LANES][MG_PTS];
   const int pl  = threadIdx.x % MG_PTS;
   const int sl  = threadIdx.x / MG_PTS;
-  const int p   = blockIdx.x * MG_PTS + pl;
-  const bool on = (p < a.m) && !(second && !a.flag[p]);
+  const int p   = bx * MG_PTS + pl;
+  const bool on = (p < a.m) && !(mode == MERGE_FLAGGED && !a.flag[p]);
+  const int ns  = (mode == MERGE_PILOT) ? a.pnsplit : a.nsplit;
   float Mx      = -INFINITY;
   double S      = 0.0;
 
   if (on)
-    for (int s = sl; s < a.nsplit; s += MG_LANES)
+    for (int s = sl; s < ns; s += MG_LANES)
     {
       Mx = fmaxf (Mx, a.partM[(size_t) s * a.m + p]);
       S += a.partS[(size_t) s * a.m + p];
@@ -1133,20 +1182,49 @@ k_mixed_merge (MixedArgs a, double bref, int second)
     S += Ss[l][pl];
   }
 
-  const float M0 = second ? a.M0[p] : 0.0f;
+  const float Mref = (Mx > -INFINITY) ? Mx : 0.0f;
 
-  if (!second)
+  if (mode == MERGE_PILOT)
   {
-    const int bad = (Mx < MIXED_FLOOR) || !(S > 0.0);
+    a.flag[p] = 1;
+    a.M0[p]   = Mref;
+
+    return;
+  }
+
+  const float M0 = (mode == MERGE_PLAIN) ? 0.0f : a.M0[p];
+
+  if (mode == MERGE_PLAIN || mode == MERGE_CHECK)
+  {
+    const int bad = (Mx - M0 < MIXED_FLOOR) || (Mx - M0 > MIXED_CEIL) || !(S > 0.0) || !(S <= DBL_MAX);
 
     a.flag[p] = bad;
-    a.M0[p]   = (Mx > -INFINITY) ? Mx : 0.0f;
+
+    if (bad || mode == MERGE_PLAIN)
+      a.M0[p] = Mref;
   }
 
   const double lnsum = bref + LN2 * ((double) M0 + log2 (S));
   const double m2lnp = -2.0 * ((lnsum - a.lnnorm_shared) - a.lnh_d);
 
   a.out[p] = a.linear ? exp (-0.5 * m2lnp) : m2lnp;
+}
+
+__global__ void __launch_bounds__ (MG_PTS * MG_LANES)
+k_mixed_merge (MixedArgs a, double bref, int mode)
+{
+  mixed_merge_block (a, bref, mode, blockIdx.x);
+}
+
+// the merges of several evaluations in one launch (segment s owns the point
+// blocks [block0[s], block0[s+1])); each uses its own b_ref
+__global__ void __launch_bounds__ (MG_PTS * MG_LANES)
+k_mixed_merge_seg (const MixedArgs *__restrict__ segs, const int *__restrict__ block0, int nseg, int mode)
+{
+  const int sg      = segment_of (block0, nseg, blockIdx.x);
+  const MixedArgs a = segs[sg];
+
+  mixed_merge_block (a, a.bref, mode, blockIdx.x - block0[sg]);
 }
 
 template <int DP, int KERNEL, bool OFFSET>
@@ -1162,46 +1240,113 @@ eval_t (const MixedArgs &a, hipStream_t s)
   return hipGetLastError ();
 }
 
-template <int DP>
+// Without a pilot: the main pass against b_ref, then the points whose terms
+// underflowed again against their own largest term.  With one: the pilot
+// pass gives every point a reference, the main pass runs against it, and
+// the few points whose largest term lies too far from it run again.
+template <int DP, int KERNEL>
 static hipError_t
-eval_d (const MixedArgs &a, hipStream_t s)
+eval_dk (const MixedArgs &a, hipStream_t s)
 {
-  hipError_t e;
+  hipError_t e = hipSuccess;
+  const bool pilot = (a.pn > 0);
+
+  if (pilot)
+  {
+    MixedArgs ap = a;
+
+    ap.f      = a.pf;
+    ap.n      = a.pn;
+    ap.nsplit = a.pnsplit;
+    e         = eval_t<DP, KERNEL, false> (ap, s);
+
+    if (e == hipSuccess)
+      e = launch_mixed_merge (a, a.bref, MERGE_PILOT, s);
+  }
 
   if (a.ev0)
     (void) hipEventRecord (a.ev0, s);
 
-  if (a.kernel == NCKDE_KERNEL_GAUSS)
-  {
-    e = eval_t<DP, NCKDE_KERNEL_GAUSS, false> (a, s);
+  if (e == hipSuccess)
+    e = pilot ? eval_t<DP, KERNEL, true> (a, s) : eval_t<DP, KERNEL, false> (a, s);
 
-    if (a.ev1)
-      (void) hipEventRecord (a.ev1, s);
-
-    if (e == hipSuccess)
-      e = launch_mixed_merge (a, a.bref, 0, s);
-
-    if (e == hipSuccess)
-      e = eval_t<DP, NCKDE_KERNEL_GAUSS, true> (a, s);
-  }
-  else
-  {
-    e = eval_t<DP, NCKDE_KERNEL_ST, false> (a, s);
-
-    if (a.ev1)
-      (void) hipEventRecord (a.ev1, s);
-
-    if (e == hipSuccess)
-      e = launch_mixed_merge (a, a.bref, 0, s);
-
-    if (e == hipSuccess)
-      e = eval_t<DP, NCKDE_KERNEL_ST, true> (a, s);
-  }
+  if (a.ev1)
+    (void) hipEventRecord (a.ev1, s);
 
   if (e == hipSuccess)
-    e = launch_mixed_merge (a, a.bref, 1, s);
+    e = launch_mixed_merge (a, a.bref, pilot ? MERGE_CHECK : MERGE_PLAIN, s);
+
+  if (e == hipSuccess)
+    e = eval_t<DP, KERNEL, true> (a, s);
+
+  if (e == hipSuccess)
+    e = launch_mixed_merge (a, a.bref, MERGE_FLAGGED, s);
 
   return e;
+}
+
+template <int DP>
+static hipError_t
+eval_d (const MixedArgs &a, hipStream_t s)
+{
+  return (a.kernel == NCKDE_KERNEL_GAUSS) ? eval_dk<DP, NCKDE_KERNEL_GAUSS> (a, s)
+                                          : eval_dk<DP, NCKDE_KERNEL_ST> (a, s);
+}
+
+template <int DP, int KERNEL>
+static hipError_t
+eval_multi_dk (const MixedArgs *dsegs, const int *dblock0, int nseg, int eval_blocks, int merge_blocks,
+               hipStream_t s)
+{
+  constexpr int NPT = tiles_per_wave (DP);
+  const dim3 eb (64 * waves_per_block (DP)), mb (MG_PTS * MG_LANES);
+
+  hipLaunchKernelGGL ((k_eval_mixed_seg<DP, KERNEL, NPT, false>), dim3 (eval_blocks), eb, 0, s, dsegs, dblock0,
+                      nseg);
+  hipLaunchKernelGGL (k_mixed_merge_seg, dim3 (merge_blocks), mb, 0, s, dsegs, dblock0 + nseg + 1, nseg,
+                      (int) MERGE_PLAIN);
+  hipLaunchKernelGGL ((k_eval_mixed_seg<DP, KERNEL, NPT, true>), dim3 (eval_blocks), eb, 0, s, dsegs, dblock0,
+                      nseg);
+  hipLaunchKernelGGL (k_mixed_merge_seg, dim3 (merge_blocks), mb, 0, s, dsegs, dblock0 + nseg + 1, nseg,
+                      (int) MERGE_FLAGGED);
+
+  return hipGetLastError ();
+}
+
+int
+mixed_eval_blocks (const MixedArgs &a)
+{
+  const int pts = mixed_points_per_block (a.f.DP);
+
+  return ((a.m + pts - 1) / pts) * a.nsplit;
+}
+
+int
+mixed_merge_blocks (const MixedArgs &a)
+{
+  return (a.m + MG_PTS - 1) / MG_PTS;
+}
+
+hipError_t
+launch_eval_mixed_multi (int DP, int kernel, const MixedArgs *dsegs, const int *dblock0, int nseg, int eval_blocks,
+                         int merge_blocks, hipStream_t s)
+{
+  const bool st = (kernel == NCKDE_KERNEL_ST);
+
+  switch (DP)
+  {
+    case 4: return st ? eval_multi_dk<4, NCKDE_KERNEL_ST> (dsegs, dblock0, nseg, eval_blocks, merge_blocks, s)
+                      : eval_multi_dk<4, NCKDE_KERNEL_GAUSS> (dsegs, dblock0, nseg, eval_blocks, merge_blocks, s);
+    case 8: return st ? eval_multi_dk<8, NCKDE_KERNEL_ST> (dsegs, dblock0, nseg, eval_blocks, merge_blocks, s)
+                      : eval_multi_dk<8, NCKDE_KERNEL_GAUSS> (dsegs, dblock0, nseg, eval_blocks, merge_blocks, s);
+    case 16: return st ? eval_multi_dk<16, NCKDE_KERNEL_ST> (dsegs, dblock0, nseg, eval_blocks, merge_blocks, s)
+                       : eval_multi_dk<16, NCKDE_KERNEL_GAUSS> (dsegs, dblock0, nseg, eval_blocks, merge_blocks, s);
+    case 32: return st ? eval_multi_dk<32, NCKDE_KERNEL_ST> (dsegs, dblock0, nseg, eval_blocks, merge_blocks, s)
+                       : eval_multi_dk<32, NCKDE_KERNEL_GAUSS> (dsegs, dblock0, nseg, eval_blocks, merge_blocks, s);
+    case 64: return st ? eval_multi_dk<64, NCKDE_KERNEL_ST> (dsegs, dblock0, nseg, eval_blocks, merge_blocks, s)
+                       : eval_multi_dk<64, NCKDE_KERNEL_GAUSS> (dsegs, dblock0, nseg, eval_blocks, merge_blocks, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 int
@@ -1225,10 +1370,10 @@ launch_eval_mixed (const MixedArgs &a, hipStream_t s)
 }
 
 hipError_t
-launch_mixed_merge (const MixedArgs &a, double bref, int second, hipStream_t s)
+launch_mixed_merge (const MixedArgs &a, double bref, int mode, hipStream_t s)
 {
   hipLaunchKernelGGL (k_mixed_merge, dim3 ((a.m + MG_PTS - 1) / MG_PTS), dim3 (MG_PTS * MG_LANES), 0, s, a, bref,
-                      second);
+                      mode);
 
   return hipGetLastError ();
 }

@@ -161,11 +161,32 @@ struct MixedArgs
   double *out;
   int linear;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // when set: recorded around the main evaluation kernel
+  // pilot (pn > 0): the factors of pn kernels spread over the sample, whose
+  // largest term per point is the main pass's per-point reference
+  MixedFactors pf;
+  int pn = 0, pnsplit = 0;
+};
+// what a merge of the split partials does
+enum MixedMerge
+{
+  MERGE_PLAIN   = 0,  // terms against b_ref: results, flags of the underflowed points, their maxima
+  MERGE_FLAGGED = 1,  // flagged points, terms against their maxima: results
+  MERGE_PILOT   = 2,  // pilot pass: every point's largest term becomes its reference, every point flagged
+  MERGE_CHECK   = 3,  // terms against the pilot reference: results; points whose largest term is too far
+                      // above it (or whose sum is not finite) flagged, with their maxima
 };
 int mixed_points_per_block (int DP);
 hipError_t launch_mixed_points (const MixedArgs &a, hipStream_t s);
 hipError_t launch_eval_mixed (const MixedArgs &a, hipStream_t s);
-hipError_t launch_mixed_merge (const MixedArgs &a, double bref, int second, hipStream_t s);
+hipError_t launch_mixed_merge (const MixedArgs &a, double bref, int mode, hipStream_t s);
+// several mixed evals (one DP and kernel type) in one launch per pass: dsegs
+// holds their arguments on the device, dblock0 the first eval block of each
+// (nseg + 1 entries) then the first merge block of each (nseg + 1 entries);
+// the points kernels run before, one per segment
+int mixed_eval_blocks (const MixedArgs &a);
+int mixed_merge_blocks (const MixedArgs &a);
+hipError_t launch_eval_mixed_multi (int DP, int kernel, const MixedArgs *dsegs, const int *dblock0, int nseg,
+                                    int eval_blocks, int merge_blocks, hipStream_t s);
 hipError_t mixed_build (const double *X, int n, int d, int D, int DP, const double *Wp, const double *Ug,
                         const double *b, double bref, double inv_h, int kernel, double nu, double *mu, double *Yc,
                         unsigned long long *ymax_bits, double *ysc_out, _Float16 *A, float *c, float *coef,

@@ -10,9 +10,9 @@ FP64 paths -- it fits nothing (a plain VKDE prepare of a seeded Gaussian
 sample is enough: the timing depends on the sizes only), then times the eval
 for split counts around the default rule (x1/4 ... x8, clamped), each the
 median of five launches after a warm-up, and keeps the fastest.  The table
-it writes is compiled into libnckde.so (rebuild afterwards): an eval uses the
-entry of its precision and dimension nearest in (log2 n, log2 m) within one
-octave, else the default rule.  The split count changes results only in the
+it writes (the fastest count of every case) is compiled into libnckde.so
+(rebuild afterwards): an eval uses the entry of its precision and dimension
+nearest in (log2 n, log2 m) within one octave, else the default rule.  The split count changes results only in the
 last bits (the partial sums are merged in a fixed order), and a given
 (n, m) always gets the same count, so evals stay reproducible.
 """
@@ -52,11 +52,11 @@ def default_nsplit(prec, d, n, m):
         ngroups = (n + kpt - 1) // kpt
         ppb = (2 if DP == 64 else (8 if DP == 16 else 4)) * 32 * (8 if DP == 64 else 4)
         bx = (m + ppb - 1) // ppb
-        ns = (2048 + bx - 1) // bx
+        ns = (8192 + bx - 1) // bx
         return max(1, min(ns, max(1, ngroups // 8))), ngroups
     nblk = (m + 255) // 256
-    ns = (2048 + nblk - 1) // nblk
-    return max(1, min(ns, (n + 63) // 64)), n
+    ns = (16384 + nblk - 1) // nblk
+    return max(1, min(ns, (n + 31) // 32)), n
 
 
 def _time_eval(sd, Xd, outd, prec, stream, reps=5):
@@ -122,12 +122,11 @@ def tune(dims=(4, 8, 16, 32), sizes=(512, 2048, 8192, 20000), precisions=(1, 0),
 
 def write_table(rows, path, device_name):
     lines = [f"// Eval split counts measured by numcosmo_amd/tune.py on {device_name}:",
-             "// {precision, dimension, log2 n, log2 m, split count} (only where the",
-             "// tuned count differs from the default rule by more than 3% in time)."]
+             "// {precision, dimension, log2 n, log2 m, split count}, the fastest count of",
+             "// each tuned case."]
     for r in rows:
-        if r["best"] != r["default"] and r["best_ms"] < 0.97 * r["default_ms"]:
-            lines.append(f"{{{r['prec']}, {r['dim']}, {int(round(math.log2(r['n'])))}, "
-                         f"{int(round(math.log2(r['m'])))}, {r['best']}}},")
+        lines.append(f"{{{r['prec']}, {r['dim']}, {int(round(math.log2(r['n'])))}, "
+                     f"{int(round(math.log2(r['m'])))}, {r['best']}}},")
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
 

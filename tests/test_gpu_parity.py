@@ -282,6 +282,10 @@ MIXED_CASES = [
     (GAUSS, 16, 2000, 0.0),
     (ST, 24, 1200, 5.0),
     (ST, 40, 1300, 5.0),
+    # d > 16: from 1024 kernels on, a pilot pass over 256 of them sets the
+    # main pass's per-point reference; below, the underflowed points run twice
+    (ST, 33, 800, 2.0),
+    (GAUSS, 60, 1100, 0.0),
 ]
 
 

@@ -196,3 +196,32 @@ def test_errors_come_back_on_the_ticket():
         t = srv.submit_eval(sd, np.zeros((4, 3)))     # not prepared
         with pytest.raises(NckdeError):
             srv.wait(t)
+
+
+def test_fused_multi_session_eval_is_bit_identical():
+    """nckde_eval_m2lnp_multi: mixed evals of sessions that share the MFMA
+    dimension and kernel type run as one launch per pass over a segment
+    table; every result equals the session's own eval bit for bit, for
+    groups of one and several sessions, mixed dimensions and kernels, and
+    in FP64 (run one by one)."""
+    import numcosmo_amd as na
+    from numcosmo_amd.ncm_stats_dist import eval_m2lnp_multi
+
+    g = np.random.default_rng(77)
+    specs = [(8, 3.0, 1200), (8, 3.0, 2500), (8, 3.0, 700), (16, 1.0, 1800), (16, 3.0, 900), (4, 0.0, 600),
+             (32, 5.0, 1000)]
+    sds, Xs = [], []
+    for d, nu, n in specs:
+        X = g.standard_normal((n, d))
+        k = na.StatsDistKernelGauss.new(d) if nu == 0.0 else na.StatsDistKernelST.new(d, nu)
+        sd = na.StatsDistVKDE.new(k)
+        sd.add_obs_array(X)
+        sd.prepare()
+        sds.append(sd)
+        Xs.append(g.standard_normal((int(g.integers(500, 9000)), d)))
+    for prec in (1, 0):
+        fused = eval_m2lnp_multi(sds, Xs, prec)
+        for sd, X, got in zip(sds, Xs, fused):
+            assert np.array_equal(got, sd.eval_m2lnp_batch(X, precision=prec))
+    one = eval_m2lnp_multi(sds[:1], Xs[:1], 1)[0]
+    assert np.array_equal(one, sds[0].eval_m2lnp_batch(Xs[0], precision=1))
