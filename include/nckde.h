@@ -400,6 +400,10 @@ int nckde_apes_get_trace_rows (nckde_apes *a, size_t *rows);
  * waiting ahead of it (per-session order is arrival order, so results do not
  * depend on the mix), and the device bytes of the requests running plus its
  * own stay within hbm_budget (a request larger than the budget runs alone).
+ * A worker that takes a mixed-precision eval also takes the waiting mixed
+ * evals of other idle sessions with the same MFMA dimension and kernel type
+ * (up to 16, within the budget) and runs them as one fused launch per pass
+ * (nckde_eval_m2lnp_multi), with results bit-identical to separate runs.
  * A session is an nckde handle; the caller keeps every buffer of a request
  * alive until nckde_server_wait returns for it.
  */
@@ -431,6 +435,7 @@ typedef struct
   double t_end;             /* results in the caller's buffers */
   size_t bytes;             /* device bytes charged at admission */
   char message[256];        /* nckde_last_error of a failing request */
+  unsigned batch;           /* requests that ran in one fused launch with this one (1: alone) */
 } nckde_request_info;
 
 int nckde_server_create (nckde_server **out, unsigned slots, size_t hbm_budget);

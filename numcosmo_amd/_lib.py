@@ -128,7 +128,7 @@ class Request(C.Structure):
 class RequestInfo(C.Structure):
     """nckde_request_info"""
     _fields_ = [("status", C.c_int), ("t_submit", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double),
-                ("bytes", C.c_size_t), ("message", C.c_char * 256)]
+                ("bytes", C.c_size_t), ("message", C.c_char * 256), ("batch", C.c_uint)]
 
 # nckde_m2lnL_batch_fn
 M2LNL_BATCH_FN = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_double), C.c_size_t, C.c_uint, C.POINTER(C.c_double),

@@ -291,7 +291,8 @@ factor_batch (nckde *h, const double *dcov, double *dU, double *dlnnorm, size_t 
   if (which.empty ())
     return NCKDE_OK;
 
-  const size_t per = (6 * (size_t) d * d + 3 * (size_t) d) * sizeof (double) + 64;
+  // the projection's cm, D_S and eigenvectors of each failing matrix
+  const size_t per = 3 * (size_t) d * d * sizeof (double);
   int zero         = 0;
 
   HIPCHK (h->dwhich.ensure (sizeof (int) * which.size ()));
@@ -4017,6 +4018,20 @@ trim_work (nckde *h)
     b->release ();
 
   return NCKDE_OK;
+}
+
+int
+mixed_fuse_key (nckde *h)
+{
+  if (check_handle (h) != NCKDE_OK)
+    return -1;
+
+  std::lock_guard<std::mutex> lk (h->mtx);
+
+  if (!h->prepared || !h->mixed_ready || !h->mixed_b_ready || h->mixed_fallback || h->pilot_n > 0)
+    return -1;
+
+  return nck::mfma_dim (h->d) * 2 + h->kernel;
 }
 
 int

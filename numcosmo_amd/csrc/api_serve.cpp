@@ -6,7 +6,9 @@
 // is one whose device bytes do not fit next to the requests running.  Every
 // session owns a HIP stream, so admitted requests of different sessions run
 // concurrently on the device; a request's results are in the caller's
-// buffers when its ticket completes.
+// buffers when its ticket completes.  A worker that takes a mixed eval also
+// takes the admissible mixed evals of other sessions with the same fusion
+// key and runs them as one fused launch per pass.
 #include "nckde_internal.h"
 
 #include <chrono>
@@ -32,8 +34,12 @@ struct Job
   nckde_request r;
   uint64_t ticket = 0;
   nckde_request_info info;
-  bool done = false;
+  bool done     = false;
+  size_t charge = 0;   // device bytes charged while it runs
 };
+
+// evals fused into one launch at most
+constexpr size_t MAX_FUSED = 16;
 
 // Refit buffers a worker keeps between its prepare_interp requests
 struct WorkerState
@@ -135,6 +141,73 @@ struct nckde_server
     return nullptr;
   }
 
+  // the fusion key of a waiting request (-1: runs alone)
+  static int
+  fuse_key (const Job *j)
+  {
+    if (j->r.op != NCKDE_REQ_EVAL_M2LNP || j->r.precision != NCKDE_PREC_MIXED)
+      return -1;
+
+    return nck::mixed_fuse_key (j->r.h);
+  }
+
+  // moves the waiting mixed evals that can share j's launch into batch
+  // (after j): their sessions idle and not behind an earlier waiting
+  // request, one per session, within the budget; mu held
+  void
+  gather_fused (Job *j, std::vector<Job *> &batch)
+  {
+    batch.assign (1, j);
+
+    const int key = fuse_key (j);
+
+    if (key < 0)
+      return;
+
+    std::unordered_set<const nckde *> blocked = {j->r.h};
+    size_t extra                              = 0;
+
+    for (auto it = pending.begin (); it != pending.end () && batch.size () < MAX_FUSED;)
+    {
+      Job *c         = *it;
+      const nckde *h = c->r.h;
+
+      if (busy.count (h) || blocked.count (h) || fuse_key (c) != key ||
+          in_flight + j->charge + extra + c->info.bytes > budget)
+      {
+        blocked.insert (h);
+        ++it;
+        continue;
+      }
+
+      c->charge = c->info.bytes;
+      extra += c->charge;
+      blocked.insert (h);
+      batch.push_back (c);
+      it = pending.erase (it);
+    }
+  }
+
+  static int
+  execute_fused (const std::vector<Job *> &batch)
+  {
+    const size_t nb = batch.size ();
+    std::vector<nckde *> hs (nb);
+    std::vector<const double *> X (nb);
+    std::vector<size_t> m (nb);
+    std::vector<double *> out (nb);
+
+    for (size_t k = 0; k < nb; ++k)
+    {
+      hs[k]  = batch[k]->r.h;
+      X[k]   = batch[k]->r.X;
+      m[k]   = batch[k]->r.m;
+      out[k] = batch[k]->r.out;
+    }
+
+    return nckde_eval_m2lnp_multi (hs.data (), X.data (), m.data (), nullptr, out.data (), nb, NCKDE_PREC_MIXED);
+  }
+
   static int
   execute (const nckde_request &r, nck::WorkSet &ws)
   {
@@ -204,6 +277,7 @@ struct nckde_server
     (void) hipSetDevice (device);
     std::unique_lock<std::mutex> lk (mu);
     WorkerState &me = *wstate[wid];
+    std::vector<Job *> batch;
 
     while (true)
     {
@@ -218,29 +292,45 @@ struct nckde_server
         continue;
       }
 
-      const size_t c = charge (j, wid);
+      j->charge = charge (j, wid);
+      gather_fused (j, batch);
 
-      busy.insert (j->r.h);
-      in_flight += c;
-      ++running;
-      me.active       = true;
-      j->info.t_start = seconds_since (t0);
+      const double t_start = seconds_since (t0);
+
+      for (Job *b : batch)
+      {
+        busy.insert (b->r.h);
+        in_flight += b->charge;
+        ++running;
+        b->info.t_start = t_start;
+        b->info.batch   = (unsigned) batch.size ();
+      }
+
+      me.active = true;
       lk.unlock ();
 
-      const int st = execute (j->r, me.set);
+      const int st = (batch.size () > 1) ? execute_fused (batch) : execute (j->r, me.set);
+      char msg[sizeof (j->info.message)] = "";
+
+      if (st != NCKDE_OK)
+        snprintf (msg, sizeof (msg), "%s", nckde_last_error ());
 
       lk.lock ();
       me.active = false;
-      j->info.t_end  = seconds_since (t0);
-      j->info.status = st;
 
-      if (st != NCKDE_OK)
-        snprintf (j->info.message, sizeof (j->info.message), "%s", nckde_last_error ());
+      const double t_end = seconds_since (t0);
 
-      busy.erase (j->r.h);
-      in_flight -= c;
-      --running;
-      j->done = true;
+      for (Job *b : batch)
+      {
+        b->info.t_end  = t_end;
+        b->info.status = st;
+        memcpy (b->info.message, msg, sizeof (msg));
+        busy.erase (b->r.h);
+        in_flight -= b->charge;
+        --running;
+        b->done = true;
+      }
+
       cv_done.notify_all ();
       cv_work.notify_all ();
     }

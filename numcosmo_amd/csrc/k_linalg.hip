@@ -131,12 +131,29 @@ k_build_IM (IMArgs a)
   }
 }
 
+// Grid: about 2048 blocks, rounded to whole rounds of the blocks the device
+// holds at once, so that no nearly empty last round doubles a block's time
+// (n_obs = 3200: 2054 blocks were 2 rounds and 6 blocks).
 template <int D, int KERNEL, int KIND>
 static hipError_t
 im_t (const IMArgs &a, hipStream_t s)
 {
-  const int bx = (a.n_obs + 255) / 256;
-  int gy       = (2048 + bx - 1) / bx;
+  static int resident = 0;
+
+  if (resident == 0)
+  {
+    int dev = 0, ncu = 0, nb = 0;
+
+    (void) hipGetDevice (&dev);
+    (void) hipDeviceGetAttribute (&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void) hipOccupancyMaxActiveBlocksPerMultiprocessor (&nb, k_build_IM<D, KERNEL, KIND>, 256, 0);
+    resident = (nb > 0 ? nb : 1) * (ncu > 0 ? ncu : 1);
+  }
+
+  const int bx     = (a.n_obs + 255) / 256;
+  const int total0 = bx * ((2048 + bx - 1) / bx);
+  const int rounds = (total0 + resident / 2) / resident;
+  int gy           = (rounds > 0 ? rounds : 1) * resident / bx;
 
   if (gy > a.n_k)
     gy = a.n_k;

@@ -288,16 +288,32 @@ launch_chol_batched (const double *cov, double *U, int *info, double *lnnorm, in
 
 // --------------------------------------------------------------------------
 // nearPD (ncm_matrix.c:1247-1344) and the diagonal fallback
-// (ncm_stats_dist_vkde.c:337-360) for the matrices whose Cholesky failed.
-// One lane per failing matrix runs the serial algorithm on global scratch:
+// (ncm_stats_dist_vkde.c:337-360) for the matrices whose Cholesky failed:
 // cyclic Jacobi for the symmetric eigenproblem, clamp of the negative
-// eigenvalues to min_pos * DBL_EPSILON, rank-1 re-assembly of the upper
-// triangle, Dykstra correction, diagonal restored, until the Cholesky
-// succeeds or maxiter + 2 projections have run.
+// eigenvalues to min_pos * DBL_EPSILON, re-assembly of the upper triangle,
+// Dykstra correction, diagonal restored, until the Cholesky succeeds or
+// maxiter + 2 projections have run.
+//
+// One wave per failing matrix.  The algorithm is serial over rotations,
+// projections and Cholesky columns, but each step's element updates are
+// independent: a rotation (p, q) changes row k's entries (k, p), (k, q) in
+// its column pass and column k's entries (p, k), (q, k) in its row pass, and
+// a Cholesky column j computes every entry (j, i > j) from columns < j.  Lane
+// k does element k of each pass with the serial code's own expression, so
+// every value is the one the serial algorithm computes (the sums of the
+// re-assembly and of the Cholesky keep their serial order per element; the
+// zero test of the off-diagonal sum only asks whether some square is
+// non-zero).  The Jacobi matrices and the Cholesky work sit in LDS (D = 64:
+// 96 KiB); the projection's cm, D_S and eigenvectors in global scratch.
 // --------------------------------------------------------------------------
+
+// Cholesky (upper, row-major, lda = d) of the LDS matrix A by one wave;
+// returns 0 or the 1-based failing column, whose pivot is left in A
 __device__ int
-dev_cholesky_upper (int d, double *A)
+wave_cholesky_upper (int d, double *A)
 {
+  const int l = threadIdx.x;
+
   for (int j = 0; j < d; j++)
   {
     double s = A[j * d + j];
@@ -309,48 +325,70 @@ dev_cholesky_upper (int d, double *A)
       s = s - ukj * ukj;
     }
 
+    __syncthreads ();
+
     if (!(s > 0.0))
     {
-      A[j * d + j] = s;
+      if (l == 0)
+        A[j * d + j] = s;
+
+      __syncthreads ();
+
       return j + 1;
     }
 
-    s            = sqrt (s);
-    A[j * d + j] = s;
+    s = sqrt (s);
 
-    for (int i = j + 1; i < d; i++)
+    if (l == 0)
+      A[j * d + j] = s;
+
+    if (l > j && l < d)
     {
-      double t = A[j * d + i];
+      double t = A[j * d + l];
 
       for (int k = 0; k < j; k++)
-        t = t - A[k * d + j] * A[k * d + i];
+        t = t - A[k * d + j] * A[k * d + l];
 
-      A[j * d + i] = t / s;
+      A[j * d + l] = t / s;
     }
+
+    __syncthreads ();
   }
 
   return 0;
 }
 
+// eigenvalues (ascending, eva) and eigenvectors (rows of eve, global) of the
+// symmetric matrix whose upper triangle is cm (global); a, v, ord in LDS
 __device__ void
-dev_jacobi_eigen (int d, const double *A, double *eva, double *eve, double *a, double *v, int *ord)
+wave_jacobi_eigen (int d, const double *cm, double *eva, double *eve, double *a, double *v, int *ord)
 {
-  for (int i = 0; i < d; i++)
-    for (int j = 0; j < d; j++)
-    {
-      a[i * d + j] = (j >= i) ? A[i * d + j] : A[j * d + i];
-      v[i * d + j] = (i == j) ? 1.0 : 0.0;
-    }
+  const int l = threadIdx.x;
+  const int dd = d * d;
+
+  for (int e = l; e < dd; e += 64)
+  {
+    const int i = e / d, j = e - (e / d) * d;
+
+    a[e] = (j >= i) ? cm[i * d + j] : cm[j * d + i];
+    v[e] = (i == j) ? 1.0 : 0.0;
+  }
+
+  __syncthreads ();
 
   for (int sweep = 0; sweep < 100; sweep++)
   {
-    double off = 0.0;
+    int nz = 0;
 
-    for (int p = 0; p < d; p++)
-      for (int q = p + 1; q < d; q++)
-        off += a[p * d + q] * a[p * d + q];
+    for (int e = l; e < dd; e += 64)
+    {
+      const int i = e / d, j = e - (e / d) * d;
 
-    if (off == 0.0)
+      if (j > i && a[e] * a[e] != 0.0)
+        nz = 1;
+    }
+
+    if (!__syncthreads_or (nz))
       break;
 
     for (int p = 0; p < d; p++)
@@ -361,10 +399,17 @@ dev_jacobi_eigen (int d, const double *A, double *eva, double *eve, double *a, d
         const double app = a[p * d + p];
         const double aqq = a[q * d + q];
 
+        __syncthreads ();
+
         if (fabs (apq) <= 2.220446049250313e-16 * 1.0e-3 * sqrt (fabs (app * aqq)) || apq == 0.0)
         {
-          a[p * d + q] = 0.0;
-          a[q * d + p] = 0.0;
+          if (l == 0)
+          {
+            a[p * d + q] = 0.0;
+            a[q * d + p] = 0.0;
+          }
+
+          __syncthreads ();
           continue;
         }
 
@@ -373,146 +418,166 @@ dev_jacobi_eigen (int d, const double *A, double *eva, double *eve, double *a, d
         const double c     = 1.0 / sqrt (t * t + 1.0);
         const double s     = t * c;
 
-        for (int k = 0; k < d; k++)
+        if (l < d)
         {
-          const double akp = a[k * d + p];
-          const double akq = a[k * d + q];
+          const double akp = a[l * d + p];
+          const double akq = a[l * d + q];
 
-          a[k * d + p] = c * akp - s * akq;
-          a[k * d + q] = s * akp + c * akq;
+          a[l * d + p] = c * akp - s * akq;
+          a[l * d + q] = s * akp + c * akq;
         }
 
-        for (int k = 0; k < d; k++)
-        {
-          const double apk = a[p * d + k];
-          const double aqk = a[q * d + k];
+        __syncthreads ();
 
-          a[p * d + k] = c * apk - s * aqk;
-          a[q * d + k] = s * apk + c * aqk;
+        if (l < d)
+        {
+          const double apk = a[p * d + l];
+          const double aqk = a[q * d + l];
+
+          a[p * d + l] = c * apk - s * aqk;
+          a[q * d + l] = s * apk + c * aqk;
+
+          const double vkp = v[l * d + p];
+          const double vkq = v[l * d + q];
+
+          v[l * d + p] = c * vkp - s * vkq;
+          v[l * d + q] = s * vkp + c * vkq;
         }
 
-        for (int k = 0; k < d; k++)
-        {
-          const double vkp = v[k * d + p];
-          const double vkq = v[k * d + q];
-
-          v[k * d + p] = c * vkp - s * vkq;
-          v[k * d + q] = s * vkp + c * vkq;
-        }
+        __syncthreads ();
       }
     }
   }
 
-  for (int i = 0; i < d; i++)
-    ord[i] = i;
-
-  for (int i = 1; i < d; i++)
+  if (l == 0)
   {
-    const int oi = ord[i];
-    int j        = i - 1;
+    for (int i = 0; i < d; i++)
+      ord[i] = i;
 
-    while (j >= 0 && a[ord[j] * d + ord[j]] > a[oi * d + oi])
+    for (int i = 1; i < d; i++)
     {
-      ord[j + 1] = ord[j];
-      j--;
+      const int oi = ord[i];
+      int j        = i - 1;
+
+      while (j >= 0 && a[ord[j] * d + ord[j]] > a[oi * d + oi])
+      {
+        ord[j + 1] = ord[j];
+        j--;
+      }
+
+      ord[j + 1] = oi;
     }
-
-    ord[j + 1] = oi;
   }
 
-  for (int i = 0; i < d; i++)
+  __syncthreads ();
+
+  if (l < d)
+    eva[l] = a[ord[l] * d + ord[l]];
+
+  for (int e = l; e < dd; e += 64)
   {
-    eva[i] = a[ord[i] * d + ord[i]];
+    const int i = e / d, j = e - (e / d) * d;
 
-    for (int j = 0; j < d; j++)
-      eve[i * d + j] = v[j * d + ord[i]];
+    eve[e] = v[j * d + ord[i]];
   }
+
+  __syncthreads ();
 }
 
-__global__ void
-k_nearpd_fix (const int *__restrict__ which, int nfix, const double *__restrict__ cov, double *__restrict__ Uout,
+template <int D>
+__global__ void __launch_bounds__ (64)
+k_nearpd_fix (const int *__restrict__ which, const double *__restrict__ cov, double *__restrict__ Uout,
               double *__restrict__ lnnorm, double *__restrict__ scratch, int d, int maxiter, int kernel,
               double lnnorm_const, int *__restrict__ status)
 {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-
-  if (f >= nfix)
-    return;
-
+  __shared__ double a[D * D], v[D * D], R[D * D];
+  __shared__ double eva[D], diag[D];
+  __shared__ int ord[D];
+  const int f     = blockIdx.x;
+  const int l     = threadIdx.x;
   const int mat   = which[f];
-  const size_t dd = (size_t) d * d;
-  double *ws      = scratch + (size_t) f * (6 * dd + 3 * d);
-  double *cm      = ws;
-  double *D_S     = ws + dd;
-  double *R       = ws + 2 * dd;
-  double *eve     = ws + 3 * dd;
-  double *a       = ws + 4 * dd;
-  double *v       = ws + 5 * dd;
-  double *eva     = ws + 6 * dd;
-  double *diag    = eva + d;
-  int *ord        = (int *) (diag + d);
+  const int dd    = d * d;
+  double *cm      = scratch + (size_t) f * 3 * dd;
+  double *D_S     = cm + dd;
+  double *eve     = cm + 2 * dd;
   const double *C = cov + (size_t) mat * dd;
   double *U       = Uout + (size_t) mat * dd;
   int ret         = 0;
   int iter        = 0;
 
-  for (size_t i = 0; i < dd; i++)
+  for (int e = l; e < dd; e += 64)
   {
-    cm[i]  = C[i];
-    D_S[i] = 0.0;
+    cm[e]  = C[e];
+    D_S[e] = 0.0;
   }
 
-  for (int i = 0; i < d; i++)
-    diag[i] = cm[i * d + i];
+  if (l < d)
+    diag[l] = C[l * d + l];
+
+  __syncthreads ();
 
   while (true)
   {
+    for (int e = l; e < dd; e += 64)
+    {
+      cm[e] = cm[e] - D_S[e];
+      R[e]  = cm[e];
+    }
+
+    __syncthreads ();
+
+    wave_jacobi_eigen (d, cm, eva, eve, a, v, ord);
+
     double min_pos_ev = INFINITY;
-
-    for (size_t i = 0; i < dd; i++)
-      cm[i] = cm[i] - D_S[i];
-
-    for (size_t i = 0; i < dd; i++)
-      R[i] = cm[i];
-
-    dev_jacobi_eigen (d, cm, eva, eve, a, v, ord);
 
     for (int i = 0; i < d; i++)
       if (eva[i] > 0.0)
         min_pos_ev = fmin (eva[i], min_pos_ev);
 
-    for (int i = 0; i < d; i++)
+    // eigenvector rows scaled by the square roots of the clamped eigenvalues
+    for (int e = l; e < dd; e += 64)
     {
+      const int i     = e / d;
       const double sc = (eva[i] < 0.0) ? sqrt (min_pos_ev * 2.220446049250313e-16) : sqrt (eva[i]);
 
-      if (eva[i] < 0.0)
-        eva[i] = min_pos_ev * 2.220446049250313e-16;
-
-      for (int j = 0; j < d; j++)
-        eve[i * d + j] = sc * eve[i * d + j];
+      eve[e] = sc * eve[e];
     }
 
-    for (int i = 0; i < d; i++)
-      for (int j = i; j < d; j++)
+    __syncthreads ();
+
+    // upper triangle of E^T E, each entry summed over k in order
+    for (int e = l; e < dd; e += 64)
+    {
+      const int i = e / d, j = e - (e / d) * d;
+
+      if (j >= i)
       {
         double s = 0.0;
 
         for (int k = 0; k < d; k++)
           s = s + eve[k * d + i] * eve[k * d + j];
 
-        cm[i * d + j] = s;
+        cm[e] = s;
       }
+    }
 
-    for (size_t i = 0; i < dd; i++)
-      D_S[i] = cm[i] - R[i];
+    __syncthreads ();
 
-    for (int i = 0; i < d; i++)
-      cm[i * d + i] = diag[i];
+    for (int e = l; e < dd; e += 64)
+    {
+      const int i = e / d, j = e - (e / d) * d;
 
-    for (size_t i = 0; i < dd; i++)
-      R[i] = cm[i];
+      D_S[e] = cm[e] - R[e];
 
-    if ((ret = dev_cholesky_upper (d, R)) == 0)
+      if (i == j)
+        cm[e] = diag[i];
+
+      R[e] = cm[e];
+    }
+
+    __syncthreads ();
+
+    if ((ret = wave_cholesky_upper (d, R)) == 0)
       break;
 
     if (iter > maxiter)
@@ -521,32 +586,52 @@ k_nearpd_fix (const int *__restrict__ which, int nfix, const double *__restrict_
     iter++;
   }
 
-  if (ret == 0)
+  if (ret != 0)
   {
-    for (size_t i = 0; i < dd; i++)
-      U[i] = R[i];
-  }
-  else
-  {
-    for (size_t i = 0; i < dd; i++)
-      U[i] = 0.0;
+    // the diagonal fallback
+    for (int e = l; e < dd; e += 64)
+    {
+      const int i = e / d, j = e - (e / d) * d;
 
-    for (int i = 0; i < d; i++)
-      U[i * d + i] = C[i * d + i];
+      R[e] = (i == j) ? C[e] : 0.0;
+    }
 
-    if (dev_cholesky_upper (d, U) != 0)
+    __syncthreads ();
+
+    if (wave_cholesky_upper (d, R) != 0 && l == 0)
       status[0] = 1;
   }
 
-  lnnorm[mat] = kernel_lnnorm (U, d, d, kernel, lnnorm_const);
+  for (int e = l; e < dd; e += 64)
+    U[e] = R[e];
+
+  if (l == 0)
+    lnnorm[mat] = kernel_lnnorm (R, d, d, kernel, lnnorm_const);
 }
 
 hipError_t
 launch_nearpd_fix (const int *which, int nfix, const double *cov, double *U, double *lnnorm, double *scratch,
                    int d, int maxiter, int kernel, double lnnorm_const, int *status, hipStream_t s)
 {
-  hipLaunchKernelGGL (k_nearpd_fix, dim3 ((nfix + 63) / 64), dim3 (64), 0, s, which, nfix, cov, U, lnnorm,
-                      scratch, d, maxiter, kernel, lnnorm_const, status);
+  if (nfix <= 0)
+    return hipSuccess;
+
+#define NEARPD_T(DM)                                                                                                \
+  hipLaunchKernelGGL (k_nearpd_fix<DM>, dim3 (nfix), dim3 (64), 0, s, which, cov, U, lnnorm, scratch, d, maxiter,  \
+                      kernel, lnnorm_const, status)
+
+  if (d <= 8)
+    NEARPD_T (8);
+  else if (d <= 16)
+    NEARPD_T (16);
+  else if (d <= 32)
+    NEARPD_T (32);
+  else if (d <= 64)
+    NEARPD_T (64);
+  else
+    return hipErrorInvalidValue;
+
+#undef NEARPD_T
 
   return hipGetLastError ();
 }

@@ -87,6 +87,11 @@ struct WorkSet
 // stream first; defined in api.cpp).
 int swap_work (nckde *h, WorkSet &w);
 
+// Key under which the handle's mixed evals can share a fused launch (MFMA
+// dimension and kernel type), or -1 (factors not built yet, FP64 fallback,
+// or a pilot pass); defined in api.cpp.
+int mixed_fuse_key (nckde *h);
+
 // Padded dimension used by the templated per-pair kernels.
 int padded_dim (int d);
 // Padded dimension of the MFMA tile (multiples of 4 up to 64, see k_eval_mixed.hip).
@@ -97,6 +102,8 @@ hipError_t launch_welford_cov (const double *X, int d, int nrows, const int *row
                                int nmats, double *cov, hipStream_t s);
 hipError_t launch_chol_batched (const double *cov, double *U, int *info, double *lnnorm, int d,
                                 int nmats, int kernel, double lnnorm_const, hipStream_t s);
+// nearPD / diagonal fallback of the nfix matrices listed in which (d <= 64),
+// one wave each; scratch holds 3 d^2 doubles per matrix
 hipError_t launch_nearpd_fix (const int *which, int nfix, const double *cov, double *U, double *lnnorm,
                               double *scratch, int d, int maxiter, int kernel, double lnnorm_const,
                               int *status, hipStream_t s);

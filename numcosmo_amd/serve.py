@@ -137,4 +137,4 @@ class Server:
         if st != 0:
             raise NckdeError(st, info.message.decode(errors="replace"))
         return t.result, {"t_submit": info.t_submit, "t_start": info.t_start, "t_end": info.t_end,
-                     "latency": info.t_end - info.t_submit, "bytes": info.bytes}
+                     "latency": info.t_end - info.t_submit, "bytes": info.bytes, "batch": info.batch}

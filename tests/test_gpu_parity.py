@@ -274,6 +274,28 @@ def test_nearpd_path_small_k(gpu, oracle):
     assert np.allclose(sd.eval_m2lnp_batch(Xe), od.eval_m2lnp_batch(Xe), rtol=1e-9)
 
 
+@pytest.mark.parametrize("d,n,local_frac,maxiter", [(12, 200, 0.05, 200), (33, 80, 0.3, 10)])
+def test_nearpd_path_rank_deficient(gpu, oracle, d, n, local_frac, maxiter):
+    """Fewer neighbours than dimensions (k = 10 < 12, k = 24 < 33): every
+    local covariance is singular and runs the nearPD iteration to its end
+    (one wave per matrix, 16- and 64-wide LDS forms).  The factors equal the
+    oracle's serial algorithm entry for entry."""
+    _say(f"[nearPD] d={d} n={n} local_frac={local_frac} maxiter={maxiter}")
+    g = np.random.default_rng(4400 + d)
+    X = synth.Gaussian(d, g).draw(g, n)
+    sd, od = _pair(gpu, oracle, 1, ST, d, X, 3.0, local_frac=local_frac, nearPD_maxiter=maxiter)
+    sd.prepare()
+    od.prepare()
+    worst = 0.0
+    for i in range(n):
+        Ug, Uo = np.triu(sd.peek_cov_decomp(i)), np.triu(od.get_cov_decomp(i))
+        worst = max(worst, float(np.max(np.abs(Ug - Uo) / np.maximum(np.abs(Uo), 1e-300))))
+    _say(f"   factor max rel diff {worst:.3e}")
+    assert worst <= 1e-12
+    Xe = synth.Gaussian(d, g).draw(g, 50)
+    assert np.allclose(sd.eval_m2lnp_batch(Xe), od.eval_m2lnp_batch(Xe), rtol=1e-9)
+
+
 MIXED_CASES = [
     # kernel, d (padded MFMA dim 4, 8, 16, 32, 64), n, nu
     (GAUSS, 3, 700, 0.0),

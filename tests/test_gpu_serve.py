@@ -225,3 +225,46 @@ def test_fused_multi_session_eval_is_bit_identical():
             assert np.array_equal(got, sd.eval_m2lnp_batch(X, precision=prec))
     one = eval_m2lnp_multi(sds[:1], Xs[:1], 1)[0]
     assert np.array_equal(one, sds[0].eval_m2lnp_batch(Xs[0], precision=1))
+
+
+def test_server_fuses_waiting_evals():
+    """A worker that takes a mixed eval also takes the waiting mixed evals
+    of other idle sessions with the same MFMA dimension and kernel type and
+    runs them as one fused launch per pass: with one slot held by a refit,
+    the evals queued behind it run together, and each result equals the
+    session's own eval bit for bit.  An FP64 eval never joins a batch."""
+    import numcosmo_amd as na
+    from numcosmo_amd.serve import Server
+
+    g = np.random.default_rng(91)
+    sds, Xs, alone = [], [], []
+    for s in range(6):
+        n = int(g.integers(600, 1200))
+        sd = na.StatsDistVKDE.new(na.StatsDistKernelST.new(8, 3.0))
+        sd.add_obs_array(g.standard_normal((n, 8)))
+        sd.prepare()
+        X = g.standard_normal((int(g.integers(2000, 6000)), 8))
+        alone.append(sd.eval_m2lnp_batch(X, precision=1))      # also builds the mixed factors
+        sds.append(sd)
+        Xs.append(X)
+    X64 = g.standard_normal((1000, 8))
+    alone64 = sds[0].eval_m2lnp_batch(X64, precision=0)
+    big = na.StatsDistVKDE.new(na.StatsDistKernelST.new(8, 3.0))
+    Xb = g.standard_normal((3000, 8))
+    big.add_obs_array(Xb)
+    big.prepare()
+    m2b = np.einsum("ij,ij->i", Xb, Xb)
+    with Server(slots=1) as srv:
+        t_big = srv.submit_prepare_interp(big, m2b)
+        t64 = srv.submit_eval(sds[0], X64, precision=0)
+        ts = [srv.submit_eval(sd, X, precision=1) for sd, X in zip(sds, Xs)]
+        srv.wait(t_big)
+        got64, i64 = srv.wait(t64)
+        res = [srv.wait(t) for t in ts]
+    batches = [info["batch"] for _, info in res]
+    _say(f"   fused batch sizes {batches}, FP64 eval batch {i64['batch']}")
+    assert i64["batch"] == 1
+    assert np.array_equal(got64, alone64)
+    assert max(batches) > 1
+    for (got, info), a in zip(res, alone):
+        assert np.array_equal(got, a)
