@@ -176,6 +176,18 @@ struct nckde
   // NNLS refinement vectors, the symmetric solve's pivot column and the QR
   // rung's work (matrix, row map)
   DBuf dref, dcolv, dqr, dqri;
+  // pinned host side of the NNLS loop: solution and gradient read-backs (n
+  // doubles each), the active list upload (n ints), solve statistics
+  nck::HBuf hnnls;
+  // flags of the chained triangular sweeps (status word, then the panel
+  // flags; flags_words of them zeroed), the last epoch used, the epoch of a
+  // chained solve whose status is still to be read, and whether chained
+  // solves are off for this handle (after one gave up)
+  DBuf dtsflags;
+  size_t flags_words     = 0;
+  unsigned trsv_epoch    = 0;
+  unsigned trsv_pending  = 0;
+  bool trsv_chain_off    = false;
   // cross-validation work buffers
   DBuf dcv1, dcv2, dorder;
 };
@@ -685,6 +697,92 @@ nnls_refine (nckde *h, int n, int np, double *dxo)
 // are badly spread), else the symmetric solve, else QR on A_P; NORMAL_LU
 // starts at the symmetric solve; QR solves min |1 - A_P x| directly.
 // The solution is scattered into dxo.
+// the pinned NNLS arena: x (n doubles), g (n doubles), P (n ints), 8 stats
+static double *
+nnls_hx (nckde *h)
+{
+  return h->hnnls.as<double> ();
+}
+
+static double *
+nnls_hg (nckde *h)
+{
+  return h->hnnls.as<double> () + h->n_k;
+}
+
+static int *
+nnls_hP (nckde *h)
+{
+  return (int *) (h->hnnls.as<double> () + 2 * h->n_k);
+}
+
+static double *
+nnls_hstat (nckde *h)
+{
+  return h->hnnls.as<double> () + 2 * h->n_k + (h->n_k + 1) / 2;
+}
+
+// slot of the pinned statistics that receives the chained sweeps' status
+constexpr int TRSV_STATUS_SLOT = 6;
+
+// (U^T U) x = b with the factor in dS and b in dbs: the flag-chained sweeps
+// when every panel block fits on the device at once, else one launch per
+// panel.  A chained solve's status is read with the caller's next read-back
+// (trsv_pending holds its epoch).
+static int
+nnls_trsv (nckde *h, int np, hipStream_t s)
+{
+  const int npan = (np + 63) / 64;
+
+  if (h->trsv_chain_off || npan > nck::trsv_chain_max_panels ())
+  {
+    HIPCHK (nck::launch_trsv_pair (h->dS.as<double> (), np, h->dbs.as<double> (), h->dtv.as<double> (), s));
+    return NCKDE_OK;
+  }
+
+  const size_t words = 1 + 2 * (size_t) npan;
+
+  if (words > h->flags_words || h->trsv_epoch == 0xffffffffu)
+  {
+    HIPCHK (h->dtsflags.ensure (sizeof (unsigned) * words));
+    HIPCHK (hipMemsetAsync (h->dtsflags.p, 0, sizeof (unsigned) * words, s));
+    h->flags_words = words;
+    h->trsv_epoch  = 0;
+  }
+
+  const unsigned epoch = ++h->trsv_epoch;
+
+  HIPCHK (nck::launch_trsv_chain (h->dS.as<double> (), np, h->dbs.as<double> (), h->dtv.as<double> (),
+                                  h->dtsflags.as<unsigned> (), epoch, s));
+  HIPCHK (hipMemcpyAsync (nnls_hstat (h) + TRSV_STATUS_SLOT, h->dtsflags.p, sizeof (unsigned), hipMemcpyDeviceToHost,
+                          s));
+  h->trsv_pending = epoch;
+
+  return NCKDE_OK;
+}
+
+// true when the chained solve whose status the last read-back brought gave
+// up (chained solves are then off for the handle)
+static bool
+nnls_trsv_failed (nckde *h)
+{
+  if (h->trsv_pending == 0)
+    return false;
+
+  unsigned stv = 0;
+
+  std::memcpy (&stv, nnls_hstat (h) + TRSV_STATUS_SLOT, sizeof (unsigned));
+
+  const bool failed = (stv == h->trsv_pending);
+
+  h->trsv_pending = 0;
+
+  if (failed)
+    h->trsv_chain_off = true;
+
+  return failed;
+}
+
 static int
 nnls_unconstrained (nckde *h, int n, const std::vector<int> &P, double *dxo)
 {
@@ -705,7 +803,9 @@ nnls_unconstrained (nckde *h, int n, const std::vector<int> &P, double *dxo)
     return NCKDE_OK;
   }
 
-  HIPCHK (hipMemcpyAsync (h->dP.p, P.data (), sizeof (int) * np, hipMemcpyHostToDevice, s));
+  // the previous upload from the pinned list ended at that solve's read-back
+  std::memcpy (nnls_hP (h), P.data (), sizeof (int) * np);
+  HIPCHK (hipMemcpyAsync (h->dP.p, nnls_hP (h), sizeof (int) * np, hipMemcpyHostToDevice, s));
 
   if (h->nnls_umethod == NCKDE_NNLS_NORMAL)
   {
@@ -721,12 +821,13 @@ nnls_unconstrained (nckde *h, int n, const std::vector<int> &P, double *dxo)
     HIPCHK (h->dpd.ensure (sizeof (double) * nck::potrf_diag_doubles (np)));
     HIPCHK (nck::launch_potrf (h->dS.as<double> (), np, h->dscal.as<int> (), h->dpd.as<double> (), s));
     HIPCHK (nck::launch_diag_range (h->dS.as<double> (), np, h->dscal.as<double> () + 1, s));
-    HIPCHK (hipMemcpyAsync (&st, h->dscal.p, sizeof (st), hipMemcpyDeviceToHost, s));
+    HIPCHK (hipMemcpyAsync (nnls_hstat (h), h->dscal.p, sizeof (st), hipMemcpyDeviceToHost, s));
     HIPCHK (hipStreamSynchronize (s));
+    std::memcpy (&st, nnls_hstat (h), sizeof (st));
 
     if (st.info == 0)
     {
-      HIPCHK (nck::launch_trsv_pair (h->dS.as<double> (), np, h->dbs.as<double> (), h->dtv.as<double> (), s));
+      TRY (nnls_trsv (h, np, s));
       h->nnls_rungs[0]++;
 
       if (h->nnls_refine > 0 && st.dmax * st.dmax >= REFINE_PIVOT_RATIO * st.dmin * st.dmin)
@@ -745,8 +846,9 @@ nnls_unconstrained (nckde *h, int n, const std::vector<int> &P, double *dxo)
     HIPCHK (hipMemsetAsync (h->dscal.p, 0, sizeof (int) * 2, s));
     HIPCHK (nck::launch_lu_factor (h->dS.as<double> (), np, h->dbs.as<double> (), h->dcolv.as<double> (),
                                    h->dscal.as<int> (), s));
-    HIPCHK (hipMemcpyAsync (&info, h->dscal.p, sizeof (int), hipMemcpyDeviceToHost, s));
+    HIPCHK (hipMemcpyAsync (nnls_hstat (h), h->dscal.p, sizeof (int), hipMemcpyDeviceToHost, s));
     HIPCHK (hipStreamSynchronize (s));
+    std::memcpy (&info, nnls_hstat (h), sizeof (int));
 
     if (info == 0)
     {
@@ -787,8 +889,15 @@ nnls_feasible (nckde *h, int n, std::vector<char> &inP, double *dxo, std::vector
         P.push_back (a);
 
     TRY (nnls_unconstrained (h, n, P, dxo));
-    HIPCHK (hipMemcpyAsync (x.data (), dxo, sizeof (double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK (hipMemcpyAsync (nnls_hx (h), dxo, sizeof (double) * n, hipMemcpyDeviceToHost, h->stream));
     HIPCHK (hipStreamSynchronize (h->stream));
+
+    // a chained triangular solve that gave up: the same solve again with
+    // one launch per panel
+    if (nnls_trsv_failed (h))
+      continue;
+
+    std::memcpy (x.data (), nnls_hx (h), sizeof (double) * n);
 
     std::vector<int> inv;
 
@@ -826,8 +935,10 @@ nnls_residual (nckde *h, const double *dxv, double *drv, double &rnorm)
   HIPCHK (nck::launch_residual (h->dAT.as<double> (), (int) h->n_k, (int) h->n_obs, dxv, nullptr, drv,
                                 h->dpart.as<double> (), h->dpart.as<double> () + nb, h->drw.as<double> (),
                                 h->stream));
-  HIPCHK (hipMemcpyAsync (&r2, h->dpart.as<double> () + nb, sizeof (double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK (hipMemcpyAsync (nnls_hstat (h), h->dpart.as<double> () + nb, sizeof (double), hipMemcpyDeviceToHost,
+                          h->stream));
   HIPCHK (hipStreamSynchronize (h->stream));
+  r2 = nnls_hstat (h)[0];
   rnorm = std::sqrt (r2);
 
   return NCKDE_OK;
@@ -837,8 +948,9 @@ static int
 nnls_mgrad (nckde *h, const double *drv, std::vector<double> &g)
 {
   HIPCHK (nck::launch_row_dot (h->dAT.as<double> (), (int) h->n_k, (int) h->n_obs, drv, h->dg.as<double> (), h->stream));
-  HIPCHK (hipMemcpyAsync (g.data (), h->dg.p, sizeof (double) * h->n_k, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK (hipMemcpyAsync (nnls_hg (h), h->dg.p, sizeof (double) * h->n_k, hipMemcpyDeviceToHost, h->stream));
   HIPCHK (hipStreamSynchronize (h->stream));
+  std::memcpy (g.data (), nnls_hg (h), sizeof (double) * h->n_k);
 
   return NCKDE_OK;
 }
@@ -871,6 +983,7 @@ nnls_solve (nckde *h)
   HIPCHK (h->dscal.ensure (64));
   HIPCHK (h->dref.ensure (sizeof (double) * 3 * n));
   HIPCHK (h->dcolv.ensure (sizeof (double) * n));
+  HIPCHK (h->hnnls.ensure (sizeof (double) * (2 * (size_t) n + ((size_t) n + 1) / 2 + 8)));
 
   // M = A^T A, b = A^T f (the QR method works on A alone, ncm_nnls.c:795-797)
   if (h->nnls_umethod != NCKDE_NNLS_QR)

@@ -1316,6 +1316,185 @@ launch_trsv_backward (const double *S, int np, double *y, double *x, hipStream_t
   return hipGetLastError ();
 }
 
+// The same two sweeps as one launch each, chained by flags: one 64-lane
+// block per panel, block k taking the k-th panel in sweep order.  A block
+// applies the contributions of the panels before it in sweep order, each
+// as soon as that panel's flag carries this call's epoch, with the order
+// and arithmetic of k_trsv_panel (the solution is the same to the bit),
+// then solves its diagonal block and publishes its part of the solution and
+// its flag.  The factor entries of a contribution are loaded before its
+// flag is awaited.  The host launches the chain only when every block of
+// the grid can be resident at once, so each wait ends; a wait longer than
+// spin_limit polls still ends the kernel, with *status = epoch (no later
+// block of either sweep does anything then), and the host solves again with
+// the panel launches.
+template <int FWD>
+__global__ void __launch_bounds__ (64)
+k_trsv_chain (const double *__restrict__ S, int np, const double *__restrict__ rhs, double *__restrict__ out,
+              unsigned *__restrict__ flags, unsigned epoch, unsigned spin_limit, unsigned *__restrict__ status)
+{
+  __shared__ double T[TS_NB][TS_NB + 1];
+  __shared__ double rdiag[TS_NB];
+  __shared__ double ys[TS_NB];
+  const int lane = threadIdx.x;
+  const int npan = (np + TS_NB - 1) / TS_NB;
+  const int k    = blockIdx.x;
+  const int pan  = FWD ? k : npan - 1 - k;
+  const int p0   = pan * TS_NB;
+  const int pb   = (np - p0 < TS_NB) ? np - p0 : TS_NB;
+
+  if (__hip_atomic_load (status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
+    return;
+
+  for (int e = lane; e < TS_NB * TS_NB; e += 64)
+  {
+    const int j = e >> 6, i = e & 63;
+
+    T[i][j] = (i < pb && j < pb) ? S[(size_t) (p0 + j) * np + p0 + i] : 0.0;
+  }
+
+  const int row = p0 + lane;
+  double r      = (lane < pb) ? rhs[row] : 0.0;
+
+  for (int kk = 0; kk < k; ++kk)
+  {
+    const int q  = FWD ? kk : npan - 1 - kk;
+    const int q0 = q * TS_NB;
+    const int qb = (np - q0 < TS_NB) ? np - q0 : TS_NB;
+    double v[TS_NB];
+
+#pragma unroll
+    for (int c = 0; c < TS_NB; ++c)
+      v[c] = (c < qb && lane < pb) ? S[(size_t) (q0 + c) * np + row] : 0.0;
+
+    unsigned polls = 0;
+    bool dead      = false;
+
+    while (__hip_atomic_load (&flags[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch)
+    {
+      if (++polls > spin_limit || __hip_atomic_load (status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
+      {
+        dead = true;
+        break;
+      }
+
+      __builtin_amdgcn_s_sleep (1);
+    }
+
+    if (dead)
+    {
+      if (lane == 0)
+        __hip_atomic_store (status, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+      return;
+    }
+
+    ys[lane] = (lane < qb) ? out[q0 + lane] : 0.0;
+    __builtin_amdgcn_fence (__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier ();
+    __builtin_amdgcn_fence (__ATOMIC_ACQUIRE, "wavefront");
+
+    double t[4] = {r, 0.0, 0.0, 0.0};
+
+#pragma unroll
+    for (int c0 = 0; c0 < TS_NB; c0 += 16)
+#pragma unroll
+      for (int qq = 0; qq < 16; ++qq)
+        if (c0 < qb)
+          t[qq & 3] = fma (-v[c0 + qq], (c0 + qq < qb) ? ys[c0 + qq] : 0.0, t[qq & 3]);
+
+    r = (t[0] + t[1]) + (t[2] + t[3]);
+    __builtin_amdgcn_wave_barrier ();
+  }
+
+  __syncthreads ();
+
+  const int i = lane;
+  double bi   = r;
+
+  rdiag[i] = (i < pb) ? 1.0 / T[i][i] : 0.0;
+  __builtin_amdgcn_fence (__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier ();
+  __builtin_amdgcn_fence (__ATOMIC_ACQUIRE, "wavefront");
+
+  if (pb == TS_NB)
+  {
+    const double rdl = rdiag[i];
+    double t[TS_NB];
+
+#pragma unroll
+    for (int j = 0; j < TS_NB; ++j)
+      t[j] = (FWD ? (j < i) : (j > i)) ? T[i][j] : 0.0;
+
+#pragma unroll
+    for (int step = 0; step < TS_NB; ++step)
+    {
+      const int j     = FWD ? step : TS_NB - 1 - step;
+      const double yj = bcast_lane (bi * rdl, j);
+
+      bi = (i == j) ? yj : fma (-t[j], yj, bi);
+    }
+  }
+  else
+  {
+    for (int step = 0; step < pb; ++step)
+    {
+      const int j     = FWD ? step : pb - 1 - step;
+      const double bj = bcast_lane (bi, j);
+      const double yj = bj * rdiag[j];
+
+      if (i == j)
+        bi = yj;
+      else if (FWD ? (i > j) : (i < j))
+        bi = fma (-T[i][j], yj, bi);
+    }
+  }
+
+  if (i < pb)
+    out[p0 + i] = bi;
+
+  __hip_atomic_store (&flags[pan], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// panels the chained sweeps take at most: the blocks of k_trsv_chain the
+// device holds at once
+int
+trsv_chain_max_panels ()
+{
+  static int cap = -1;
+
+  if (cap < 0)
+  {
+    int dev = 0, ncu = 0, nb = 0;
+
+    (void) hipGetDevice (&dev);
+    (void) hipDeviceGetAttribute (&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void) hipOccupancyMaxActiveBlocksPerMultiprocessor (&nb, k_trsv_chain<1>, 64, 0);
+
+    int nb0 = 0;
+
+    (void) hipOccupancyMaxActiveBlocksPerMultiprocessor (&nb0, k_trsv_chain<0>, 64, 0);
+    nb  = (nb0 < nb) ? nb0 : nb;
+    cap = (nb > 0 && ncu > 0) ? nb * ncu : 0;
+  }
+
+  return cap;
+}
+
+// flags: status word, then 2 npan panel flags (forward, backward)
+hipError_t
+launch_trsv_chain (const double *S, int np, double *b, double *tmp, unsigned *flags, unsigned epoch, hipStream_t s)
+{
+  const int npan = (np + TS_NB - 1) / TS_NB;
+
+  hipLaunchKernelGGL (k_trsv_chain<1>, dim3 (npan), dim3 (64), 0, s, S, np, b, tmp, flags + 1, epoch, 1u << 22,
+                      flags);
+  hipLaunchKernelGGL (k_trsv_chain<0>, dim3 (npan), dim3 (64), 0, s, S, np, tmp, b, flags + 1 + npan, epoch,
+                      1u << 22, flags);
+
+  return hipGetLastError ();
+}
+
 hipError_t
 launch_mirror_lower (double *S, int np, hipStream_t s)
 {

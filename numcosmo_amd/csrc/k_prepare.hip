@@ -33,14 +33,31 @@ pair_of (int p, int d, int &a, int &b)
 // per (a <= b) entry follows the one-pass West/Welford recurrence over the
 // rows in their insertion order; the per-entry recurrence needs only the
 // running means of columns a and b, which the thread tracks itself.
+//
+// The division R = delta / W_t (W_t = t + 1, the weight after row t) sits on
+// the recurrence's critical path.  The reciprocals y_t = RN(1 / W_t) are
+// formed beside each staged chunk, off that path, and the quotient is
+// rebuilt from them: q = RN(delta y_t), r = delta - q W_t (exact in an FMA),
+// RN(q + r y_t), which is the correctly rounded delta / W_t for a faithful q
+// and a correctly rounded y_t (Markstein's theorem); r = 0 keeps q, and with
+// it the sign of a zero quotient.
 // --------------------------------------------------------------------------
 constexpr int WF_LDS = 4096;   // doubles of LDS per block: 4096 / d rows staged per step
+
+__device__ inline double
+div_by_recip (double a, double b, double y)
+{
+  const double q = a * y;
+  const double r = fma (-q, b, a);
+
+  return (r == 0.0) ? q : fma (r, y, q);
+}
 
 __global__ void __launch_bounds__ (256)
 k_welford_cov (const double *__restrict__ X, int d, int nrows, const int *__restrict__ rows,
                int rows_stride, int nmats, double *__restrict__ cov_out)
 {
-  extern __shared__ double xs[];   // (WF_LDS / d) rows x d
+  extern __shared__ double xs[];   // (WF_LDS / d) rows x d, then their reciprocal weights
   const int npairs = d * (d + 1) / 2;
   const long mat   = blockIdx.y + (long) blockIdx.z * gridDim.y;
   const int p      = blockIdx.x * blockDim.x + threadIdx.x;
@@ -64,6 +81,8 @@ k_welford_cov (const double *__restrict__ X, int d, int nrows, const int *__rest
   {
     const int cnt = (nrows - t0 < chunk) ? nrows - t0 : chunk;
 
+    double *rc = xs + chunk * d;
+
     for (int e = threadIdx.x; e < cnt * d; e += blockDim.x)
     {
       const int t    = e / d, col = e - t * d;
@@ -71,6 +90,9 @@ k_welford_cov (const double *__restrict__ X, int d, int nrows, const int *__rest
 
       xs[e] = X[r * d + col];
     }
+
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x)
+      rc[t] = 1.0 / (double) (t0 + t + 1);
 
     __syncthreads ();
 
@@ -80,8 +102,9 @@ k_welford_cov (const double *__restrict__ X, int d, int nrows, const int *__rest
         const double xa        = xs[t * d + a];
         const double xb        = xs[t * d + b];
         const double curweight = weight + w;
+        const double y         = rc[t];
         const double delta_a   = xa - mean_a;
-        const double R_a       = delta_a * w / curweight;
+        const double R_a       = div_by_recip (delta_a * w, curweight, y);
 
         if (a == b)
         {
@@ -95,7 +118,7 @@ k_welford_cov (const double *__restrict__ X, int d, int nrows, const int *__rest
           const double mean_a_new = mean_a + R_a;
           const double dC         = w * (xa - mean_a_new) * (xb - mean_b);
           const double delta_b    = xb - mean_b;
-          const double R_b        = delta_b * w / curweight;
+          const double R_b        = div_by_recip (delta_b * w, curweight, y);
 
           acc    = acc + dC;
           mean_a = mean_a_new;
@@ -130,8 +153,8 @@ launch_welford_cov (const double *X, int d, int nrows, const int *rows, int rows
   const int gz     = (nmats + gy - 1) / gy;
   dim3 grid ((npairs + bs - 1) / bs, gy, gz);
 
-  hipLaunchKernelGGL (k_welford_cov, grid, dim3 (bs), sizeof (double) * (WF_LDS / d) * d, s, X, d, nrows, rows,
-                      rows_stride, nmats, cov);
+  hipLaunchKernelGGL (k_welford_cov, grid, dim3 (bs), sizeof (double) * ((WF_LDS / d) * d + WF_LDS / d), s, X, d,
+                      nrows, rows, rows_stride, nmats, cov);
 
   return hipGetLastError ();
 }

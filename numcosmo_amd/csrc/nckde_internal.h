@@ -52,6 +52,53 @@ struct DBuf
   template <class T> T *as () const { return static_cast<T *> (p); }
 };
 
+// Pinned host buffer (grows, never shrinks) for the read-backs and uploads
+// of host-driven loops: copies from and to it are DMA transfers, without the
+// staging copy of pageable memory.
+struct HBuf
+{
+  void *p      = nullptr;
+  size_t bytes = 0;
+
+  HBuf () = default;
+  HBuf (const HBuf &) = delete;
+  HBuf &operator= (const HBuf &) = delete;
+  ~HBuf () { release (); }
+
+  hipError_t
+  ensure (size_t nbytes)
+  {
+    if (nbytes <= bytes && p != nullptr)
+      return hipSuccess;
+
+    release ();
+
+    const hipError_t e = hipHostMalloc (&p, nbytes ? nbytes : 8, hipHostMallocDefault);
+
+    if (e != hipSuccess)
+    {
+      p = nullptr;
+      return e;
+    }
+
+    bytes = nbytes;
+
+    return hipSuccess;
+  }
+
+  void
+  release ()
+  {
+    if (p)
+      (void) hipHostFree (p);
+
+    p     = nullptr;
+    bytes = 0;
+  }
+
+  template <class T> T *as () const { return static_cast<T *> (p); }
+};
+
 // Frees a handle's transient prepare buffers (defined in api.cpp).
 int trim_work (nckde *h);
 
@@ -228,6 +275,13 @@ hipError_t launch_gather_sym (const double *M, int n, const int *P, int np, doub
 size_t potrf_diag_doubles (int np);
 hipError_t launch_potrf (double *S, int np, int *info, double *diagbuf, hipStream_t s);
 hipError_t launch_trsv_pair (const double *S, int np, double *b, double *tmp, hipStream_t s);
+// the same solve as two flag-chained launches (np / 64 panels at most
+// trsv_chain_max_panels ()); flags holds 1 + 2 ceil(np / 64) words, zero or
+// from earlier epochs; flags[0] == epoch afterwards means the chain gave up
+// and b is not the solution
+int trsv_chain_max_panels ();
+hipError_t launch_trsv_chain (const double *S, int np, double *b, double *tmp, unsigned *flags, unsigned epoch,
+                              hipStream_t s);
 hipError_t launch_lu_solve (double *S, int np, double *b, int *info, hipStream_t s);
 hipError_t launch_trsv_backward (const double *S, int np, double *y, double *x, hipStream_t s);
 hipError_t launch_mirror_lower (double *S, int np, hipStream_t s);
