@@ -772,8 +772,8 @@ factor_diag_block (const double *__restrict__ S, int np, int kb, int nb, double 
 }
 
 // Panel step for a panel narrower than 64 (the last one when np is not a
-// multiple of 64; full panels go through k_potrf_diag64 and
-// k_trsm_panel64): every block factors the diagonal block (redundantly, from
+// multiple of 64; full panels go through k_potrf_rowpanel): every block
+// factors the diagonal block (redundantly, from
 // the unfactored values, which stay in S until k_copy_diag), then solves
 // U_kk^T x = S[kb:kb+nb][c] for its 64 columns c >= kb+nb in LDS (wave 0, one
 // lane per column, four partial sums per row) and writes them back.  Block
@@ -883,182 +883,126 @@ bcast_lane (double v, int src)
   return __hiloint2double (hi, lo);
 }
 
-// Full panels (nb = 64) take two launches.
-// k_potrf_diag64 (one wave): the diagonal block, one column per lane in
-// registers.  Row j of U is published through LDS (one write per lane, wide
-// broadcast reads; a single wave's LDS operations complete in order, so no
-// barrier) and applied as a rank-one update.  The next pivot is taken as
-// soon as its own column is updated: its readlane, square root and
-// reciprocal (all wave-uniform) then overlap the rest of the update, and the
-// scaled row goes to the other half of a double-buffered LDS row.  Entries
-// below the diagonal take part in the update unmasked: they only ever
-// combine with each other, are never read as factor entries and are not
-// stored.  The factor goes to Ud (upper triangle, row-major 64 x 64) and the
+// A full panel (nb = 64) is one launch: the diagonal block's factor and the
+// solve of the row panel right of it are one right-looking factorisation of
+// the 64 x (64 + cols) row panel, each column held whole by one lane (its
+// 64 entries in registers).  Block b has two waves: wave 0 holds the
+// diagonal block (every block factors it, from the same values with the
+// same arithmetic), wave 1 holds RP_EXTRA = 64 columns of the row panel.
+// Step j: wave 0 takes the pivot from lane j by readlane, forms u_jj =
+// sqrt(pivot) and r_j = 1 / u_jj, scales row j of the diagonal block by
+// r_j and publishes it with r_j in LDS (double-buffered, one barrier per
+// step); every lane of the row panel scales its own row-j entry by r_j; then
+// each lane applies the rank-one update a_i -= u_ji * u_jc to its rows i >
+// j, row j's entries read from LDS all at once.  Entries below the diagonal
+// take part in the diagonal block's updates unmasked: they only ever
+// combine with each other and are never stored.  A non-positive pivot is
+// recorded (the first one), the steps run on, and nothing is stored.  Two
+// waves per block measured fastest (2.0 ms for a 3200-column factor, against
+// 2.3 ms with four waves and 2.8 ms for a one-wave diagonal factor followed
+// by a separate panel solve): the step barrier is cheap between two waves.
+// The factor goes to Ud (upper triangle, row-major 64 x 64) and the
 // reciprocals of its diagonal to rd.
-__global__ void __launch_bounds__ (64)
-k_potrf_diag64 (const double *__restrict__ S, int np, int kb, int *__restrict__ info, double *__restrict__ Ud,
-                double *__restrict__ rd)
+constexpr int RP_EXTRA = 64;
+
+__global__ void __launch_bounds__ (64 + RP_EXTRA)
+k_potrf_rowpanel (double *__restrict__ S, int np, int kb, int *__restrict__ info, double *__restrict__ Ud,
+                  double *__restrict__ rd)
 {
   __shared__ double R[2][CH_NB];
-  const int lane = threadIdx.x;
+  __shared__ double Rr[2];
+  __shared__ int s_fail;
+  const int tid  = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  // wave 0: diagonal block column `lane`; wave 1: row-panel columns
+  const int c    = (w == 0) ? kb + lane : kb + CH_NB + blockIdx.x * RP_EXTRA + (w - 1) * 64 + lane;
+  const bool act = c < np;
+  const int cc   = act ? c : kb;
 
   if (*info != 0)
     return;
 
   double a[CH_NB];
-  double rdl = 0.0;   // lane j keeps 1 / u_jj
+  double rdl = 0.0;   // wave 0, lane j: 1 / u_jj
+  int fail   = 0;
 
 #pragma unroll
   for (int r = 0; r < CH_NB; ++r)
-    a[r] = (r <= lane) ? S[(size_t) (kb + r) * np + kb + lane] : 0.0;
-
-  {
-    const double d0 = bcast_lane (a[0], 0);
-
-    if (!(d0 > 0.0))
-    {
-      if (lane == 0)
-        *info = kb + 1;
-
-      return;
-    }
-
-    const double u0 = sqrt (d0);
-    const double r0 = 1.0 / u0;
-
-    a[0]    = (lane == 0) ? u0 : a[0] * r0;
-    rdl     = (lane == 0) ? r0 : rdl;
-    R[0][lane] = a[0];
-  }
+    a[r] = (w > 0 || r <= lane) ? S[(size_t) (kb + r) * np + cc] : 0.0;
 
 #pragma unroll
-  for (int j = 0; j + 1 < CH_NB; ++j)
+  for (int j = 0; j < CH_NB; ++j)
   {
-    __builtin_amdgcn_fence (__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier ();
-    __builtin_amdgcn_fence (__ATOMIC_ACQUIRE, "wavefront");
+    const int bsel = j & 1;
 
-    // row j (scaled) from LDS, all reads issued together
+    if (w == 0)
+    {
+      const double dn = bcast_lane (a[j], j);
+
+      fail = (fail == 0 && !(dn > 0.0)) ? j + 1 : fail;
+
+      const double un = sqrt (dn);
+      const double rn = 1.0 / un;
+
+      a[j]          = (lane == j) ? un : a[j] * rn;
+      R[bsel][lane] = a[j];
+      rdl           = (lane == j) ? rn : rdl;
+
+      if (lane == j)
+        Rr[bsel] = rn;
+    }
+
+    __syncthreads ();
+
+    const double rn = Rr[bsel];
+
+    if (w > 0)
+      a[j] = a[j] * rn;
+
     double rr[CH_NB];
 
 #pragma unroll
-    for (int i = j + 1; i < CH_NB; ++i)
-      rr[i] = R[j & 1][i];
+    for (int r = j + 1; r < CH_NB; ++r)
+      rr[r] = R[bsel][r];
 
-    __builtin_amdgcn_sched_barrier (0);
-
-    // column j + 1 first: its pivot is needed next
-    a[j + 1]        = fma (-rr[j + 1], a[j], a[j + 1]);
-    const double dn = bcast_lane (a[j + 1], j + 1);
-
-    if (!(dn > 0.0))
-    {
-      if (lane == 0)
-        *info = kb + j + 2;
-
-      return;
-    }
-
-    const double un = sqrt (dn);
-    const double rn = 1.0 / un;
+    const double uj = a[j];
 
 #pragma unroll
-    for (int i = j + 2; i < CH_NB; ++i)
-      a[i] = fma (-rr[i], a[j], a[i]);
-
-    a[j + 1] = (lane == j + 1) ? un : a[j + 1] * rn;
-    rdl      = (lane == j + 1) ? rn : rdl;
-    R[(j + 1) & 1][lane] = a[j + 1];
+    for (int r = j + 1; r < CH_NB; ++r)
+      a[r] = fma (-rr[r], uj, a[r]);
   }
 
-#pragma unroll
-  for (int r = 0; r < CH_NB; ++r)
-    if (r <= lane)
-      Ud[r * CH_NB + lane] = a[r];
-
-  rd[lane] = rdl;
-}
-
-// U_kk^T x = S[kb:kb+64][c] for the columns c of the row panel by forward
-// substitution.  Four lanes share a column, 16 rows each: at step r the
-// lane holding row r scales it by 1/u_rr, a DPP quad broadcast hands x_r to
-// its three partners, and every lane applies x_r to its rows.  U_kk (32 KiB)
-// sits in LDS with zeros on and below the diagonal, so the updates need no
-// masks; the reciprocals of its diagonal sit beside it.  A block covers 64
-// columns.
-constexpr int CH_COLS2 = 64;
-
-template <int QB>
-__device__ inline double
-quad_bcast (double v)
-{
-  constexpr int ctrl = QB * 0x55;   // quad_perm [QB, QB, QB, QB]
-  const int lo       = __builtin_amdgcn_mov_dpp (__double2loint (v), ctrl, 0xF, 0xF, true);
-  const int hi       = __builtin_amdgcn_mov_dpp (__double2hiint (v), ctrl, 0xF, 0xF, true);
-
-  return __hiloint2double (hi, lo);
-}
-
-template <int QB>
-__device__ inline void
-trsm_quad_steps (double (&x)[16], const double *__restrict__ Us, const double *__restrict__ rs, int q)
-{
-#pragma unroll
-  for (int tr = 0; tr < 16; ++tr)
-  {
-    const int r     = QB * 16 + tr;
-    const double xr = quad_bcast<QB> (x[tr] * rs[r]);
-
-    if (q == QB)
-      x[tr] = xr;
-
-    const double *u = Us + r * CH_NB + q * 16;
-
-#pragma unroll
-    for (int t = 0; t < 16; ++t)
-      x[t] = fma (-u[t], xr, x[t]);
-  }
-}
-
-__global__ void __launch_bounds__ (256)
-k_trsm_panel64 (double *__restrict__ S, int np, int kb, const int *__restrict__ info, const double *__restrict__ Ud,
-                const double *__restrict__ rd)
-{
-  __shared__ double Us[CH_NB * CH_NB];
-  __shared__ double rs[CH_NB];
-  const int tid = threadIdx.x;
-  const int q   = tid & 3;
-  const int c   = kb + CH_NB + blockIdx.x * CH_COLS2 + (tid >> 2);
-
-  if (*info != 0)
-    return;
-
-  for (int e = tid; e < CH_NB * CH_NB; e += 256)
-    Us[e] = ((e % CH_NB) > (e / CH_NB)) ? Ud[e] : 0.0;
-
-  if (tid < CH_NB)
-    rs[tid] = rd[tid];
+  if (tid == 0)
+    s_fail = fail;
 
   __syncthreads ();
 
-  // lanes past the last column work on a copy of it and store nothing
-  const bool act = c < np;
-  const int cc   = act ? c : np - 1;
-  double x[16];
+  if (s_fail != 0)
+  {
+    if (blockIdx.x == 0 && tid == 0)
+      *info = kb + s_fail;
 
+    return;
+  }
+
+  if (w == 0)
+  {
+    if (blockIdx.x == 0)
+    {
 #pragma unroll
-  for (int t = 0; t < 16; ++t)
-    x[t] = S[(size_t) (kb + q * 16 + t) * np + cc];
+      for (int r = 0; r < CH_NB; ++r)
+        if (r <= lane)
+          Ud[r * CH_NB + lane] = a[r];
 
-  trsm_quad_steps<0> (x, Us, rs, q);
-  trsm_quad_steps<1> (x, Us, rs, q);
-  trsm_quad_steps<2> (x, Us, rs, q);
-  trsm_quad_steps<3> (x, Us, rs, q);
-
-  if (act)
+      rd[lane] = rdl;
+    }
+  }
+  else if (act)
+  {
 #pragma unroll
-    for (int t = 0; t < 16; ++t)
-      S[(size_t) (kb + q * 16 + t) * np + c] = x[t];
+    for (int r = 0; r < CH_NB; ++r)
+      S[(size_t) (kb + r) * np + c] = a[r];
+  }
 }
 
 // the factored diagonal blocks (upper triangles) into S
@@ -1090,8 +1034,8 @@ k_mirror_lower (double *__restrict__ S, int np)
 }
 
 // Right-looking blocked Cholesky S = U^T U in 64-row panels: per panel, the
-// diagonal factor (one wave), the row-panel solve, then the FP64-MFMA
-// trailing update of the upper triangle.  Factored diagonal blocks wait in
+// row-panel factorisation (diagonal factor and row-panel solve in one
+// launch), then the FP64-MFMA trailing update of the upper triangle.  Factored diagonal blocks wait in
 // diagbuf until the end; *info is the 1-based failing pivot, 0 on success.
 hipError_t
 launch_potrf (double *S, int np, int *info, double *diagbuf, hipStream_t s)
@@ -1107,11 +1051,10 @@ launch_potrf (double *S, int np, int *info, double *diagbuf, hipStream_t s)
 
     if (nb == CH_NB)
     {
-      hipLaunchKernelGGL (k_potrf_diag64, dim3 (1), dim3 (64), 0, s, S, np, kb, info, db, rdall + kb);
+      const int gr = (rest + RP_EXTRA - 1) / RP_EXTRA;
 
-      if (rest > 0)
-        hipLaunchKernelGGL (k_trsm_panel64, dim3 ((rest + CH_COLS2 - 1) / CH_COLS2), dim3 (256), 0, s, S, np, kb,
-                            info, db, rdall + kb);
+      hipLaunchKernelGGL (k_potrf_rowpanel, dim3 (gr > 0 ? gr : 1), dim3 (64 + RP_EXTRA), 0, s, S, np, kb, info, db,
+                          rdall + kb);
     }
     else
     {
