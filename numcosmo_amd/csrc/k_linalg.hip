@@ -23,9 +23,13 @@ namespace nck
 // --------------------------------------------------------------------------
 constexpr int IM_KB = 256;
 
+// The kernel factors come through scalar loads only when the compiler knows
+// that the stores into AT cannot reach them: hence the separate __restrict__
+// arguments (with the pointers inside IMArgs the loop loaded every factor
+// entry with vector loads and waited on each, 3x slower).
 template <int D, int KERNEL, int KIND>
 __global__ void __launch_bounds__ (256)
-k_build_IM (IMArgs a)
+k_build_IM (IMArgs a, const double *__restrict__ Wp, const double *__restrict__ Th, double *__restrict__ AT)
 {
   __shared__ double kscale[IM_KB];
   const int j     = blockIdx.x * 256 + threadIdx.x;
@@ -77,13 +81,13 @@ k_build_IM (IMArgs a)
 
     for (int i = ib; i < ie; ++i)
     {
-      const double *th = a.Th + (size_t) i * D;
+      const double *th = Th + (size_t) i * D;
       double chi2      = 0.0;
       double scale;
 
       if (KIND == NCKDE_VKDE)
       {
-        const double *W = a.Wp + (size_t) i * (D * (D + 1) / 2);
+        const double *W = Wp + (size_t) i * (D * (D + 1) / 2);
         double delta[D];
 
 #pragma unroll
@@ -126,7 +130,7 @@ k_build_IM (IMArgs a)
       else
         K = exp (kappa * log1p (chi2 * inv_nu));
 
-      a.AT[(size_t) i * a.n_obs + j] = (K * scale) * rowscale;
+      AT[(size_t) i * a.n_obs + j] = (K * scale) * rowscale;
     }
   }
 }
@@ -161,7 +165,7 @@ im_t (const IMArgs &a, hipStream_t s)
   if (gy < 1)
     gy = 1;
 
-  hipLaunchKernelGGL ((k_build_IM<D, KERNEL, KIND>), dim3 (bx, gy), dim3 (256), 0, s, a);
+  hipLaunchKernelGGL ((k_build_IM<D, KERNEL, KIND>), dim3 (bx, gy), dim3 (256), 0, s, a, a.Wp, a.Th, a.AT);
 
   return hipGetLastError ();
 }
@@ -551,18 +555,31 @@ launch_row_dot (const double *AT, int n, int K, const double *v, double *out, hi
 }
 
 // r = f - A x (f == NULL: ones) and |r|^2.  Pass 1: block (jb, c) sums the
-// kernels of chunk c (RES_CH kernels, inactive ones skipped: x is uniform
+// kernels of chunk c (res_chunk kernels, inactive ones skipped: x is uniform
 // over the block) for 256 observations; pass 2 adds the chunks in a fixed
 // order and forms per-block partial sums of r^2; a last pass adds those.
-constexpr int RES_CH = 512;
+// Chunks: about 2048 blocks in all, at most 4096 chunks (n = 3200: 21
+// kernels per chunk; 512-kernel chunks left 91 blocks whose threads each ran
+// a 512-step serial chain, 188 us).
+static int
+res_chunk (int nk, int nobs)
+{
+  const int nb = (nobs + 255) / 256;
+  int nch      = (2048 + nb - 1) / nb;
+
+  nch = (nch < 1) ? 1 : ((nch > 4096) ? 4096 : nch);
+  nch = (nch > nk) ? nk : nch;
+
+  return (nk + nch - 1) / (nch > 0 ? nch : 1);
+}
 
 __global__ void __launch_bounds__ (256)
-k_residual_part (const double *__restrict__ AT, int nk, int nobs, const double *__restrict__ x,
+k_residual_part (const double *__restrict__ AT, int nk, int nobs, const double *__restrict__ x, int ch,
                  double *__restrict__ part)
 {
   const int j  = blockIdx.x * 256 + threadIdx.x;
-  const int a0 = blockIdx.y * RES_CH;
-  const int a1 = (a0 + RES_CH < nk) ? a0 + RES_CH : nk;
+  const int a0 = blockIdx.y * ch;
+  const int a1 = (a0 + ch < nk) ? a0 + ch : nk;
   double acc   = 0.0;
 
   if (j < nobs)
@@ -629,7 +646,9 @@ k_sum_partials (const double *__restrict__ partial, int np, double *__restrict__
 size_t
 residual_work_doubles (int nk, int nobs)
 {
-  return (size_t) ((nk + RES_CH - 1) / RES_CH) * nobs;
+  const int ch = res_chunk (nk, nobs);
+
+  return (size_t) ((nk + ch - 1) / ch) * nobs;
 }
 
 hipError_t
@@ -637,9 +656,10 @@ launch_residual (const double *AT, int nk, int nobs, const double *x, const doub
                  double *rnorm2, double *work, hipStream_t s)
 {
   const int nb  = (nobs + 255) / 256;
-  const int nch = (nk + RES_CH - 1) / RES_CH;
+  const int ch  = res_chunk (nk, nobs);
+  const int nch = (nk + ch - 1) / ch;
 
-  hipLaunchKernelGGL (k_residual_part, dim3 (nb, nch), dim3 (256), 0, s, AT, nk, nobs, x, work);
+  hipLaunchKernelGGL (k_residual_part, dim3 (nb, nch), dim3 (256), 0, s, AT, nk, nobs, x, ch, work);
   hipLaunchKernelGGL (k_residual, dim3 (nb), dim3 (256), 0, s, work, nch, nobs, f, r, partial);
   hipLaunchKernelGGL (k_sum_partials, dim3 (1), dim3 (64), 0, s, partial, nb, rnorm2);
 

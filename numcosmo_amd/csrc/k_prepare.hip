@@ -13,21 +13,6 @@
 namespace nck
 {
 
-__device__ inline void
-pair_of (int p, int d, int &a, int &b)
-{
-  int aa = 0, rem = p;
-
-  while (rem >= d - aa)
-  {
-    rem -= d - aa;
-    ++aa;
-  }
-
-  a = aa;
-  b = aa + rem;
-}
-
 // --------------------------------------------------------------------------
 // NcmStatsVec covariance (ncm_stats_vec.c:510-565, :2376-2399).  One thread
 // per (a <= b) entry follows the one-pass West/Welford recurrence over the
@@ -53,23 +38,51 @@ div_by_recip (double a, double b, double y)
   return (r == 0.0) ? q : fma (r, y, q);
 }
 
+// Thread p of a matrix: the diagonal entries (p < d) take the first d
+// threads, the off-diagonal pairs a < b start at the next multiple of 64, so
+// that no wave runs both recurrences.
+__device__ inline bool
+welford_pair (int p, int d, int &a, int &b)
+{
+  const int dpad = (d + 63) & ~63;
+
+  if (p < d)
+  {
+    a = b = p;
+    return true;
+  }
+
+  int q = p - dpad;
+
+  if (q < 0 || q >= d * (d - 1) / 2)
+    return false;
+
+  a = 0;
+
+  while (q >= d - 1 - a)
+  {
+    q -= d - 1 - a;
+    ++a;
+  }
+
+  b = a + 1 + q;
+
+  return true;
+}
+
 __global__ void __launch_bounds__ (256)
 k_welford_cov (const double *__restrict__ X, int d, int nrows, const int *__restrict__ rows,
                int rows_stride, int nmats, double *__restrict__ cov_out)
 {
   extern __shared__ double xs[];   // (WF_LDS / d) rows x d, then their reciprocal weights
-  const int npairs = d * (d + 1) / 2;
   const long mat   = blockIdx.y + (long) blockIdx.z * gridDim.y;
   const int p      = blockIdx.x * blockDim.x + threadIdx.x;
 
   if (mat >= nmats)
     return;
 
-  const bool active = p < npairs;
   int a = 0, b = 0;
-
-  if (active)
-    pair_of (p, d, a, b);
+  const bool active = welford_pair (p, d, a, b);
 
   const int *rw  = rows ? rows + mat * (size_t) rows_stride : nullptr;
   const double w = 1.0;
@@ -147,11 +160,11 @@ hipError_t
 launch_welford_cov (const double *X, int d, int nrows, const int *rows, int rows_stride, int nmats,
                     double *cov, hipStream_t s)
 {
-  const int npairs = d * (d + 1) / 2;
+  const int nthr   = ((d + 63) & ~63) + d * (d - 1) / 2;
   const int bs     = 256;
   const int gy     = nmats < 32768 ? nmats : 32768;
   const int gz     = (nmats + gy - 1) / gy;
-  dim3 grid ((npairs + bs - 1) / bs, gy, gz);
+  dim3 grid ((nthr + bs - 1) / bs, gy, gz);
 
   hipLaunchKernelGGL (k_welford_cov, grid, dim3 (bs), sizeof (double) * ((WF_LDS / d) * d + WF_LDS / d), s, X, d,
                       nrows, rows, rows_stride, nmats, cov);
@@ -792,9 +805,18 @@ k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int n
 
   const int q = q0 + qb;
 
-  // 1. distances (coordinates summed in order, as the global-memory kernel);
-  //    the extremes of the positive keys and the number of zero keys (the
-  //    query itself and its duplicates)
+  // 1. distances (coordinates summed in order); the extremes of the positive
+  //    keys and the number of zero keys (the query itself and its
+  //    duplicates).  The query's coordinates sit in LDS and each point's are
+  //    loaded 16 at a time, all in flight together (a load-and-wait per
+  //    coordinate made this phase latency-bound).
+  __shared__ double qs[64];
+
+  if (tid < d)
+    qs[tid] = Yt[(size_t) tid * n + q];
+
+  __syncthreads ();
+
   uint64_t kmin = ~(uint64_t) 0, kmax = 0;
   unsigned nzero = 0;
 
@@ -802,11 +824,22 @@ k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int n
   {
     double dist = 0.0;
 
-    for (int l = 0; l < d; l++)
+    for (int l0 = 0; l0 < d; l0 += 16)
     {
-      const double t = Yt[(size_t) l * n + j] - Yt[(size_t) l * n + q];
+      double yv[16];
 
-      dist = dist + t * t;
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        yv[u] = (l0 + u < d) ? Yt[(size_t) (l0 + u) * n + j] : 0.0;
+
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (l0 + u < d)
+        {
+          const double t = yv[u] - qs[l0 + u];
+
+          dist = dist + t * t;
+        }
     }
 
     const uint64_t key = (uint64_t) __double_as_longlong (dist);
