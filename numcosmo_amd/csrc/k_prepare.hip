@@ -109,38 +109,60 @@ k_welford_cov (const double *__restrict__ X, int d, int nrows, const int *__rest
 
     __syncthreads ();
 
-    if (active)
-      for (int t = 0; t < cnt; ++t)
+    // one row of the recurrence
+    auto step = [&] (double xa, double xb, double y) {
+      const double curweight = weight + w;
+      const double delta_a   = xa - mean_a;
+      const double R_a       = div_by_recip (delta_a * w, curweight, y);
+
+      if (a == b)
       {
-        const double xa        = xs[t * d + a];
-        const double xb        = xs[t * d + b];
-        const double curweight = weight + w;
-        const double y         = rc[t];
-        const double delta_a   = xa - mean_a;
-        const double R_a       = div_by_recip (delta_a * w, curweight, y);
+        const double dvar = weight * delta_a * R_a;
 
-        if (a == b)
-        {
-          const double dvar = weight * delta_a * R_a;
-
-          acc    = acc + dvar;
-          mean_a = mean_a + R_a;
-        }
-        else
-        {
-          const double mean_a_new = mean_a + R_a;
-          const double dC         = w * (xa - mean_a_new) * (xb - mean_b);
-          const double delta_b    = xb - mean_b;
-          const double R_b        = div_by_recip (delta_b * w, curweight, y);
-
-          acc    = acc + dC;
-          mean_a = mean_a_new;
-          mean_b = mean_b + R_b;
-        }
-
-        weight  = curweight;
-        weight2 = weight2 + w * w;
+        acc    = acc + dvar;
+        mean_a = mean_a + R_a;
       }
+      else
+      {
+        const double mean_a_new = mean_a + R_a;
+        const double dC         = w * (xa - mean_a_new) * (xb - mean_b);
+        const double delta_b    = xb - mean_b;
+        const double R_b        = div_by_recip (delta_b * w, curweight, y);
+
+        acc    = acc + dC;
+        mean_a = mean_a_new;
+        mean_b = mean_b + R_b;
+      }
+
+      weight  = curweight;
+      weight2 = weight2 + w * w;
+    };
+
+    if (active)
+    {
+      // rows four at a time, their LDS reads issued before the recurrence
+      int t = 0;
+
+      for (; t + 4 <= cnt; t += 4)
+      {
+        double xa[4], xb[4], y[4];
+
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+          xa[u] = xs[(t + u) * d + a];
+          xb[u] = xs[(t + u) * d + b];
+          y[u]  = rc[t + u];
+        }
+
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          step (xa[u], xb[u], y[u]);
+      }
+
+      for (; t < cnt; ++t)
+        step (xs[t * d + a], xs[t * d + b], rc[t]);
+    }
 
     __syncthreads ();
   }
@@ -791,7 +813,7 @@ k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int n
   __shared__ unsigned hist[KNN_THREADS / 64][256];
   __shared__ uint64_t s_prefix, s_wmin[KNN_THREADS / 64], s_wmax[KNN_THREADS / 64];
   __shared__ unsigned s_wzero[KNN_THREADS / 64];
-  __shared__ int s_kk, s_nlt, s_eq_taken, s_pass0;
+  __shared__ int s_kk, s_nlt, s_eq_taken, s_pass0, s_cnt, s_zero;
   __shared__ int s_scan[KNN_THREADS];
   __shared__ unsigned s_wsum[KNN_THREADS / 64];
 
@@ -892,6 +914,8 @@ k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int n
       s_pass0  = 8;
       s_prefix = 0;
       s_kk     = k;
+      s_cnt    = (int) nz;
+      s_zero   = 1;
     }
     else
     {
@@ -903,6 +927,8 @@ k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int n
       s_pass0  = lead;
       s_prefix = (lead == 0) ? 0 : (lead == 8 ? mn : (mn & (~(uint64_t) 0 << (64 - 8 * lead))));
       s_kk     = k - (int) nz;
+      s_cnt    = n - (int) nz;   // every positive key shares the prefix
+      s_zero   = 0;
     }
   }
 
@@ -971,18 +997,30 @@ k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int n
       if (excl < (unsigned) kk && incl >= (unsigned) kk)
       {
         s_kk     = kk - (int) excl;
+        s_cnt    = (int) h;
         s_prefix = prefix | ((uint64_t) tid << shift);
       }
     }
 
     mask |= (uint64_t) 255 << shift;
     __syncthreads ();
+
+    // the bucket holding rank k is needed whole: the remaining bytes cannot
+    // change which keys are taken
+    if (s_cnt == s_kk)
+      break;
   }
 
+  // the k nearest are the keys below the bucket (key & mask) < T (zero keys
+  // too, on the positive path) and need_eq keys of the bucket itself, all of
+  // it unless the bucket is a single key value with more copies than needed
+  // (ties, taken by smallest index)
   const uint64_t T  = s_prefix;
   const int need_eq = s_kk;
+  const bool zpath  = s_zero != 0;
+  const bool whole  = (s_cnt == need_eq);
 
-  // 3. collect: keys < T, then the need_eq smallest indices with key == T
+  // 3. collect
   if (tid == 0)
   {
     s_nlt      = 0;
@@ -994,8 +1032,10 @@ k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int n
   for (int j = tid; j < n; j += KNN_THREADS)
   {
     const uint64_t key = keys[j];
+    const bool below   = !zpath && (key == 0 || (key & mask) < T);
+    const bool inb     = zpath ? (key == 0) : (key != 0 && (key & mask) == T);
 
-    if (key < T)
+    if (below || (whole && inb))
     {
       const int slot = atomicAdd (&s_nlt, 1);
 
@@ -1011,13 +1051,13 @@ k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int n
 
   const int nlt = s_nlt;
 
-  for (int base = 0; base < n; base += KNN_THREADS)
+  for (int base = 0; base < n && !whole; base += KNN_THREADS)
   {
     if (s_eq_taken >= need_eq)
       break;
 
     const int j    = base + tid;
-    const int flag = (j < n && keys[j] == T) ? 1 : 0;
+    const int flag = (j < n && (zpath ? keys[j] == 0 : (keys[j] != 0 && keys[j] == T))) ? 1 : 0;
 
     s_scan[tid] = flag;
     __syncthreads ();
@@ -1053,7 +1093,7 @@ k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int n
     __syncthreads ();
   }
 
-  if (nlt + need_eq != k && tid == 0)
+  if ((whole ? nlt : nlt + need_eq) != k && tid == 0)
     status[0] = 2;
 
   // 4. bitonic sort of (key, idx) over npow slots
