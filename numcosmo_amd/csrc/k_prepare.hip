@@ -1096,7 +1096,35 @@ k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int n
   if ((whole ? nlt : nlt + need_eq) != k && tid == 0)
     status[0] = 2;
 
-  // 4. bitonic sort of (key, idx) over npow slots
+  int *o = out + (size_t) q * k;
+
+  // 4. (key, idx) order of the k taken.  Up to 512 of them: each one's rank
+  // is the number of pairs below it (the indices are distinct, so the ranks
+  // are a permutation), one barrier instead of a bitonic network's dozens.
+  if (k <= 512)
+  {
+    __syncthreads ();
+
+    for (int i = tid; i < k; i += KNN_THREADS)
+    {
+      const uint64_t ki = skey[i];
+      const int ii      = sidx[i];
+      int r             = 0;
+
+      for (int jj = 0; jj < k; ++jj)
+      {
+        const uint64_t kj = skey[jj];
+
+        r += (kj < ki || (kj == ki && sidx[jj] < ii)) ? 1 : 0;
+      }
+
+      o[r] = ii;
+    }
+
+    return;
+  }
+
+  // bitonic sort of (key, idx) over npow slots
   for (int i = k + tid; i < npow; i += KNN_THREADS)
   {
     skey[i] = ~(uint64_t) 0;
@@ -1134,8 +1162,6 @@ k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int n
     }
   }
 
-  int *o = out + (size_t) q * k;
-
   for (int i = tid; i < k; i += KNN_THREADS)
     o[i] = sidx[i];
 }
@@ -1161,44 +1187,72 @@ launch_knn (const double *Yt, int n, int d, int q0, int nq, int k, uint64_t *key
 // rows and zero-padded to D, and the padded centres.  KDE: padded whitened
 // centres.  Each column of W_i solves U_i^T w = e_c by forward substitution.
 // --------------------------------------------------------------------------
-__global__ void
-k_vkde_factors (const double *__restrict__ Uall, const double *__restrict__ X, int nk, int d, int D,
+// One thread per (kernel, column c): column c of W_i by forward substitution,
+// held in registers (the columns are independent; each entry's products are
+// subtracted in the serial order), then stored with the zero padding; thread
+// c also writes the padded centre coordinate c.
+template <int D>
+__global__ void __launch_bounds__ (256)
+k_vkde_factors (const double *__restrict__ Uall, const double *__restrict__ X, int nk, int d,
                 double *__restrict__ Wp, double *__restrict__ Th)
 {
-  const long i = blockIdx.x * (long) blockDim.x + threadIdx.x;
+  const long gid = blockIdx.x * (long) blockDim.x + threadIdx.x;
+  const long i   = gid / D;
+  const int c    = (int) (gid - i * D);
 
   if (i >= nk)
     return;
 
   const double *U = Uall + i * (size_t) d * d;
   double *W       = Wp + i * (size_t) (D * (D + 1) / 2);
-  double *th      = Th + i * (size_t) D;
+  double w[D];
 
-  for (int l = 0; l < D * (D + 1) / 2; ++l)
-    W[l] = 0.0;
-
-  for (int c = 0; c < d; ++c)
+#pragma unroll
+  for (int l = 0; l < D; ++l)
   {
-    // w_l for l >= c; w_l = 0 for l < c
-    for (int l = c; l < d; ++l)
+    if (c < d && l >= c && l < d)
     {
       double t = (l == c) ? 1.0 : 0.0;
 
-      for (int m = c; m < l; ++m)
-        t = t - U[m * d + l] * W[m * (m + 1) / 2 + c];
+#pragma unroll
+      for (int m = 0; m < l; ++m)
+        if (m >= c)
+          t = t - U[m * d + l] * w[m];
 
-      W[l * (l + 1) / 2 + c] = t / U[l * d + l];
+      w[l] = t / U[l * d + l];
+    }
+    else
+    {
+      w[l] = 0.0;
     }
   }
 
+#pragma unroll
   for (int l = 0; l < D; ++l)
-    th[l] = (l < d) ? X[i * d + l] : 0.0;
+    if (l >= c)
+      W[l * (l + 1) / 2 + c] = w[l];
+
+  Th[i * D + c] = (c < d) ? X[i * d + c] : 0.0;
 }
 
 hipError_t
 launch_vkde_factors (const double *U, const double *X, int nk, int d, int D, double *Wp, double *Th, hipStream_t s)
 {
-  hipLaunchKernelGGL (k_vkde_factors, dim3 ((nk + 127) / 128), dim3 (128), 0, s, U, X, nk, d, D, Wp, Th);
+  const long nt = (long) nk * D;
+  const dim3 grid ((unsigned) ((nt + 255) / 256));
+
+  switch (D)
+  {
+    case 4: hipLaunchKernelGGL (k_vkde_factors<4>, grid, dim3 (256), 0, s, U, X, nk, d, Wp, Th); break;
+    case 8: hipLaunchKernelGGL (k_vkde_factors<8>, grid, dim3 (256), 0, s, U, X, nk, d, Wp, Th); break;
+    case 12: hipLaunchKernelGGL (k_vkde_factors<12>, grid, dim3 (256), 0, s, U, X, nk, d, Wp, Th); break;
+    case 16: hipLaunchKernelGGL (k_vkde_factors<16>, grid, dim3 (256), 0, s, U, X, nk, d, Wp, Th); break;
+    case 24: hipLaunchKernelGGL (k_vkde_factors<24>, grid, dim3 (256), 0, s, U, X, nk, d, Wp, Th); break;
+    case 32: hipLaunchKernelGGL (k_vkde_factors<32>, grid, dim3 (256), 0, s, U, X, nk, d, Wp, Th); break;
+    case 48: hipLaunchKernelGGL (k_vkde_factors<48>, grid, dim3 (256), 0, s, U, X, nk, d, Wp, Th); break;
+    case 64: hipLaunchKernelGGL (k_vkde_factors<64>, grid, dim3 (256), 0, s, U, X, nk, d, Wp, Th); break;
+    default: return hipErrorInvalidValue;
+  }
 
   return hipGetLastError ();
 }
