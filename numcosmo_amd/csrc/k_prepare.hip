@@ -1111,6 +1111,7 @@ k_knn (const double *__restrict__ Yt, int n, int d, int q0, int nq, int k, int n
       const int ii      = sidx[i];
       int r             = 0;
 
+#pragma unroll 8
       for (int jj = 0; jj < k; ++jj)
       {
         const uint64_t kj = skey[jj];
