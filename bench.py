@@ -494,6 +494,34 @@ def pmc_probe(case, steps):
     log("pmc probe done: copy kernel moves 256 MiB read + 256 MiB written per launch")
 
 
+def prep_probe(case, steps):
+    """Short run for kernel traces of prepare_interp: "apes" fits the VKDE of
+    one APES half-ensemble (Cauchy kernel, d=16, n=3200, the seeded mixture)
+    and "d4" the largest served refit (Student-t nu=3, d=4, n=20000, seeded
+    correlated Gaussian), each `steps` times on one handle after a warm-up."""
+    import torch
+
+    import numcosmo_amd as na
+    from numcosmo_amd import synth
+
+    if case == "apes":
+        tgt, X, m2lnp, _ = synth.workload("mixture", 3200, 10, 16, 3)
+        kern = na.StatsDistKernelST.new(16, 1.0)
+    else:
+        tgt, X, m2lnp, _ = synth.workload("gaussian", 20000, 10, 4, 4)
+        kern = na.StatsDistKernelST.new(4, 3.0)
+    sd = na.StatsDistVKDE.new(kern, na.StatsDistCV.NONE)
+    sd.add_obs_array(X)
+    sd.prepare_interp(m2lnp)
+    for r in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sd.prepare_interp(m2lnp)
+        torch.cuda.synchronize()
+        log(f"prep probe {case}: prepare_interp {(time.perf_counter() - t0) * 1e3:.2f} ms, NNLS rungs "
+            f"{sd.get_nnls_rungs()}")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -501,6 +529,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-c2", action="store_true", help="skip the d=8, n=4000 banana VKDE eval")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--prep-probe", choices=["apes", "d4"], default=None,
+                    help="short run for kernel traces of prepare_interp")
     ap.add_argument("--pmc-probe", choices=["d16", "c2"], default=None,
                     help="short run for rocprofv3 --pmc passes")
     ap.add_argument("--apes-steps", type=int, default=200, help="APES ensemble steps (0: skip)")
@@ -524,6 +554,9 @@ def main(argv=None):
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.pmc_probe:
         pmc_probe(args.pmc_probe, max(2, min(args.steps, 5)))
+        return 0
+    if args.prep_probe:
+        prep_probe(args.prep_probe, max(1, min(args.steps, 10)))
         return 0
 
     from oracle import oracle as O

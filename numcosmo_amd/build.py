@@ -50,10 +50,10 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))
-    tmp = OUT + ".tmp"
+    # the offload linker leaves per-target intermediates beside its output:
+    # link in the build folder, then move the library into place
+    tmp = os.path.join(BUILD_DIR, "libnckde.so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
-    # the offload linker leaves per-target intermediates in its working
-    # directory: keep them in the build folder
     r = subprocess.run(cmd, capture_output=True, text=True, cwd=BUILD_DIR)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
