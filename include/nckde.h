@@ -378,6 +378,13 @@ int nckde_apes_init (nckde_apes *a, const double *theta, const double *m2lnL);
 /* ncm_fit_esmcmc_run (ncm_fit_esmcmc.h:78): nsteps ensemble steps; chain
  * (may be NULL) receives nsteps x nwalkers rows of (m2lnL, theta) */
 int nckde_apes_run (nckde_apes *a, unsigned nsteps, nckde_m2lnL_batch_fn like, void *user, double *chain);
+/* The resumed run of ncm_fit_esmcmc_run (ncm_fit_esmcmc.c:2244-2286): as
+ * nckde_apes_run, but the first step moves only walkers [k_first, nwalkers),
+ * the rest of an ensemble whose first k_first walkers a catalog already holds
+ * at that step.  chain gets nsteps x nwalkers rows as in nckde_apes_run; the
+ * first step's rows below k_first are the unmoved walkers. */
+int nckde_apes_run_from (nckde_apes *a, unsigned k_first, unsigned nsteps, nckde_m2lnL_batch_fn like, void *user,
+                         double *chain);
 int nckde_apes_get_state (nckde_apes *a, double *theta, double *m2lnL);
 /* acceptance counters and seconds spent in (prepare, sample, eval, likelihood) */
 int nckde_apes_get_stats (nckde_apes *a, uint64_t *accepted, uint64_t *proposed, double *seconds);

@@ -105,6 +105,7 @@ SIGNATURES = {
     "nckde_apes_set_bounds": (C.c_int, [_h, _dp, _dp]),
     "nckde_apes_init": (C.c_int, [_h, _dp, _dp]),
     "nckde_apes_run": (C.c_int, [_h, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "nckde_apes_run_from": (C.c_int, [_h, C.c_uint, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nckde_apes_get_state": (C.c_int, [_h, _dp, _dp]),
     "nckde_apes_get_stats": (C.c_int, [_h, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _dp]),
     "nckde_apes_set_trace": (C.c_int, [_h, C.c_void_p, _sz]),

@@ -19,6 +19,7 @@ binary table data.  ``APES.write_catalog`` stores a run's chain this way.
 from __future__ import annotations
 
 import os
+import warnings
 from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -156,6 +157,42 @@ class MSetCatalog:
 
     def get_m2lnp_var(self) -> int:
         return self.m2lnp_var
+
+    def get_first_id(self) -> int:
+        return self.first_id
+
+    def get_cur_id(self) -> int:
+        """ncm_mset_catalog_get_cur_id (ncm_mset_catalog.c:2731): id of the
+        last row, first_id - 1 when the catalog is empty."""
+        return self.first_id + len(self._rows) - 1
+
+    def get_row_from_time(self, t: int) -> int:
+        """ncm_mset_catalog_get_row_from_time (ncm_mset_catalog.c:2699): the
+        row holding sample id t."""
+        row = t - self.first_id
+        if t < self.first_id or row >= len(self._rows):
+            raise IndexError(f"sample id {t} is not in the catalog (ids {self.first_id}..{self.get_cur_id()})")
+        return row
+
+    def remove_last_ensemble(self, path: Optional[str] = None) -> None:
+        """ncm_mset_catalog_remove_last_ensemble (ncm_mset_catalog.c:5390-5446):
+        drop the last NCHAINS rows, or the trailing partial ensemble
+        (len % NCHAINS rows) when there is one.  With ``path``, the file
+        there is first renamed to ``path.<k>.bak`` (the first k not in use)
+        and the shortened catalog is written to ``path``."""
+        n = len(self._rows)
+        if n == 0:
+            warnings.warn("MSetCatalog.remove_last_ensemble: called on an empty catalog.")
+            return
+        drop = self.nchains if n % self.nchains == 0 else n % self.nchains
+        del self._rows[n - drop:]
+        if path is not None:
+            k = 0
+            while os.path.exists(f"{path}.{k}.bak"):
+                k += 1
+            if os.path.exists(path):
+                os.rename(path, f"{path}.{k}.bak")
+            self.write(path)
 
     def trim(self, tc: int, thin: int = 1) -> None:
         """ncm_mset_catalog_trim (ncm_mset_catalog.c:5255-5315): drop the

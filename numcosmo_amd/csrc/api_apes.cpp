@@ -565,27 +565,34 @@ nckde_apes_init (nckde_apes *a, const double *theta, const double *m2lnL)
   return NCKDE_OK;
 }
 
-int
-nckde_apes_run (nckde_apes *a, unsigned nsteps, nckde_m2lnL_batch_fn like, void *user, double *chain)
-{
-  if (a == nullptr || like == nullptr || !a->initialised)
-  {
-    set_error ("nckde_apes_run: sampler not initialised or no likelihood");
-    return NCKDE_ESTATE;
-  }
+}  // extern "C"
 
+// nsteps ensemble steps, the first of which moves only walkers
+// [k_first, n): the loop of ncm_fit_esmcmc.c:2244-2286.  A step that starts
+// inside the first half moves [k_first, n/2) with a KDE of the second half
+// and then the whole second half; one that starts inside the second half
+// moves [k_first, n) only.  The jump uniforms are drawn for the walkers that
+// move, in walker order (_ncm_fit_esmcmc_get_jumps (esmcmc, ki, nwalkers)).
+static int
+apes_steps (nckde_apes *a, unsigned k_first, unsigned nsteps, nckde_m2lnL_batch_fn like, void *user,
+            double *chain)
+{
   std::uniform_real_distribution<double> U01 (0.0, 1.0);
   std::vector<double> jumps (a->n);
   const size_t row = 1 + a->d;
 
   for (unsigned s = 0; s < nsteps; ++s)
   {
-    for (unsigned k = 0; k < a->n; ++k)
+    const unsigned k0 = (s == 0) ? k_first : 0;
+
+    for (unsigned k = k0; k < a->n; ++k)
       jumps[k] = U01 (a->rng);
 
     // first half moves with a KDE of the second half, then the reverse
-    TRYA (apes_half (a, 0, a->n2, a->n2, 0, jumps.data (), like, user));
-    TRYA (apes_half (a, a->n2, a->n, 0, 1, jumps.data (), like, user));
+    if (k0 < a->n2)
+      TRYA (apes_half (a, k0, a->n2, a->n2, 0, jumps.data (), like, user));
+
+    TRYA (apes_half (a, std::max (k0, a->n2), a->n, 0, 1, jumps.data (), like, user));
 
     if (chain)
     {
@@ -600,6 +607,39 @@ nckde_apes_run (nckde_apes *a, unsigned nsteps, nckde_m2lnL_batch_fn like, void 
   }
 
   return NCKDE_OK;
+}
+
+extern "C" {
+
+int
+nckde_apes_run (nckde_apes *a, unsigned nsteps, nckde_m2lnL_batch_fn like, void *user, double *chain)
+{
+  if (a == nullptr || like == nullptr || !a->initialised)
+  {
+    set_error ("nckde_apes_run: sampler not initialised or no likelihood");
+    return NCKDE_ESTATE;
+  }
+
+  return apes_steps (a, 0, nsteps, like, user, chain);
+}
+
+int
+nckde_apes_run_from (nckde_apes *a, unsigned k_first, unsigned nsteps, nckde_m2lnL_batch_fn like, void *user,
+                     double *chain)
+{
+  if (a == nullptr || like == nullptr || !a->initialised)
+  {
+    set_error ("nckde_apes_run_from: sampler not initialised or no likelihood");
+    return NCKDE_ESTATE;
+  }
+
+  if (k_first >= a->n)
+  {
+    set_error ("nckde_apes_run_from: first walker %u out of range (%u walkers)", k_first, a->n);
+    return NCKDE_EINVAL;
+  }
+
+  return apes_steps (a, k_first, nsteps, like, user, chain);
 }
 
 int

@@ -96,4 +96,5 @@ def test_apes_entry_points_reject_null_handles():
     assert L.nckde_apes_get_trace_rows(None, ctypes.byref(rows)) == -1
     assert rows.value == 7
     assert L.nckde_apes_run(None, 1, None, None, None) != 0
+    assert L.nckde_apes_run_from(None, 0, 1, None, None, None) != 0
     assert L.nckde_apes_destroy(None) == 0

@@ -179,3 +179,166 @@ def test_apes_moves_match_oracle(oracle, ktype, precision, rtol):
     _say(f"   worst transition rel err {worst:.2e}, {decided} of {steps * nw} decisions checked")
     assert decided >= 0.9 * steps * nw
     assert 0 < T[:, 8].sum() < steps * nw
+
+
+def test_apes_resume_continues_the_run(tmp_path):
+    """A run stopped after 3 steps, written to FITS and resumed to 7
+    ensembles is the uninterrupted 6-step run, row for row: the walkers start
+    from the catalog's last ensemble and the sampler's streams go on."""
+    from numcosmo_amd.catalog import MSetCatalog
+    _say("[apes] resume from a FITS catalog vs the uninterrupted run, dim=2 ST3")
+    _, a = _run(2, 1, 6, seed=41)
+    _, b = _run(2, 1, 3, seed=41)
+    nw = b.nwalkers
+    p = str(tmp_path / "b.fits")
+    b.write_catalog(p)
+    b.resume(p, 7)
+    cat = MSetCatalog.read(p)
+    assert len(cat) == 7 * nw
+    np.testing.assert_array_equal(cat.as_array(), a.get_catalog().as_array())
+    np.testing.assert_array_equal(b.chain, a.chain[3:])
+    # ncm_fit_esmcmc_run: nothing to do once the catalog holds n_total ensembles
+    b.resume(p, 5)
+    assert b.chain is None and len(MSetCatalog.read(p)) == 7 * nw
+
+
+def _apes_small(ktype, seed, lb, ub):
+    from numcosmo_amd.apes import APES, InterpolationKernel
+    g = np.random.default_rng(17)
+    tgt = synth.Gaussian(2, g)
+    s = APES(nwalkers=120, ndim=2, log_prob_batch=lambda th: -0.5 * tgt.m2lnp(th),
+             interpolation_kernel=InterpolationKernel(ktype), over_smooth=0.2, local_fraction=0.2,
+             random_walk_prob=0.25, bounds=(lb, ub), seed=seed)
+    return tgt, s
+
+
+@pytest.mark.parametrize("ki", [25, 73])
+def test_apes_resume_inside_an_ensemble_matches_oracle(oracle, ki):
+    """A catalog that ends ki walkers into an ensemble: the first resumed
+    step moves walkers [ki, nwalkers) only, from each walker's latest row
+    (ncm_fit_esmcmc.c:1855-1880, 2244-2268).  Every move of the two resumed
+    steps is replayed on the oracle as in test_apes_moves_match_oracle."""
+    import math
+    from numcosmo_amd.catalog import MSetCatalog
+    g = np.random.default_rng(17)
+    tgt = synth.Gaussian(2, g)
+    sig = np.sqrt(np.diag(tgt.cov))
+    lb, ub = tgt.mean - 1.2 * sig, tgt.mean + 1.8 * sig
+    _say(f"[apes] resume inside an ensemble, ki={ki}, nw=120, replayed on the oracle")
+    tgt, s1 = _apes_small(0, 5, lb, ub)
+    x0 = tgt.draw(g, 2400)
+    x0 = x0[np.all((x0 >= lb) & (x0 <= ub), axis=1)][:120]
+    s1.run_mcmc(x0, 2)
+    nw, n2, dim = 120, 60, 2
+    full = s1.get_catalog().as_array()
+    cat = MSetCatalog([f"APES:theta_{i}" for i in range(dim)], nchains=nw)
+    cat.add_rows(full[:2 * nw + ki])
+    # each walker's latest row: ensemble 2 below ki, ensemble 1 from ki on
+    theta = np.vstack([full[2 * nw:2 * nw + ki, 1:], full[nw + ki:2 * nw, 1:]])
+    m2 = np.concatenate([full[2 * nw:2 * nw + ki, 0], full[nw + ki:2 * nw, 0]])
+
+    _, s2 = _apes_small(0, 9, lb, ub)
+    s2.resume(cat, 4, trace=True)
+    assert len(cat) == 4 * nw and s2.chain.shape == (2, nw, 1 + dim)
+    np.testing.assert_array_equal(cat.as_array()[:2 * nw + ki], full[:2 * nw + ki])
+    np.testing.assert_array_equal(cat.as_array()[2 * nw + ki:], s2.chain.reshape(-1, 1 + dim)[ki:])
+
+    halves = ([(0, ki, n2)] if ki < n2 else []) + [(1, max(ki, n2), nw), (0, 0, n2), (1, n2, nw)]
+    T = s2.trace
+    assert T.shape[0] == sum(k1 - k0 for _, k0, k1 in halves)
+    pos, worst, decided = 0, 0.0, 0
+    for h, (which, k0, k1) in enumerate(halves):
+        other0 = n2 if which == 0 else 0
+        R = T[pos:pos + k1 - k0]
+        pos += k1 - k0
+        th, ts = R[:, 9:9 + dim], R[:, 9 + dim:]
+        np.testing.assert_array_equal(R[:, 0], which)
+        np.testing.assert_array_equal(R[:, 1], np.arange(k0, k1))
+        np.testing.assert_array_equal(th, theta[k0:k1])
+        np.testing.assert_array_equal(R[:, 3], m2[k0:k1])
+        np.testing.assert_array_equal(R[:, 2], 0.0)
+        od = oracle.StatsDist(oracle.VKDE, oracle.ST, dim, 1.0)
+        od.set_over_smooth(0.2)
+        od.set_local_frac(0.2)
+        od.set_shrink(0.01)
+        od.add_obs_array(theta[other0:other0 + n2])
+        od.prepare_interp(m2[other0:other0 + n2])
+        std = oracle.apes_rw_std(od.get_full_cov())
+        for j in range(k1 - k0):
+            ps = oracle.apes_transition_m2lnp(od, th[j], ts[j], lb, ub, std, 0.25)
+            pc = oracle.apes_transition_m2lnp(od, ts[j], th[j], lb, ub, std, 0.25)
+            e = max(abs(R[j, 5] - ps) / max(abs(ps), 1.0), abs(R[j, 6] - pc) / max(abs(pc), 1.0))
+            worst = max(worst, e)
+            assert e <= 1e-7, (h, j, R[j, 5], ps, R[j, 6], pc)
+            prob = min(math.exp(-0.5 * (R[j, 4] - R[j, 3] + pc - ps)), 1.0)
+            if abs(R[j, 7] - prob) > 1e-4:
+                assert R[j, 8] == float(R[j, 7] < prob), (h, j, R[j, 7], prob)
+                decided += 1
+        acc = R[:, 8] == 1.0
+        theta[k0:k1][acc] = ts[acc]
+        m2[k0:k1][acc] = R[acc, 4]
+        if k1 == nw:
+            step = 0 if h < len(halves) - 2 else 1
+            np.testing.assert_array_equal(s2.chain[step, :, 1:], theta)
+            np.testing.assert_array_equal(s2.chain[step, :, 0], m2)
+    _say(f"   worst transition rel err {worst:.2e}, {decided} of {T.shape[0]} decisions checked")
+    assert decided >= 0.9 * T.shape[0]
+
+
+def test_apes_resume_drops_a_corrupt_last_ensemble(tmp_path):
+    """ncm_fit_esmcmc.c:1883-1909: a last ensemble whose -2 ln L no longer
+    matches the likelihood (relative difference > 1e-3) is removed, the file
+    backed up, and the run resumes from the ensemble before it;
+    skip_check keeps it."""
+    import shutil
+    from numcosmo_amd.catalog import MSetCatalog
+    _say("[apes] resume drops a corrupt last ensemble")
+    _, s = _run(2, 0, 3, seed=51)
+    nw = s.nwalkers
+    p = str(tmp_path / "c.fits")
+    s.write_catalog(p)
+    cat = MSetCatalog.read(p)
+    rows = cat.as_array().copy()
+    rows[-5, 0] *= 1.01
+    bad = MSetCatalog(cat.param_names, nchains=nw)
+    bad.add_rows(rows)
+    bad.write(p)
+    q = str(tmp_path / "skip.fits")
+    shutil.copy(p, q)
+
+    s.resume(p, 6, trace=True)
+    np.testing.assert_array_equal(MSetCatalog.read(p + ".0.bak").as_array(), rows)
+    out = MSetCatalog.read(p).as_array()
+    assert out.shape[0] == 6 * nw
+    np.testing.assert_array_equal(out[:3 * nw], rows[:3 * nw])
+    # the first resumed half-step starts from ensemble 2
+    np.testing.assert_array_equal(s.trace[:nw // 2, 9:11], rows[2 * nw:2 * nw + nw // 2, 1:])
+    assert s.chain.shape[0] == 3
+
+    s.resume(q, 5, skip_check=True)
+    out = MSetCatalog.read(q).as_array()
+    assert out.shape[0] == 5 * nw
+    np.testing.assert_array_equal(out[:4 * nw], rows)
+    assert not (tmp_path / "skip.fits.0.bak").exists()
+
+
+def test_apes_resume_completes_the_first_ensemble():
+    """A catalog with fewer rows than walkers: the missing walkers come from
+    initial_sample, their rows are added, then the run goes on."""
+    from numcosmo_amd.catalog import MSetCatalog
+    _say("[apes] resume from an incomplete first ensemble")
+    g = np.random.default_rng(61)
+    tgt = synth.Gaussian(2, g)
+    from numcosmo_amd.apes import APES
+    s = APES(nwalkers=200, ndim=2, log_prob_batch=lambda th: -0.5 * tgt.m2lnp(th), seed=61)
+    x0 = tgt.draw(g, 200)
+    cat = MSetCatalog(["a", "b"], nchains=200)
+    cat.add_rows(np.hstack([tgt.m2lnp(x0[:70])[:, None], x0[:70]]))
+    with pytest.raises(ValueError, match="initial_sample"):
+        s.resume(cat, 3)
+    s.resume(cat, 3, initial_sample=x0)
+    out = cat.as_array()
+    assert out.shape[0] == 3 * 200
+    np.testing.assert_array_equal(out[:200, 1:], x0)
+    np.testing.assert_allclose(out[:200, 0], tgt.m2lnp(x0), rtol=1e-14)
+    np.testing.assert_array_equal(out[200:], s.chain.reshape(-1, 3))
