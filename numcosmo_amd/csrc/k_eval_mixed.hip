@@ -493,7 +493,11 @@ lse_push (float c2, float4 cf, float kappa, float M0, float &Mb, float &S32)
   else
     lnt = fmaf (kappa, __builtin_amdgcn_logf (fmaf (c2, cf.x, 1.0f)), cf.y);
 
-  Mb = fmaxf (Mb, lnt);
+  // the running maximum as an unsigned minimum of the bit patterns: for
+  // floats <= 0 (every lnt here, and Mb from its -inf start) the larger
+  // value has the smaller pattern (+0 is 0, -inf is 0xff800000).  One
+  // v_min_u32; fmaxf costs two more v_max_f32 that quiet signalling NaNs.
+  Mb = __uint_as_float (min (__float_as_uint (Mb), __float_as_uint (lnt)));
 
   if (OFFSET)
     lnt -= M0;
