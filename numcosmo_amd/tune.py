@@ -5,8 +5,8 @@ Usage (on the GPU the library will run on):
     python -m numcosmo_amd.tune [--out numcosmo_amd/csrc/eval_tune.inc] [--report FILE]
 
 For every case of a grid covering the served sizes -- d in {4, 8, 16, 32},
-n in {512, 2048, 8192, 20000} kernels and m = 10 n points, the mixed and
-FP64 paths -- it fits nothing (a plain VKDE prepare of a seeded Gaussian
+n in {512, 2048, 8192, 20000} kernels and m = 10 n points, plus n = 4000 at
+m = 25 n (1e5 points), the mixed and FP64 paths -- it fits nothing (a plain VKDE prepare of a seeded Gaussian
 sample is enough: the timing depends on the sizes only), then times the eval
 for split counts around the default rule (x1/4 ... x8, clamped), each the
 median of five launches after a warm-up, and keeps the fastest.  The table
@@ -84,7 +84,11 @@ def _time_eval(sd, Xd, outd, prec, stream, reps=5):
     return float(np.median(ts))
 
 
-def tune(dims=(4, 8, 16, 32), sizes=(512, 2048, 8192, 20000), precisions=(1, 0), log=print):
+# (kernels, points) of the tuned cases
+CASES = ((512, 5120), (2048, 20480), (4000, 100000), (8192, 81920), (20000, 200000))
+
+
+def tune(dims=(4, 8, 16, 32), cases=CASES, precisions=(1, 0), log=print):
     import torch
 
     import numcosmo_amd as na
@@ -92,8 +96,7 @@ def tune(dims=(4, 8, 16, 32), sizes=(512, 2048, 8192, 20000), precisions=(1, 0),
     torch.cuda.init()
     rows = []
     for d in dims:
-        for n in sizes:
-            m = 10 * n
+        for n, m in cases:
             g = np.random.default_rng(1000 * d + n)
             X = g.standard_normal((n, d))
             Xe = g.standard_normal((m, d))
@@ -133,9 +136,10 @@ def write_table(rows, path, device_name):
 
 def write_report(rows, path, device_name, seconds):
     out = [f"# Eval split tuning ({device_name}, {seconds:.0f} s)", "",
-           "`python -m numcosmo_amd.tune`: VKDE Student-t nu=3 on a seeded Gaussian sample, m = 10 n, each time the "
-           "median of five event-timed launches after a warm-up.", "",
-           "| precision | d | n | m | default splits | ms | best splits | ms | gain |", "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+           "`python -m numcosmo_amd.tune`: VKDE Student-t nu=3 on a seeded Gaussian sample, m = 10 n (25 n at "
+           "n = 4000), each time the median of five event-timed launches after a warm-up.", "",
+           "| precision | d | n | m | default splits | ms | best splits | ms | gain |",
+           "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
     for r in rows:
         out.append(f"| {'mixed' if r['prec'] == 1 else 'FP64'} | {r['d']} | {r['n']} | {r['m']} | {r['default']} | "
                    f"{r['default_ms']:.3f} | {r['best']} | {r['best_ms']:.3f} | "
