@@ -175,9 +175,9 @@ class APES:
         ensembles (sample ids up to n_total x nwalkers - 1, counted from id 0
         as ncm_fit_esmcmc_run (ncm_fit_esmcmc.c:2012-2034) counts them).
 
-        ``catalog`` is an ``MSetCatalog`` or the path of one in FITS; a path
-        is written back with the new rows at the end, as NcmMSetCatalog keeps
-        its file in sync.  The walkers start where ncm_fit_esmcmc_start_run
+        ``catalog`` is an ``MSetCatalog``, which is extended in place, or the
+        path of one in FITS, which is written back with the new rows at the
+        end, as NcmMSetCatalog keeps its file in sync.  The walkers start where ncm_fit_esmcmc_start_run
         (ncm_fit_esmcmc.c:1808-1910) puts them: each at its latest row, so a
         catalog that ends inside an ensemble resumes with that ensemble's
         remaining walkers (``catalog_state``).  Unless ``skip_check``, the
