@@ -496,9 +496,11 @@ def pmc_probe(case, steps):
 
 def prep_probe(case, steps):
     """Short run for kernel traces of prepare_interp: "apes" fits the VKDE of
-    one APES half-ensemble (Cauchy kernel, d=16, n=3200, the seeded mixture)
-    and "d4" the largest served refit (Student-t nu=3, d=4, n=20000, seeded
-    correlated Gaussian), each `steps` times on one handle after a warm-up."""
+    one APES half-ensemble (Cauchy kernel, d=16, n=3200, the seeded mixture),
+    "d4" the largest served refit (Student-t nu=3, d=4, n=20000, seeded
+    correlated Gaussian) and "c5" the large VKDE (Student-t nu=5, d=64,
+    n=50000, Gaussian target of condition number 1e4, two refinement steps),
+    each `steps` times on one handle after a warm-up."""
     import torch
 
     import numcosmo_amd as na
@@ -507,10 +509,15 @@ def prep_probe(case, steps):
     if case == "apes":
         tgt, X, m2lnp, _ = synth.workload("mixture", 3200, 10, 16, 3)
         kern = na.StatsDistKernelST.new(16, 1.0)
+    elif case == "c5":
+        tgt, X, m2lnp, _ = synth.workload("cond", 50000, 10, 64, 5)
+        kern = na.StatsDistKernelST.new(64, 5.0)
     else:
         tgt, X, m2lnp, _ = synth.workload("gaussian", 20000, 10, 4, 4)
         kern = na.StatsDistKernelST.new(4, 3.0)
     sd = na.StatsDistVKDE.new(kern, na.StatsDistCV.NONE)
+    if case == "c5":
+        sd.set_nnls_refine(2)
     sd.add_obs_array(X)
     sd.prepare_interp(m2lnp)
     for r in range(steps):
@@ -529,7 +536,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-c2", action="store_true", help="skip the d=8, n=4000 banana VKDE eval")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--prep-probe", choices=["apes", "d4"], default=None,
+    ap.add_argument("--prep-probe", choices=["apes", "d4", "c5"], default=None,
                     help="short run for kernel traces of prepare_interp")
     ap.add_argument("--pmc-probe", choices=["d16", "c2"], default=None,
                     help="short run for rocprofv3 --pmc passes")
