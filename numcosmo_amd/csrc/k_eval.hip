@@ -29,13 +29,29 @@ k_eval_f64 (EvalArgs a)
   const int i1    = min (a.n, i0 + chunk);
   const int pp    = (p < a.m) ? p : (a.m - 1);
   const int d     = a.d;
-  double x[D];
+  // VKDE at D = 64: the point is read again for every kernel (from L1/L2)
+  // instead of being held beside the differences, which left one wave per
+  // SIMD (1.5x as long at n = 8000, m = 20000; at D = 32, and in the FP32
+  // rung, the re-reads cost more than the occupancy gains).  KDE keeps its
+  // whitened point in registers.
+  constexpr bool XREG = (KIND == NCKDE_KDE) || (D < 64);
+  const double *xrow  = a.X + (size_t) pp * d;
+  double x[XREG ? D : 1];
+  auto xv = [&] (int k) -> double {
+    if constexpr (XREG)
+      return x[k];
+    else
+      return (k < d) ? xrow[k] : 0.0;
+  };
 
+  if constexpr (XREG)
+  {
 #pragma unroll
-  for (int k = 0; k < D; ++k)
-    x[k] = (k < d) ? a.X[(size_t) pp * d + k] : 0.0;
+    for (int k = 0; k < D; ++k)
+      x[k] = (k < d) ? xrow[k] : 0.0;
+  }
 
-  if (KIND == NCKDE_KDE)
+  if constexpr (KIND == NCKDE_KDE)
   {
     // v = U_g^{-T} x (gsl_blas_dtrsv Upper/Trans, kde.c:648-652)
     double v[D];
@@ -78,7 +94,7 @@ k_eval_f64 (EvalArgs a)
 
 #pragma unroll
       for (int k = 0; k < D; ++k)
-        delta[k] = x[k] - th[k];
+        delta[k] = xv (k) - th[k];
 
 #pragma unroll
       for (int r = 0; r < D; ++r)
@@ -97,7 +113,7 @@ k_eval_f64 (EvalArgs a)
 #pragma unroll
       for (int k = 0; k < D; ++k)
       {
-        const double t = th[k] - x[k];
+        const double t = th[k] - xv (k);
 
         chi2 = fma (t, t, chi2);
       }
